@@ -1,0 +1,221 @@
+/*
+ * ldf_hip.h -- C ABI of the MI355X (gfx950) hash join / groupby / hash-partition path.
+ *
+ * This is the boundary the GPU task variants of legate-dataframe would bind: plain pointers
+ * and sizes, no C++ types, no exceptions crossing it.  Every entry point enqueues its work on
+ * the caller's HIP stream; the only host synchronisation is reading an output row count.
+ *
+ * Reference interfaces replaced (paths relative to the legate-dataframe source tree):
+ *   ldf_join            <- JoinTask<bool>::gpu_variant (cpp/src/join.cu:187-244), which calls
+ *                          cudf_join + cudf_join_and_gather (cpp/src/join.cu:32-155); argument
+ *                          order follows cpp/src/join.cpp:410-420.
+ *   ldf_groupby         <- GroupByAggregationTask::gpu_variant (cpp/src/groupby_aggregation.cu:90-167);
+ *                          argument packing follows cpp/src/groupby_aggregation.cpp:211-223.
+ *   ldf_hash_partition  <- the local step of task::repartition_by_hash
+ *                          (cpp/src/core/repartition_by_hash.cu:296-310: cudf::hash_partition with
+ *                          HASH_MURMUR3 and DEFAULT_HASH_SEED, then split at the offsets).
+ *   ldf_murmur3_hash    <- the row hash used by that partition step (libcudf MurmurHash3_x86_32
+ *                          per column, combined with hash_combine).
+ *
+ * Column layout in HBM (Arrow-compatible, see ldf_column_view):
+ *   fixed width : `data` = length * width bytes, little endian.
+ *   validity    : LSB-first bitmap, bit i of byte i/8 set = row i valid; NULL = no nulls and the
+ *                 column is not nullable.
+ *   large_utf8  : `offsets` = length+1 int64 (first is 0), `data` = chars.
+ *
+ * Status codes: 0 = success, < 0 = error (see LDF_E_*); ldf_last_error() returns a
+ * thread-local message for the last failing call on this thread.
+ */
+#ifndef LDF_HIP_H
+#define LDF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Type ids: the numeric values of arrow::Type::type for the supported subset
+ * (the reference describes columns by Arrow type, cpp/include/legate_dataframe/core/column.hpp). */
+enum {
+  LDF_TYPE_BOOL         = 1, /* one byte per row, 0 or 1 (Legate's bool layout) */
+  LDF_TYPE_UINT8        = 2,
+  LDF_TYPE_INT8         = 3,
+  LDF_TYPE_UINT16       = 4,
+  LDF_TYPE_INT16        = 5,
+  LDF_TYPE_UINT32       = 6,
+  LDF_TYPE_INT32        = 7,
+  LDF_TYPE_UINT64       = 8,
+  LDF_TYPE_INT64        = 9,
+  LDF_TYPE_FLOAT        = 11,
+  LDF_TYPE_DOUBLE       = 12,
+  LDF_TYPE_DATE32       = 16,
+  LDF_TYPE_TIMESTAMP    = 18, /* int64 storage; the unit is the caller's metadata */
+  LDF_TYPE_LARGE_STRING = 34
+};
+
+/* JoinType / BroadcastInput values are ABI (cpp/include/legate_dataframe/join.hpp:25-26). */
+enum { LDF_JOIN_INNER = 0, LDF_JOIN_LEFT = 1, LDF_JOIN_FULL = 2, LDF_JOIN_SEMI = 3, LDF_JOIN_ANTI = 4 };
+
+/* Aggregation kinds (cpp/src/groupby_aggregation.cu:46-88 lists the same set). */
+enum {
+  LDF_AGG_SUM            = 0,
+  LDF_AGG_PRODUCT        = 1,
+  LDF_AGG_MIN            = 2,
+  LDF_AGG_MAX            = 3,
+  LDF_AGG_COUNT          = 4,
+  LDF_AGG_MEAN           = 5,
+  LDF_AGG_VARIANCE       = 6,
+  LDF_AGG_STDDEV         = 7,
+  LDF_AGG_APPROX_MEDIAN  = 8,
+  LDF_AGG_COUNT_DISTINCT = 9,
+  LDF_AGG_COUNT_ALL      = 10
+};
+
+/* arrow::compute::CountOptions::CountMode values. */
+enum { LDF_COUNT_ONLY_VALID = 0, LDF_COUNT_ONLY_NULL = 1, LDF_COUNT_ALL = 2 };
+
+enum {
+  LDF_OK             = 0,
+  LDF_E_INVALID      = -1, /* invalid argument (std::invalid_argument in the reference) */
+  LDF_E_OOM          = -2, /* the allocator returned NULL */
+  LDF_E_HIP          = -3, /* a HIP runtime call failed */
+  LDF_E_UNSUPPORTED  = -4, /* valid request this build does not implement */
+  LDF_E_INTERNAL     = -5
+};
+
+typedef struct ldf_column_view {
+  int32_t type_id;
+  int32_t reserved;
+  int64_t length;
+  const void* data;          /* values, or chars for LDF_TYPE_LARGE_STRING */
+  const uint8_t* validity;   /* LSB bitmap or NULL */
+  const int64_t* offsets;    /* large_utf8 only */
+  int64_t null_count;        /* -1 = unknown */
+} ldf_column_view;
+
+typedef struct ldf_table_view {
+  int32_t ncols;
+  const ldf_column_view* cols;
+} ldf_table_view;
+
+/* Output column: every buffer comes from the caller's allocator and is owned by the caller. */
+typedef struct ldf_column {
+  int32_t type_id;
+  int32_t reserved;
+  int64_t length;
+  void* data;
+  uint8_t* validity; /* NULL when the output column is not nullable */
+  int64_t* offsets;  /* large_utf8 only */
+  int64_t null_count;
+  int64_t data_bytes;     /* bytes allocated for `data` (chars for strings) */
+  int64_t validity_bytes; /* bytes allocated for `validity` */
+} ldf_column;
+
+typedef struct ldf_table_out {
+  int32_t ncols;     /* set by the caller to the capacity of `cols`, by the callee to the count */
+  ldf_column* cols;  /* caller-provided array */
+} ldf_table_out;
+
+/* Allocation callbacks (the role TaskMemoryResource plays in the reference,
+ * cpp/src/core/allocator.cu:27-81).  `stream` is the call's HIP stream; allocations are
+ * stream ordered.  A NULL ldf_allocator* selects the library default (hipMallocAsync on the
+ * stream from the device's default memory pool). */
+typedef void* (*ldf_alloc_fn)(void* ctx, size_t bytes, void* stream);
+typedef void (*ldf_free_fn)(void* ctx, void* ptr, size_t bytes, void* stream);
+typedef struct ldf_allocator {
+  ldf_alloc_fn alloc;
+  ldf_free_fn free;
+  void* ctx;
+} ldf_allocator;
+
+typedef struct ldf_agg {
+  int32_t in_col;     /* input column index (ignored for COUNT_ALL) */
+  int32_t kind;       /* LDF_AGG_* */
+  int32_t count_mode; /* LDF_COUNT_* for COUNT / COUNT_DISTINCT */
+  int32_t reserved;
+} ldf_agg;
+
+/* Join of lhs (probe side) with rhs (build side), as cudf::hash_join(rhs keys) probed with the
+ * lhs keys in cpp/src/join.cu:32-57, gathered as cpp/src/join.cu:110-155.  The output holds
+ * the lhs_out columns then the rhs_out columns (cpp/src/join.cpp:398-400).  An output column
+ * is nullable when its input column is, and additionally every rhs column for LEFT and every
+ * column for FULL (cpp/src/join.cpp:348-383; unmatched rows are null,
+ * cpp/src/join.cu:79-101).  SEMI/ANTI take no rhs output columns.  `nulls_equal` != 0 makes
+ * a null key equal a null key (cpp/include/legate_dataframe/join.hpp:85).
+ * `out->cols` must have room for n_lhs_out + n_rhs_out columns. */
+int ldf_join(const ldf_table_view* lhs,
+             const ldf_table_view* rhs,
+             const int32_t* lhs_keys,
+             const int32_t* rhs_keys,
+             int32_t nkeys,
+             int32_t join_type,
+             int32_t nulls_equal,
+             const int32_t* lhs_out,
+             int32_t n_lhs_out,
+             const int32_t* rhs_out,
+             int32_t n_rhs_out,
+             const ldf_allocator* alloc,
+             void* stream,
+             ldf_table_out* out);
+
+/* Groupby-aggregate (cudf::groupby with null_policy::INCLUDE on the keys,
+ * cpp/src/groupby_aggregation.cu:143; Arrow hash_<kind> result types and null rules,
+ * cpp/src/groupby_aggregation.cpp:41-132).  Output = the key columns then one column per
+ * aggregation in order.  Group order is unspecified.  `out->cols` must have room for
+ * nkeys + naggs columns. */
+int ldf_groupby(const ldf_table_view* table,
+                const int32_t* keys,
+                int32_t nkeys,
+                const ldf_agg* aggs,
+                int32_t naggs,
+                const ldf_allocator* alloc,
+                void* stream,
+                ldf_table_out* out);
+
+/* Local hash partition (cudf::hash_partition(table, cols, nparts, HASH_MURMUR3, seed),
+ * cpp/src/core/repartition_by_hash.cu:296-302): row r goes to partition
+ * murmur3_row_hash(r) mod nparts.  `out` receives every column of `table`, rows grouped by
+ * partition (rows keep their input order inside a partition); `offsets` (host memory,
+ * nparts+1 entries) receives the partition start rows, offsets[nparts] = row count. */
+int ldf_hash_partition(const ldf_table_view* table,
+                       const int32_t* cols,
+                       int32_t ncols,
+                       int32_t nparts,
+                       uint32_t seed,
+                       const ldf_allocator* alloc,
+                       void* stream,
+                       ldf_table_out* out,
+                       int64_t* offsets);
+
+/* Row hash of the selected columns into a device uint32 array of table rows (the value the
+ * partition step reduces mod nparts). */
+int ldf_murmur3_hash(const ldf_table_view* table,
+                     const int32_t* cols,
+                     int32_t ncols,
+                     uint32_t seed,
+                     uint32_t* out_hashes,
+                     void* stream);
+
+/* Legate-layout adapters (cpp/src/core/null_mask.cu:38-114, cpp/src/core/ranges.cu:33-138):
+ * one byte per row validity <-> LSB bitmap, Rect<1>{lo,hi} (inclusive, global) string ranges
+ * <-> local int64 offsets. */
+int ldf_bools_to_bitmask(const uint8_t* bools, int64_t n, uint8_t* bitmask, int64_t* null_count_dev,
+                         void* stream);
+int ldf_bitmask_to_bools(const uint8_t* bitmask, int64_t n, uint8_t* bools, void* stream);
+int ldf_ranges_to_offsets(const int64_t* ranges /* 2 per row */, int64_t n, int64_t* offsets,
+                          void* stream);
+int ldf_offsets_to_ranges(const int64_t* offsets, int64_t n, int64_t global_base, int64_t* ranges,
+                          void* stream);
+
+/* Frees every buffer of an output table through `alloc` (NULL = library default). */
+void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream);
+
+const char* ldf_last_error(void);
+const char* ldf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LDF_HIP_H */

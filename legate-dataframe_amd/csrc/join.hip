@@ -1,0 +1,444 @@
+// Hash join (INNER / LEFT / FULL / SEMI / ANTI) of lhs (probe side) with rhs (build side).
+//
+// Reference: JoinTask::gpu_variant (cpp/src/join.cu:187-244) -> cudf_join builds
+// cudf::hash_join on the rhs keys and probes with the lhs keys (cpp/src/join.cu:32-74); the
+// gather maps then gather lhs_out / rhs_out with NULLIFY on the side that may be unmatched
+// (cpp/src/join.cu:79-155).  Null keys: nulls_equal -> null matches null, otherwise a row with
+// a null key never matches (the CPU variant emulates this with coalesce + mask key columns,
+// cpp/src/join.cpp:117-209).  Floating-point keys compare bitwise, as Arrow's hash join does.
+//
+// Layout of the build side in HBM (a multimap in CSR form, so a key repeated 10^5 times in
+// the build table costs one slot, not a 10^5-long probe chain):
+//   table   : open addressing, linear probing, 16-byte slots {key, cnt, start};
+//             key = the 64-bit key bits (single fixed-width key column) or
+//             (fingerprint32 << 32 | representative build row) for composite / string keys,
+//             installed with one 64-bit atomicCAS.  Two extra slots hold the key whose bits
+//             equal the empty marker and the null key.
+//   rows    : build row ids grouped by slot; slot s owns rows[start, start + cnt).
+// Probe: k_join_count finds each lhs row's slot once and records {start, cnt} (8 B/row) so
+// the emit pass streams instead of re-probing; per-block output counts are scanned; the emit
+// kernel expands every block's rows over all its threads (binary search over the block's row
+// offsets in LDS), so one hot key with 10^5 matches does not serialise on one thread.
+#include <hip/hip_runtime.h>
+
+#include "ldf_common.hpp"
+#include "ldf_device.hpp"
+
+namespace ldf {
+
+struct alignas(16) JSlot {
+  unsigned long long key;
+  uint32_t cnt;
+  uint32_t start;
+};
+constexpr unsigned long long kEmpty = ~0ull;
+
+constexpr int kJBlock = 256;
+constexpr int kJItems = 8;
+constexpr int kJRows  = kJBlock * kJItems;  // rows per block in the probe passes
+
+__global__ void k_jslot_fill(JSlot* __restrict__ tab, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    tab[i].key   = kEmpty;
+    tab[i].cnt   = 0;
+    tab[i].start = 0;
+  }
+}
+
+// ---- slot lookup ------------------------------------------------------------------------
+template <bool GENERIC>
+__device__ __forceinline__ int64_t build_find_or_insert(const DKeys& bk, int64_t r, JSlot* tab, uint64_t mask,
+                                                        uint64_t cap, bool nulls_equal)
+{
+  if (!GENERIC) {
+    const DCol& c = bk.c[0];
+    if (!valid_at(c.valid, r)) return nulls_equal ? (int64_t)cap + 1 : -1;
+    unsigned long long k = load_bits(c.data, c.width, r);
+    if (k == kEmpty) return (int64_t)cap;
+    uint64_t i = mix64(k) & mask;
+    while (true) {
+      unsigned long long cur = atomic_load_relaxed(&tab[i].key);
+      if (cur == k) return (int64_t)i;
+      if (cur == kEmpty) {
+        unsigned long long old = atomicCAS(&tab[i].key, kEmpty, k);
+        if (old == kEmpty || old == k) return (int64_t)i;
+      }
+      i = (i + 1) & mask;
+    }
+  } else {
+    if (!nulls_equal && any_key_null(bk, r)) return -1;
+    uint64_t h                = row_hash64(bk, r);
+    uint32_t fp               = (uint32_t)(h >> 32);
+    unsigned long long packed = ((unsigned long long)fp << 32) | (uint32_t)r;
+    uint64_t i                = h & mask;
+    while (true) {
+      unsigned long long cur = atomic_load_relaxed(&tab[i].key);
+      if (cur == kEmpty) {
+        unsigned long long old = atomicCAS(&tab[i].key, kEmpty, packed);
+        if (old == kEmpty) return (int64_t)i;
+        cur = old;
+      }
+      if ((uint32_t)(cur >> 32) == fp && rows_equal(bk, r, bk, (int64_t)(uint32_t)cur)) return (int64_t)i;
+      i = (i + 1) & mask;
+    }
+  }
+}
+
+// Returns the slot of the lhs row's key, or -1 when no build row has that key.
+template <bool GENERIC>
+__device__ __forceinline__ int64_t probe_find(const DKeys& pk, const DKeys& bk, int64_t r, const JSlot* tab,
+                                              uint64_t mask, uint64_t cap, bool nulls_equal)
+{
+  if (!GENERIC) {
+    const DCol& c = pk.c[0];
+    if (!valid_at(c.valid, r)) return nulls_equal ? (int64_t)cap + 1 : -1;
+    unsigned long long k = load_bits(c.data, c.width, r);
+    if (k == kEmpty) return (int64_t)cap;
+    uint64_t i = mix64(k) & mask;
+    while (true) {
+      unsigned long long cur = tab[i].key;
+      if (cur == k) return (int64_t)i;
+      if (cur == kEmpty) return -1;
+      i = (i + 1) & mask;
+    }
+  } else {
+    if (!nulls_equal && any_key_null(pk, r)) return -1;
+    uint64_t h  = row_hash64(pk, r);
+    uint32_t fp = (uint32_t)(h >> 32);
+    uint64_t i  = h & mask;
+    while (true) {
+      unsigned long long cur = tab[i].key;
+      if (cur == kEmpty) return -1;
+      if ((uint32_t)(cur >> 32) == fp && rows_equal(pk, r, bk, (int64_t)(uint32_t)cur)) return (int64_t)i;
+      i = (i + 1) & mask;
+    }
+  }
+}
+
+// ---- build ------------------------------------------------------------------------------
+template <bool GENERIC>
+__global__ __launch_bounds__(256) void k_join_build(DKeys bk, int64_t nb, JSlot* __restrict__ tab, uint64_t mask,
+                                                    uint64_t cap, bool nulls_equal, int32_t* __restrict__ slot_of_row)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nb; r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = build_find_or_insert<GENERIC>(bk, r, tab, mask, cap, nulls_equal);
+    if (s >= 0) atomicAdd(&tab[s].cnt, 1u);
+    slot_of_row[r] = (int32_t)s;
+  }
+}
+
+__global__ void k_jslot_counts(const JSlot* __restrict__ tab, int64_t n, uint32_t* __restrict__ cnt)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    cnt[i] = tab[i].cnt;
+}
+
+__global__ void k_jslot_starts(JSlot* __restrict__ tab, int64_t n, const uint64_t* __restrict__ start)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    tab[i].start = (uint32_t)start[i];
+}
+
+__global__ void k_join_group_rows(const int32_t* __restrict__ slot_of_row, int64_t nb, const JSlot* __restrict__ tab,
+                                  uint32_t* __restrict__ fill, uint32_t* __restrict__ rows)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nb; r += (int64_t)gridDim.x * blockDim.x) {
+    int32_t s = slot_of_row[r];
+    if (s < 0) continue;
+    uint32_t pos = tab[s].start + atomicAdd(&fill[s], 1u);
+    rows[pos]    = (uint32_t)r;
+  }
+}
+
+// ---- probe ------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t out_rows_for(uint32_t cnt, int jt)
+{
+  switch (jt) {
+    case LDF_JOIN_INNER: return cnt;
+    case LDF_JOIN_LEFT:
+    case LDF_JOIN_FULL: return cnt > 0 ? cnt : 1;
+    case LDF_JOIN_SEMI: return cnt > 0 ? 1 : 0;
+    default: return cnt > 0 ? 0 : 1;  // ANTI
+  }
+}
+
+// info[r] = start | (cnt << 32) of the lhs row's build rows (cnt = 0: no match)
+template <bool GENERIC>
+__global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int64_t np, const JSlot* __restrict__ tab,
+                                                        uint64_t mask, uint64_t cap, bool nulls_equal, int jt,
+                                                        uint64_t* __restrict__ info, uint8_t* __restrict__ matched,
+                                                        uint64_t* __restrict__ block_tot)
+{
+  __shared__ uint64_t smem[17];
+  int64_t base = (int64_t)blockIdx.x * kJRows;
+  uint64_t sum = 0;
+#pragma unroll 2
+  for (int j = 0; j < kJItems; ++j) {
+    int64_t r = base + j * kJBlock + threadIdx.x;
+    if (r < np) {
+      int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, cap, nulls_equal);
+      uint32_t cnt = 0, st = 0;
+      if (s >= 0) {
+        cnt = tab[s].cnt;
+        st  = tab[s].start;
+        if (matched != nullptr && cnt > 0) matched[s] = 1;
+      }
+      info[r] = (uint64_t)st | ((uint64_t)cnt << 32);
+      sum += out_rows_for(cnt, jt);
+    }
+  }
+  uint64_t t = block_reduce_sum(sum, smem);
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kJBlock) void k_join_emit(const uint64_t* __restrict__ info, int64_t np, int jt,
+                                                       const uint32_t* __restrict__ rows,
+                                                       const uint64_t* __restrict__ block_off, int32_t* __restrict__ lmap,
+                                                       int32_t* __restrict__ rmap)
+{
+  __shared__ uint64_t off[kJRows + 1];
+  __shared__ uint32_t st[kJRows];
+  __shared__ uint32_t mc[kJRows];
+  __shared__ uint64_t smem[17];
+  int64_t base = (int64_t)blockIdx.x * kJRows;
+  int nrows    = (int)((np - base) < kJRows ? (np - base) : kJRows);
+
+  // each thread owns kJItems consecutive rows for the block scan
+  uint64_t v[kJItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kJItems; ++j) {
+    int li = threadIdx.x * kJItems + j;
+    uint64_t o = 0;
+    if (li < nrows) {
+      uint64_t x = info[base + li];
+      uint32_t c = (uint32_t)(x >> 32);
+      st[li]     = (uint32_t)x;
+      mc[li]     = c;
+      o          = out_rows_for(c, jt);
+    }
+    v[j] = o;
+    s += o;
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_scan(s, smem, &total);
+#pragma unroll
+  for (int j = 0; j < kJItems; ++j) {
+    int li = threadIdx.x * kJItems + j;
+    if (li < nrows) off[li] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 0) off[nrows] = total;
+  __syncthreads();
+
+  const uint64_t gbase = block_off[blockIdx.x];
+  for (uint64_t q = threadIdx.x; q < total; q += blockDim.x) {
+    // last row li with off[li] <= q (rows with zero output have equal offsets; take the last)
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    uint64_t k      = q - off[lo];
+    int32_t lrow    = (int32_t)(base + lo);
+    uint64_t gq     = gbase + q;
+    lmap[gq]        = lrow;
+    if (rmap != nullptr) rmap[gq] = (mc[lo] == 0) ? -1 : (int32_t)rows[st[lo] + k];
+  }
+}
+
+// FULL join: build rows whose key no lhs row matched (or whose key is null with !nulls_equal).
+__global__ __launch_bounds__(kJBlock) void k_join_unmatched(const int32_t* __restrict__ slot_of_row, int64_t nb,
+                                                            const uint8_t* __restrict__ matched,
+                                                            const uint64_t* __restrict__ block_off, uint64_t base_out,
+                                                            uint64_t* __restrict__ block_tot, int32_t* __restrict__ lmap,
+                                                            int32_t* __restrict__ rmap)
+{
+  __shared__ uint64_t smem[17];
+  int64_t base = (int64_t)blockIdx.x * kJRows;
+  bool f[kJItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kJItems; ++j) {
+    int64_t r = base + (int64_t)threadIdx.x * kJItems + j;
+    bool u    = false;
+    if (r < nb) {
+      int32_t sl = slot_of_row[r];
+      u          = (sl < 0) || (matched[sl] == 0);
+    }
+    f[j] = u;
+    s += u ? 1 : 0;
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_scan(s, smem, &total);
+  if (block_off == nullptr) {
+    if (threadIdx.x == 0) block_tot[blockIdx.x] = total;
+    return;
+  }
+  uint64_t pos = base_out + block_off[blockIdx.x] + run;
+#pragma unroll
+  for (int j = 0; j < kJItems; ++j) {
+    if (f[j]) {
+      lmap[pos] = -1;
+      rmap[pos] = (int32_t)(base + (int64_t)threadIdx.x * kJItems + j);
+      ++pos;
+    }
+  }
+}
+
+// ---- host -------------------------------------------------------------------------------
+static DKeys make_keys(const ldf_table_view* t, const int32_t* idx, int32_t nkeys, const char* side)
+{
+  DKeys k;
+  k.n = nkeys;
+  for (int i = 0; i < nkeys; ++i) {
+    if (idx[i] < 0 || idx[i] >= t->ncols) throw Error(LDF_E_INVALID, std::string(side) + " key index out of range");
+    k.c[i] = to_dcol(t->cols[idx[i]]);
+  }
+  return k;
+}
+
+static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const int32_t* lhs_keys,
+                     const int32_t* rhs_keys, int32_t nkeys, int32_t jt, bool nulls_equal, const int32_t* lhs_out,
+                     int32_t n_lhs_out, const int32_t* rhs_out, int32_t n_rhs_out, Alloc& a, ldf_table_out* out)
+{
+  hipStream_t s = a.stream();
+  LDF_EXPECTS(lhs != nullptr && rhs != nullptr && out != nullptr && out->cols != nullptr, "join: NULL argument");
+  LDF_EXPECTS(nkeys >= 1 && nkeys <= kMaxKeyCols, "join: between 1 and 8 key columns");
+  LDF_EXPECTS(jt >= LDF_JOIN_INNER && jt <= LDF_JOIN_ANTI, "Unknown JoinType");
+  if ((jt == LDF_JOIN_SEMI || jt == LDF_JOIN_ANTI) && n_rhs_out > 0)
+    throw Error(LDF_E_INVALID, "rhs must have no output columns for SEMI and ANTI join");
+  LDF_EXPECTS(out->ncols >= n_lhs_out + n_rhs_out, "join: output table has too few column slots");
+  int64_t np = lhs->ncols > 0 ? lhs->cols[0].length : 0;
+  int64_t nb = rhs->ncols > 0 ? rhs->cols[0].length : 0;
+  for (int i = 0; i < lhs->ncols; ++i) validate_view(lhs->cols[i], np, "join lhs");
+  for (int i = 0; i < rhs->ncols; ++i) validate_view(rhs->cols[i], nb, "join rhs");
+  LDF_EXPECTS(np < INT32_MAX && nb < INT32_MAX, "join: each side is limited to 2^31-1 rows per call");
+  for (int i = 0; i < n_lhs_out; ++i) LDF_EXPECTS(lhs_out[i] >= 0 && lhs_out[i] < lhs->ncols, "join: lhs_out index out of range");
+  for (int i = 0; i < n_rhs_out; ++i) LDF_EXPECTS(rhs_out[i] >= 0 && rhs_out[i] < rhs->ncols, "join: rhs_out index out of range");
+  DKeys pk = make_keys(lhs, lhs_keys, nkeys, "lhs");
+  DKeys bk = make_keys(rhs, rhs_keys, nkeys, "rhs");
+  for (int i = 0; i < nkeys; ++i)
+    if (pk.c[i].type != bk.c[i].type) throw Error(LDF_E_INVALID, "join: lhs and rhs key column types differ");
+  const bool generic = !(nkeys == 1 && pk.c[0].width > 0);
+
+  uint64_t cap  = next_pow2((uint64_t)(nb > 32 ? 2 * nb : 64));
+  uint64_t mask = cap - 1;
+  int64_t nslot = (int64_t)cap + 2;
+  Buf<JSlot> tab(a, nslot);
+  hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
+  LDF_HIP_TRY(hipGetLastError());
+
+  // build
+  Buf<int32_t> slot_of_row(a, nb > 0 ? nb : 1);
+  Buf<uint32_t> rows(a, nb > 0 ? nb : 1);
+  if (nb > 0) {
+    if (generic)
+      hipLaunchKernelGGL(k_join_build<true>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, cap,
+                         nulls_equal, slot_of_row.get());
+    else
+      hipLaunchKernelGGL(k_join_build<false>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, cap,
+                         nulls_equal, slot_of_row.get());
+    LDF_HIP_TRY(hipGetLastError());
+    {
+      Buf<uint32_t> cnt(a, nslot);
+      Buf<uint64_t> start(a, nslot + 1);
+      hipLaunchKernelGGL(k_jslot_counts, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot, cnt.get());
+      LDF_HIP_TRY(hipGetLastError());
+      exclusive_scan_u32_to_u64(cnt.get(), start.get(), nslot, a);
+      hipLaunchKernelGGL(k_jslot_starts, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot, start.get());
+      LDF_HIP_TRY(hipGetLastError());
+      fill_u32(cnt.get(), 0, nslot, s);
+      hipLaunchKernelGGL(k_join_group_rows, dim3(grid_for(nb, 256)), dim3(256), 0, s, slot_of_row.get(), nb, tab.get(),
+                         cnt.get(), rows.get());
+      LDF_HIP_TRY(hipGetLastError());
+    }
+  }
+
+  // probe: count
+  int64_t nblk = (np + kJRows - 1) / kJRows;
+  if (nblk < 1) nblk = 1;
+  Buf<uint64_t> info(a, np > 0 ? np : 1);
+  Buf<uint64_t> btot(a, nblk);
+  Buf<uint64_t> boff(a, nblk + 1);
+  Buf<uint8_t> matched;
+  if (jt == LDF_JOIN_FULL) {
+    matched = Buf<uint8_t>(a, nslot);
+    fill_u8(matched.get(), 0, nslot, s);
+  }
+  if (generic)
+    hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
+                       nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
+  else
+    hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
+                       nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
+  LDF_HIP_TRY(hipGetLastError());
+  exclusive_scan_u64(btot.get(), boff.get(), nblk, a);
+
+  int64_t ublk = (nb + kJRows - 1) / kJRows;
+  if (ublk < 1) ublk = 1;
+  Buf<uint64_t> utot, uoff;
+  if (jt == LDF_JOIN_FULL) {
+    utot = Buf<uint64_t>(a, ublk);
+    uoff = Buf<uint64_t>(a, ublk + 1);
+    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, matched.get(),
+                       (const uint64_t*)nullptr, (uint64_t)0, utot.get(), (int32_t*)nullptr, (int32_t*)nullptr);
+    LDF_HIP_TRY(hipGetLastError());
+    exclusive_scan_u64(utot.get(), uoff.get(), ublk, a);
+  }
+  uint64_t n1 = read_u64(boff.get() + nblk, s);
+  uint64_t n2 = (jt == LDF_JOIN_FULL) ? read_u64(uoff.get() + ublk, s) : 0;
+  int64_t n_out = (int64_t)(n1 + n2);
+
+  // emit gather maps
+  const bool with_rmap = !(jt == LDF_JOIN_SEMI || jt == LDF_JOIN_ANTI);
+  Buf<int32_t> lmap(a, n_out > 0 ? n_out : 1);
+  Buf<int32_t> rmap;
+  if (with_rmap) rmap = Buf<int32_t>(a, n_out > 0 ? n_out : 1);
+  if (n1 > 0) {
+    hipLaunchKernelGGL(k_join_emit, dim3(nblk), dim3(kJBlock), 0, s, (const uint64_t*)info.get(), np, (int)jt,
+                       (const uint32_t*)rows.get(), (const uint64_t*)boff.get(), lmap.get(), rmap.get());
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  if (n2 > 0) {
+    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, matched.get(),
+                       (const uint64_t*)uoff.get(), n1, utot.get(), lmap.get(), rmap.get());
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  // release the probe-side temporaries before the output columns are allocated
+  info.reset();
+  tab.reset();
+
+  // gather output columns
+  out->ncols = n_lhs_out + n_rhs_out;
+  int done   = 0;
+  try {
+    for (int i = 0; i < n_lhs_out; ++i, ++done)
+      gather_column(lhs->cols[lhs_out[i]], lmap.get(), n_out, jt == LDF_JOIN_FULL, a, &out->cols[done]);
+    for (int i = 0; i < n_rhs_out; ++i, ++done)
+      gather_column(rhs->cols[rhs_out[i]], rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
+                    &out->cols[done]);
+  } catch (...) {
+    for (int i = 0; i < done; ++i) free_column(&out->cols[i], a);
+    out->ncols = 0;
+    throw;
+  }
+}
+
+}  // namespace ldf
+
+using namespace ldf;
+
+extern "C" int ldf_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const int32_t* lhs_keys,
+                        const int32_t* rhs_keys, int32_t nkeys, int32_t join_type, int32_t nulls_equal,
+                        const int32_t* lhs_out, int32_t n_lhs_out, const int32_t* rhs_out, int32_t n_rhs_out,
+                        const ldf_allocator* alloc, void* stream, ldf_table_out* out)
+{
+  return guarded([&] {
+    Alloc a(alloc, (hipStream_t)stream);
+    run_join(lhs, rhs, lhs_keys, rhs_keys, nkeys, join_type, nulls_equal != 0, lhs_out, n_lhs_out, rhs_out, n_rhs_out, a,
+             out);
+  });
+}

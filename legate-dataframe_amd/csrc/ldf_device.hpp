@@ -1,0 +1,251 @@
+// Device helpers shared by the join, groupby and hash-partition kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ldf_common.hpp"
+
+namespace ldf {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ bool valid_at(const uint8_t* v, int64_t i)
+{
+  return v == nullptr || ((v[i >> 3] >> (i & 7)) & 1);
+}
+
+// Raw value bits of a fixed-width column, zero-extended to 64 bits.  Keys compare by these bits:
+// Arrow's hash join and grouper compare floating-point keys bitwise (0.0 != -0.0, NaN == NaN).
+__device__ __forceinline__ uint64_t load_bits(const void* data, int width, int64_t row)
+{
+  switch (width) {
+    case 1: return static_cast<const uint8_t*>(data)[row];
+    case 2: return static_cast<const uint16_t*>(data)[row];
+    case 4: return static_cast<const uint32_t*>(data)[row];
+    default: return static_cast<const uint64_t*>(data)[row];
+  }
+}
+
+// Sign-extended integer value of a fixed-width integer column.
+__device__ __forceinline__ int64_t load_signed(const void* data, int width, int64_t row)
+{
+  switch (width) {
+    case 1: return static_cast<const int8_t*>(data)[row];
+    case 2: return static_cast<const int16_t*>(data)[row];
+    case 4: return static_cast<const int32_t*>(data)[row];
+    default: return static_cast<const int64_t*>(data)[row];
+  }
+}
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__device__ __forceinline__ uint32_t fmix32(uint32_t h)
+{
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__device__ __forceinline__ uint32_t murmur3_mix_block(uint32_t h, uint32_t k)
+{
+  k *= 0xcc9e2d51u;
+  k = rotl32(k, 15);
+  k *= 0x1b873593u;
+  h ^= k;
+  h = rotl32(h, 13);
+  return h * 5u + 0xe6546b64u;
+}
+
+// MurmurHash3_x86_32 (Austin Appleby, public domain) of a fixed-width value's bytes.
+__device__ __forceinline__ uint32_t murmur3_fixed(uint64_t bits, int width, uint32_t seed)
+{
+  uint32_t h = seed;
+  if (width >= 4) {
+    h = murmur3_mix_block(h, (uint32_t)bits);
+    if (width == 8) h = murmur3_mix_block(h, (uint32_t)(bits >> 32));
+  } else {
+    uint32_t k1 = (uint32_t)bits;  // tail bytes, little endian
+    k1 *= 0xcc9e2d51u;
+    k1 = rotl32(k1, 15);
+    k1 *= 0x1b873593u;
+    h ^= k1;
+  }
+  h ^= (uint32_t)width;
+  return fmix32(h);
+}
+
+// MurmurHash3_x86_32 over an arbitrary byte string.
+__device__ inline uint32_t murmur3_bytes(const uint8_t* p, int64_t len, uint32_t seed)
+{
+  uint32_t h       = seed;
+  int64_t nblocks  = len / 4;
+  for (int64_t i = 0; i < nblocks; ++i) {
+    uint32_t k = (uint32_t)p[4 * i] | ((uint32_t)p[4 * i + 1] << 8) | ((uint32_t)p[4 * i + 2] << 16) |
+                 ((uint32_t)p[4 * i + 3] << 24);
+    h = murmur3_mix_block(h, k);
+  }
+  const uint8_t* tail = p + nblocks * 4;
+  uint32_t k1         = 0;
+  switch (len & 3) {
+    case 3: k1 ^= (uint32_t)tail[2] << 16; [[fallthrough]];
+    case 2: k1 ^= (uint32_t)tail[1] << 8; [[fallthrough]];
+    case 1:
+      k1 ^= tail[0];
+      k1 *= 0xcc9e2d51u;
+      k1 = rotl32(k1, 15);
+      k1 *= 0x1b873593u;
+      h ^= k1;
+  }
+  h ^= (uint32_t)len;
+  return fmix32(h);
+}
+
+// boost-style combine used by libcudf's row hasher.
+__device__ __forceinline__ uint32_t hash_combine32(uint32_t lhs, uint32_t rhs)
+{
+  return lhs ^ (rhs + 0x9e3779b9u + (lhs << 6) + (lhs >> 2));
+}
+
+// 64-bit finaliser (MurmurHash3 fmix64) for the internal hash tables.
+__device__ __forceinline__ uint64_t mix64(uint64_t k)
+{
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ inline uint64_t string_hash64(const uint8_t* p, int64_t len)
+{
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)len;
+  int64_t i  = 0;
+  for (; i + 8 <= len; i += 8) {
+    uint64_t w = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) w |= (uint64_t)p[i + b] << (8 * b);
+    h = mix64(h ^ w) + 0x632be59bd9b4e019ull;
+  }
+  uint64_t w = 0;
+  for (int b = 0; i + b < len; ++b) w |= (uint64_t)p[i + b] << (8 * b);
+  return mix64(h ^ w ^ 0x5851f42d4c957f2dull);
+}
+
+// Internal 64-bit hash of one key element (not the partition hash).
+__device__ __forceinline__ uint64_t element_hash64(const DCol& c, int64_t row)
+{
+  if (!valid_at(c.valid, row)) return 0x2545f4914f6cdd1dull;
+  if (c.width == 0) {
+    const int64_t* off = c.offsets;
+    int64_t b = off[row], e = off[row + 1];
+    return string_hash64(static_cast<const uint8_t*>(c.data) + b, e - b);
+  }
+  return mix64(load_bits(c.data, c.width, row) + 0x9e3779b97f4a7c15ull);
+}
+
+__device__ __forceinline__ uint64_t row_hash64(const DKeys& k, int64_t row)
+{
+  uint64_t h = 0;
+  for (int i = 0; i < k.n; ++i) {
+    uint64_t e = element_hash64(k.c[i], row);
+    h          = mix64(h ^ (e + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
+  }
+  return h;
+}
+
+__device__ inline bool strings_equal(const DCol& a, int64_t ra, const DCol& b, int64_t rb)
+{
+  int64_t ab = a.offsets[ra], ae = a.offsets[ra + 1];
+  int64_t bb = b.offsets[rb], be = b.offsets[rb + 1];
+  if (ae - ab != be - bb) return false;
+  const uint8_t* pa = static_cast<const uint8_t*>(a.data) + ab;
+  const uint8_t* pb = static_cast<const uint8_t*>(b.data) + bb;
+  for (int64_t i = 0; i < ae - ab; ++i)
+    if (pa[i] != pb[i]) return false;
+  return true;
+}
+
+// Key-tuple equality; a null equals a null (only reached when nulls compare equal).
+__device__ inline bool rows_equal(const DKeys& a, int64_t ra, const DKeys& b, int64_t rb)
+{
+  for (int i = 0; i < a.n; ++i) {
+    bool va = valid_at(a.c[i].valid, ra), vb = valid_at(b.c[i].valid, rb);
+    if (va != vb) return false;
+    if (!va) continue;
+    if (a.c[i].width == 0) {
+      if (!strings_equal(a.c[i], ra, b.c[i], rb)) return false;
+    } else if (load_bits(a.c[i].data, a.c[i].width, ra) != load_bits(b.c[i].data, b.c[i].width, rb)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool any_key_null(const DKeys& k, int64_t row)
+{
+  for (int i = 0; i < k.n; ++i)
+    if (!valid_at(k.c[i].valid, row)) return true;
+  return false;
+}
+
+// Relaxed device-scope atomic load (bypasses the non-coherent vector L1).
+template <typename T>
+__device__ __forceinline__ T atomic_load_relaxed(const T* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------------------------------
+// block-wide scans / reductions (blockDim.x a multiple of 64, <= 1024)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_inclusive_scan(uint64_t v)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan across the block; returns the thread's exclusive prefix, *total = block sum.
+// `smem` needs 17 uint64 entries.
+__device__ inline uint64_t block_exclusive_scan(uint64_t v, uint64_t* smem, uint64_t* total)
+{
+  const int lane  = threadIdx.x & (kWave - 1);
+  const int wave  = threadIdx.x / kWave;
+  const int waves = blockDim.x / kWave;
+  uint64_t inc    = wave_inclusive_scan(v);
+  if (lane == kWave - 1) smem[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t run = 0;
+    for (int w = 0; w < waves; ++w) {
+      uint64_t t = smem[w];
+      smem[w]    = run;
+      run += t;
+    }
+    smem[16] = run;
+  }
+  __syncthreads();
+  uint64_t res = smem[wave] + inc - v;
+  *total       = smem[16];
+  __syncthreads();
+  return res;
+}
+
+__device__ inline uint64_t block_reduce_sum(uint64_t v, uint64_t* smem)
+{
+  uint64_t total;
+  (void)block_exclusive_scan(v, smem, &total);
+  return total;
+}
+
+}  // namespace ldf

@@ -1,0 +1,340 @@
+// Local murmur3 hash partition: one GPU's share of task::repartition_by_hash.
+//
+// Reference: cpp/src/core/repartition_by_hash.cu:296-310 calls
+//   cudf::hash_partition(table, cols, nparts, HASH_MURMUR3, DEFAULT_HASH_SEED)
+// and splits the result at the returned offsets.  The row hash restated here is libcudf's:
+// h = seed; for each hashed column h = hash_combine(h, MurmurHash3_x86_32(value, seed)),
+// a null element hashing to 0xFFFFFFFF; the partition is h mod nparts.
+//
+// Kernels (all HBM-streaming, no MFMA):
+//   k_hp_hist    block b counts the partition ids of its contiguous row range in LDS and
+//                writes counts[p * nblk + b] (partition-major, so one exclusive scan of the
+//                whole array yields every (partition, block) write cursor);
+//   k_hp_scatter the same rows again, 8192 at a time: each wave ranks its 512 rows by
+//                partition with wave64 ballots (stable: rank = rows of the same partition
+//                earlier in the wave's chunk), the ranks are combined across the 16 waves in
+//                LDS, rows are staged in partition order in LDS and then written out as
+//                contiguous runs, so rows keep their input order inside a partition.
+#include <hip/hip_runtime.h>
+
+#include "ldf_common.hpp"
+#include "ldf_device.hpp"
+
+namespace ldf {
+
+constexpr int kHpThreads = 1024;                   // 16 waves
+constexpr int kHpWaves   = kHpThreads / kWave;
+constexpr int kHpSub     = 8192;                   // rows staged in LDS per step
+constexpr int kHpPerWave = kHpSub / kHpWaves;      // 512 contiguous rows per wave
+constexpr int kHpGroups  = kHpPerWave / kWave;     // 8 groups of 64
+constexpr int kHpMaxParts = 2048;
+
+struct HashCols {
+  DCol c[kMaxKeyCols];
+  int32_t n;
+  uint32_t seed;
+};
+
+__device__ __forceinline__ uint32_t partition_row_hash(const HashCols& hc, int64_t row)
+{
+  uint32_t h = hc.seed;
+  for (int i = 0; i < hc.n; ++i) {
+    const DCol& c = hc.c[i];
+    uint32_t e;
+    if (!valid_at(c.valid, row)) {
+      e = 0xFFFFFFFFu;
+    } else if (c.width == 0) {
+      int64_t b = c.offsets[row], en = c.offsets[row + 1];
+      e = murmur3_bytes(static_cast<const uint8_t*>(c.data) + b, en - b, hc.seed);
+    } else {
+      e = murmur3_fixed(load_bits(c.data, c.width, row), c.width, hc.seed);
+    }
+    h = hash_combine32(h, e);
+  }
+  return h;
+}
+
+__global__ void k_hp_hash(HashCols hc, int64_t n, uint32_t* __restrict__ out)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = partition_row_hash(hc, i);
+}
+
+__global__ __launch_bounds__(kHpThreads) void k_hp_hist(HashCols hc, int64_t n, int64_t rows_per_block, uint32_t nparts,
+                                                        uint32_t* __restrict__ counts)
+{
+  extern __shared__ uint32_t hist[];
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  int64_t begin = blockIdx.x * rows_per_block;
+  int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
+  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
+    uint32_t p = partition_row_hash(hc, r) % nparts;
+    atomicAdd(&hist[p], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) counts[(int64_t)p * gridDim.x + blockIdx.x] = hist[p];
+}
+
+struct ScatterCols {
+  const void* src[kMaxKeyCols];
+  void* dst[kMaxKeyCols];
+  int32_t width[kMaxKeyCols];
+  int32_t n;
+};
+
+// LDS layout (dynamic): cursor u64[P] | wtab u16[W][P] | total u32[P] | start u32[P] |
+//                       stage_row u16[kHpSub] | stage_part u16[kHpSub]
+__global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t n, int64_t rows_per_block, uint32_t nparts,
+                                                           int pbits, const uint64_t* __restrict__ offsets,
+                                                           ScatterCols sc, int64_t* __restrict__ perm)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* cursor   = reinterpret_cast<uint64_t*>(smem_raw);
+  uint16_t* wtab     = reinterpret_cast<uint16_t*>(cursor + nparts);
+  uint32_t* total    = reinterpret_cast<uint32_t*>(wtab + kHpWaves * nparts + ((kHpWaves * nparts) & 1));
+  uint32_t* start    = total + nparts;
+  uint16_t* stage_r  = reinterpret_cast<uint16_t*>(start + nparts);
+  uint16_t* stage_p  = stage_r + kHpSub;
+  __shared__ uint64_t scan_smem[17];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) cursor[p] = offsets[(int64_t)p * gridDim.x + blockIdx.x];
+
+  int64_t begin = blockIdx.x * rows_per_block;
+  int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
+
+  for (int64_t sub = begin; sub < end; sub += kHpSub) {
+    const int sub_n = (int)((end - sub) < kHpSub ? (end - sub) : kHpSub);
+    for (uint32_t i = threadIdx.x; i < kHpWaves * nparts; i += blockDim.x) wtab[i] = 0;
+    __syncthreads();
+
+    // 1. per-wave stable ranks over the wave's contiguous 512-row chunk
+    uint16_t my_part[kHpGroups];
+    uint16_t my_rank[kHpGroups];
+#pragma unroll
+    for (int g = 0; g < kHpGroups; ++g) {
+      const int local = wave * kHpPerWave + g * kWave + lane;
+      const bool active = local < sub_n;
+      uint32_t p = 0;
+      if (active) p = partition_row_hash(hc, sub + local) % nparts;
+      uint64_t same = __ballot(active);
+      for (int b = 0; b < pbits; ++b) {
+        uint64_t bb = __ballot(active && ((p >> b) & 1));
+        same &= ((p >> b) & 1) ? bb : ~bb;
+      }
+      uint32_t rank_in_group = (uint32_t)__popcll(same & lanes_below);
+      uint32_t cnt           = (uint32_t)__popcll(same);
+      int leader             = __ffsll((unsigned long long)same) - 1;
+      uint32_t old           = 0;
+      if (active && leader == lane) {
+        old = wtab[wave * nparts + p];
+        wtab[wave * nparts + p] = (uint16_t)(old + cnt);
+      }
+      old = __shfl(old, leader < 0 ? 0 : leader, kWave);
+      my_part[g] = (uint16_t)p;
+      my_rank[g] = (uint16_t)(old + rank_in_group);
+    }
+    __syncthreads();
+
+    // 2. prefix over waves for every partition; per-partition totals of this step
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) {
+      uint32_t run = 0;
+      for (int w = 0; w < kHpWaves; ++w) {
+        uint32_t c = wtab[w * nparts + p];
+        wtab[w * nparts + p] = (uint16_t)run;
+        run += c;
+      }
+      total[p] = run;
+    }
+    __syncthreads();
+
+    // 3. exclusive scan of the totals -> start of each partition in the staging area
+    {
+      uint32_t per = (nparts + blockDim.x - 1) / blockDim.x;
+      uint32_t p0  = threadIdx.x * per;
+      uint64_t s   = 0;
+      for (uint32_t k = 0; k < per; ++k)
+        if (p0 + k < nparts) s += total[p0 + k];
+      uint64_t tot;
+      uint64_t run = block_exclusive_scan(s, scan_smem, &tot);
+      for (uint32_t k = 0; k < per; ++k)
+        if (p0 + k < nparts) {
+          start[p0 + k] = (uint32_t)run;
+          run += total[p0 + k];
+        }
+    }
+    __syncthreads();
+
+    // 4. stage row ids in partition order
+#pragma unroll
+    for (int g = 0; g < kHpGroups; ++g) {
+      const int local = wave * kHpPerWave + g * kWave + lane;
+      if (local < sub_n) {
+        uint32_t p   = my_part[g];
+        uint32_t pos = start[p] + wtab[wave * nparts + p] + my_rank[g];
+        stage_r[pos] = (uint16_t)local;
+        stage_p[pos] = (uint16_t)p;
+      }
+    }
+    __syncthreads();
+
+    // 5. write contiguous runs
+    for (int s = threadIdx.x; s < sub_n; s += blockDim.x) {
+      uint32_t p     = stage_p[s];
+      int64_t src    = sub + stage_r[s];
+      int64_t dst    = (int64_t)cursor[p] + (s - (int64_t)start[p]);
+      for (int c = 0; c < sc.n; ++c) {
+        switch (sc.width[c]) {
+          case 1: static_cast<uint8_t*>(sc.dst[c])[dst] = static_cast<const uint8_t*>(sc.src[c])[src]; break;
+          case 2: static_cast<uint16_t*>(sc.dst[c])[dst] = static_cast<const uint16_t*>(sc.src[c])[src]; break;
+          case 4: static_cast<uint32_t*>(sc.dst[c])[dst] = static_cast<const uint32_t*>(sc.src[c])[src]; break;
+          default: static_cast<uint64_t*>(sc.dst[c])[dst] = static_cast<const uint64_t*>(sc.src[c])[src]; break;
+        }
+      }
+      if (perm != nullptr) perm[dst] = src;
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) cursor[p] += total[p];
+    __syncthreads();
+  }
+}
+
+__global__ void k_hp_pick_offsets(const uint64_t* __restrict__ scan, uint32_t nparts, uint32_t nblk,
+                                  uint64_t* __restrict__ out)
+{
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= nparts; p += gridDim.x * blockDim.x)
+    out[p] = scan[(int64_t)p * nblk];
+}
+
+__global__ void k_i64_to_i32(const int64_t* __restrict__ in, int64_t n, int32_t* __restrict__ out)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (int32_t)in[i];
+}
+
+static HashCols make_hash_cols(const ldf_table_view* t, const int32_t* cols, int32_t ncols, uint32_t seed)
+{
+  LDF_EXPECTS(ncols >= 1 && ncols <= kMaxKeyCols, "hash partition: between 1 and 8 hashed columns");
+  HashCols hc;
+  hc.n    = ncols;
+  hc.seed = seed;
+  for (int i = 0; i < ncols; ++i) {
+    LDF_EXPECTS(cols[i] >= 0 && cols[i] < t->ncols, "hash partition: column index out of range");
+    hc.c[i] = to_dcol(t->cols[cols[i]]);
+  }
+  return hc;
+}
+
+static size_t scatter_lds_bytes(uint32_t nparts)
+{
+  size_t b = nparts * 8 + (size_t)kHpWaves * nparts * 2 + (((size_t)kHpWaves * nparts) & 1) * 2 + nparts * 8 + kHpSub * 4;
+  return (b + 15) & ~size_t(15);
+}
+
+}  // namespace ldf
+
+using namespace ldf;
+
+extern "C" int ldf_murmur3_hash(const ldf_table_view* table, const int32_t* cols, int32_t ncols, uint32_t seed,
+                                uint32_t* out_hashes, void* stream)
+{
+  return guarded([&] {
+    LDF_EXPECTS(table != nullptr && out_hashes != nullptr, "NULL argument");
+    int64_t n = table->ncols > 0 ? table->cols[0].length : 0;
+    for (int i = 0; i < table->ncols; ++i) validate_view(table->cols[i], n, "murmur3_hash");
+    HashCols hc = make_hash_cols(table, cols, ncols, seed);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_hp_hash, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, hc, n, out_hashes);
+    LDF_HIP_TRY(hipGetLastError());
+  });
+}
+
+extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* cols, int32_t ncols, int32_t nparts,
+                                  uint32_t seed, const ldf_allocator* alloc, void* stream, ldf_table_out* out,
+                                  int64_t* offsets)
+{
+  return guarded([&] {
+    LDF_EXPECTS(table != nullptr && out != nullptr && offsets != nullptr, "NULL argument");
+    LDF_EXPECTS(nparts >= 1, "hash partition: nparts must be >= 1");
+    if (nparts > kHpMaxParts)
+      throw Error(LDF_E_UNSUPPORTED, "hash partition: at most 2048 partitions are supported");
+    LDF_EXPECTS(out->ncols >= table->ncols, "hash partition: output table has too few column slots");
+    hipStream_t s = (hipStream_t)stream;
+    Alloc a(alloc, s);
+    int64_t n = table->ncols > 0 ? table->cols[0].length : 0;
+    for (int i = 0; i < table->ncols; ++i) validate_view(table->cols[i], n, "hash_partition");
+    HashCols hc = make_hash_cols(table, cols, ncols, seed);
+    LDF_EXPECTS(n < (int64_t(1) << 40), "hash partition: too many rows");
+
+    int pbits = 0;
+    while ((1u << pbits) < (uint32_t)nparts) ++pbits;
+
+    // every column fixed width and non-nullable -> rows move in the scatter kernel itself
+    bool direct = table->ncols <= kMaxKeyCols;
+    for (int i = 0; i < table->ncols; ++i)
+      if (type_width(table->cols[i].type_id) == 0 || table->cols[i].validity != nullptr) direct = false;
+
+    out->ncols = table->ncols;
+    if (n == 0) {
+      for (int i = 0; i < table->ncols; ++i) {
+        alloc_column(table->cols[i].type_id, 0, table->cols[i].validity != nullptr, 0, a, &out->cols[i]);
+        if (type_width(table->cols[i].type_id) == 0) fill_u64((uint64_t*)out->cols[i].offsets, 0, 1, s);
+      }
+      for (int p = 0; p <= nparts; ++p) offsets[p] = 0;
+      return;
+    }
+
+    int64_t nblk = (n + kHpSub - 1) / kHpSub;
+    if (nblk > 1536) nblk = 1536;
+    int64_t rpb = (n + nblk - 1) / nblk;
+    rpb         = (rpb + kHpSub - 1) / kHpSub * kHpSub;
+    nblk        = (n + rpb - 1) / rpb;
+
+    Buf<uint32_t> counts(a, (size_t)nparts * nblk);
+    Buf<uint64_t> scan(a, (size_t)nparts * nblk + 1);
+    hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), nparts * sizeof(uint32_t), s, hc, n, rpb,
+                       (uint32_t)nparts, counts.get());
+    LDF_HIP_TRY(hipGetLastError());
+    exclusive_scan_u32_to_u64(counts.get(), scan.get(), (int64_t)nparts * nblk, a);
+
+    ScatterCols sc;
+    sc.n = 0;
+    Buf<int64_t> perm;
+    if (direct) {
+      for (int i = 0; i < table->ncols; ++i) {
+        alloc_column(table->cols[i].type_id, n, false, 0, a, &out->cols[i]);
+        sc.src[i]   = table->cols[i].data;
+        sc.dst[i]   = out->cols[i].data;
+        sc.width[i] = type_width(table->cols[i].type_id);
+      }
+      sc.n = table->ncols;
+    } else {
+      LDF_EXPECTS(n < (int64_t(1) << 31), "hash partition: nullable or string columns limited to 2^31-1 rows");
+      perm = Buf<int64_t>(a, n);
+    }
+    size_t lds = scatter_lds_bytes((uint32_t)nparts);
+    hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, pbits,
+                       (const uint64_t*)scan.get(), sc, perm.get());
+    LDF_HIP_TRY(hipGetLastError());
+
+    if (!direct) {
+      Buf<int32_t> map(a, n);
+      hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, perm.get(), n, map.get());
+      LDF_HIP_TRY(hipGetLastError());
+      for (int i = 0; i < table->ncols; ++i) gather_column(table->cols[i], map.get(), n, false, a, &out->cols[i]);
+    }
+
+    Buf<uint64_t> offs(a, (size_t)nparts + 1);
+    hipLaunchKernelGGL(k_hp_pick_offsets, dim3(grid_for(nparts + 1, 256)), dim3(256), 0, s, (const uint64_t*)scan.get(),
+                       (uint32_t)nparts, (uint32_t)nblk, offs.get());
+    LDF_HIP_TRY(hipGetLastError());
+    std::vector<uint64_t> h(nparts + 1);
+    LDF_HIP_TRY(hipMemcpyAsync(h.data(), offs.get(), (nparts + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    for (int p = 0; p <= nparts; ++p) offsets[p] = (int64_t)h[p];
+  });
+}

@@ -1,0 +1,339 @@
+"""ctypes binding of the C ABI in include/ldf_hip.h (libldf_hip.so).
+
+Device memory is plumbing here: buffers are torch tensors on the current HIP device, the
+library allocates its outputs and temporaries through callbacks that hand out torch tensors
+(so the caching allocator serves them), and calls run on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+_PKG = Path(__file__).resolve().parent.parent
+_LIB_PATH = _PKG / "libldf_hip.so"
+
+# type ids (arrow::Type::type values)
+BOOL, UINT8, INT8, UINT16, INT16, UINT32, INT32, UINT64, INT64 = 1, 2, 3, 4, 5, 6, 7, 8, 9
+FLOAT, DOUBLE, DATE32, TIMESTAMP, LARGE_STRING = 11, 12, 16, 18, 34
+
+WIDTH = {BOOL: 1, UINT8: 1, INT8: 1, UINT16: 2, INT16: 2, UINT32: 4, INT32: 4, UINT64: 8, INT64: 8,
+         FLOAT: 4, DOUBLE: 8, DATE32: 4, TIMESTAMP: 8, LARGE_STRING: 0}
+
+AGG = {"sum": 0, "product": 1, "min": 2, "max": 3, "count": 4, "mean": 5, "variance": 6,
+       "stddev": 7, "approximate_median": 8, "count_distinct": 9, "count_all": 10}
+COUNT_MODE = {"only_valid": 0, "only_null": 1, "all": 2}
+
+EXPORTED = [
+    "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash",
+    "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
+    "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
+]
+
+
+class ColumnView(ctypes.Structure):
+    _fields_ = [("type_id", ctypes.c_int32), ("reserved", ctypes.c_int32), ("length", ctypes.c_int64),
+                ("data", ctypes.c_void_p), ("validity", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
+                ("null_count", ctypes.c_int64)]
+
+
+class TableView(ctypes.Structure):
+    _fields_ = [("ncols", ctypes.c_int32), ("cols", ctypes.POINTER(ColumnView))]
+
+
+class Column(ctypes.Structure):
+    _fields_ = [("type_id", ctypes.c_int32), ("reserved", ctypes.c_int32), ("length", ctypes.c_int64),
+                ("data", ctypes.c_void_p), ("validity", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
+                ("null_count", ctypes.c_int64), ("data_bytes", ctypes.c_int64), ("validity_bytes", ctypes.c_int64)]
+
+
+class TableOut(ctypes.Structure):
+    _fields_ = [("ncols", ctypes.c_int32), ("cols", ctypes.POINTER(Column))]
+
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+
+
+class Allocator(ctypes.Structure):
+    _fields_ = [("alloc", ALLOC_FN), ("free", FREE_FN), ("ctx", ctypes.c_void_p)]
+
+
+class Agg(ctypes.Structure):
+    _fields_ = [("in_col", ctypes.c_int32), ("kind", ctypes.c_int32), ("count_mode", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class LdfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load(require: bool = True):
+    """Loads libldf_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            if not require:
+                return None
+            raise ImportError(f"{_LIB_PATH} is missing: run `python legate-dataframe_amd/build.py` first")
+        L = ctypes.CDLL(str(_LIB_PATH))
+        P = ctypes.POINTER
+        L.ldf_join.argtypes = [P(TableView), P(TableView), P(ctypes.c_int32), P(ctypes.c_int32), ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_int32),
+                               ctypes.c_int32, P(Allocator), ctypes.c_void_p, P(TableOut)]
+        L.ldf_groupby.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, P(Agg), ctypes.c_int32,
+                                  P(Allocator), ctypes.c_void_p, P(TableOut)]
+        L.ldf_hash_partition.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_uint32, P(Allocator), ctypes.c_void_p, P(TableOut),
+                                         P(ctypes.c_int64)]
+        L.ldf_murmur3_hash.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_uint32,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.ldf_bools_to_bitmask.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+        L.ldf_bitmask_to_bools.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.ldf_ranges_to_offsets.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.ldf_offsets_to_ranges.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        L.ldf_free_table.argtypes = [P(TableOut), P(Allocator), ctypes.c_void_p]
+        L.ldf_last_error.restype = ctypes.c_char_p
+        L.ldf_version.restype = ctypes.c_char_p
+        for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_bools_to_bitmask",
+                  "ldf_bitmask_to_bools", "ldf_ranges_to_offsets", "ldf_offsets_to_ranges"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().ldf_last_error().decode()
+        if rc == -1:
+            raise ValueError(msg)
+        raise LdfError(rc, msg)
+
+
+class TorchAllocator:
+    """ldf_allocator whose buffers are torch uint8 tensors (kept alive until freed / adopted)."""
+
+    def __init__(self, device=None):
+        import torch
+
+        self.torch = torch
+        self.device = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.live: Dict[int, "torch.Tensor"] = {}
+        self._alloc = ALLOC_FN(self._do_alloc)
+        self._free = FREE_FN(self._do_free)
+        self.struct = Allocator(self._alloc, self._free, None)
+
+    def _do_alloc(self, ctx, nbytes, stream):
+        try:
+            t = self.torch.empty(int(nbytes), dtype=self.torch.uint8, device=self.device)
+        except Exception:  # out of memory -> NULL -> LDF_E_OOM
+            return None
+        p = t.data_ptr()
+        self.live[p] = t
+        return p
+
+    def _do_free(self, ctx, ptr, nbytes, stream):
+        self.live.pop(ptr, None)
+
+    def adopt(self, ptr: int) -> "torch.Tensor":
+        """Takes ownership of a buffer the library returned (removes it from the live set)."""
+        return self.live.pop(ptr)
+
+
+def stream_ptr() -> int:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+def make_table_view(cols: Sequence[ColumnView]):
+    arr = (ColumnView * max(1, len(cols)))(*cols)
+    return TableView(len(cols), arr), arr
+
+
+def int32_array(xs: Sequence[int]):
+    xs = list(xs)
+    return (ctypes.c_int32 * max(1, len(xs)))(*xs)
+
+
+# ---------------------------------------------------------------------------------------------
+# device columns as torch tensors (used by the bindings' tests and the benchmark)
+# ---------------------------------------------------------------------------------------------
+class DeviceColumn:
+    """A column resident in HBM: torch uint8/typed tensors for data, validity and offsets."""
+
+    def __init__(self, type_id: int, length: int, data, validity=None, offsets=None, null_count: int = -1):
+        self.type_id = type_id
+        self.length = length
+        self.data = data
+        self.validity = validity
+        self.offsets = offsets
+        self.null_count = null_count
+
+    def view(self) -> ColumnView:
+        return ColumnView(self.type_id, 0, self.length,
+                          self.data.data_ptr() if self.data is not None else None,
+                          self.validity.data_ptr() if self.validity is not None else None,
+                          self.offsets.data_ptr() if self.offsets is not None else None,
+                          self.null_count)
+
+    @property
+    def nullable(self) -> bool:
+        return self.validity is not None
+
+
+_NP_OF = {BOOL: "uint8", UINT8: "uint8", INT8: "int8", UINT16: "uint16", INT16: "int16", UINT32: "uint32",
+          INT32: "int32", UINT64: "uint64", INT64: "int64", FLOAT: "float32", DOUBLE: "float64",
+          DATE32: "int32", TIMESTAMP: "int64"}
+
+
+def np_dtype(type_id: int):
+    import numpy as np
+
+    return np.dtype(_NP_OF[type_id])
+
+
+def column_from_numpy(values, type_id: int, valid=None, device=None) -> DeviceColumn:
+    """Fixed-width column from a numpy array (+ optional bool validity array)."""
+    import numpy as np
+    import torch
+
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    v = np.ascontiguousarray(np.asarray(values).astype(np_dtype(type_id), copy=False))
+    data = torch.from_numpy(v.view(np.uint8).reshape(-1).copy()).to(dev)
+    validity = None
+    nc = 0
+    if valid is not None:
+        valid = np.asarray(valid, dtype=bool)
+        bits = np.packbits(valid, bitorder="little")
+        pad = (-len(bits)) % 8
+        bits = np.concatenate([bits, np.zeros(pad + 8, np.uint8)])
+        validity = torch.from_numpy(bits).to(dev)
+        nc = int((~valid).sum())
+    return DeviceColumn(type_id, len(v), data, validity, None, nc)
+
+
+def column_from_strings(strs, valid=None, device=None) -> DeviceColumn:
+    import numpy as np
+    import torch
+
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    enc = [("" if s is None else s).encode() for s in strs]
+    offs = np.zeros(len(enc) + 1, np.int64)
+    offs[1:] = np.cumsum([len(e) for e in enc])
+    chars = np.frombuffer(b"".join(enc) + b"\0", np.uint8).copy()
+    if valid is None and any(s is None for s in strs):
+        valid = [s is not None for s in strs]
+    col = DeviceColumn(LARGE_STRING, len(enc), torch.from_numpy(chars).to(dev), None,
+                       torch.from_numpy(offs).to(dev), 0)
+    if valid is not None:
+        valid = np.asarray(valid, dtype=bool)
+        bits = np.packbits(valid, bitorder="little")
+        col.validity = torch.from_numpy(np.concatenate([bits, np.zeros(8, np.uint8)])).to(dev)
+        col.null_count = int((~valid).sum())
+    return col
+
+
+def column_to_numpy(c: DeviceColumn):
+    """(values, valid) on the host; strings come back as a list of str (None for null)."""
+    import numpy as np
+
+    n = c.length
+    valid = None
+    if c.validity is not None:
+        bits = c.validity.cpu().numpy()
+        valid = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+    if c.type_id == LARGE_STRING:
+        offs = c.offsets.cpu().numpy()
+        chars = c.data.cpu().numpy().tobytes()
+        vals = [chars[offs[i]:offs[i + 1]].decode() for i in range(n)]
+        if valid is not None:
+            vals = [v if ok else None for v, ok in zip(vals, valid)]
+        return vals, valid
+    raw = c.data.cpu().numpy()[: n * WIDTH[c.type_id]]
+    return raw.view(np_dtype(c.type_id)).copy(), valid
+
+
+def _adopt_columns(alloc: TorchAllocator, out: TableOut) -> List[DeviceColumn]:
+    import torch
+
+    cols = []
+    for i in range(out.ncols):
+        c = out.cols[i]
+        data = alloc.adopt(c.data) if c.data else None
+        validity = alloc.adopt(c.validity) if c.validity else None
+        offsets = None
+        if c.type_id == LARGE_STRING and c.offsets:
+            offsets = alloc.adopt(c.offsets)[: (c.length + 1) * 8].view(torch.int64)
+        cols.append(DeviceColumn(c.type_id, c.length, data, validity, offsets, c.null_count))
+    return cols
+
+
+def join(lhs: Sequence[DeviceColumn], rhs: Sequence[DeviceColumn], lhs_keys, rhs_keys, join_type: int,
+         nulls_equal: bool = True, lhs_out=None, rhs_out=None, alloc: Optional[TorchAllocator] = None):
+    L = load()
+    alloc = alloc or TorchAllocator()
+    lhs_out = list(range(len(lhs))) if lhs_out is None else list(lhs_out)
+    if rhs_out is None:
+        rhs_out = [] if join_type in (3, 4) else list(range(len(rhs)))
+    lv, _la = make_table_view([c.view() for c in lhs])
+    rv, _ra = make_table_view([c.view() for c in rhs])
+    ncols = len(lhs_out) + len(rhs_out)
+    cols = (Column * max(1, ncols))()
+    out = TableOut(ncols, cols)
+    check(L.ldf_join(ctypes.byref(lv), ctypes.byref(rv), int32_array(lhs_keys), int32_array(rhs_keys), len(lhs_keys),
+                     join_type, int(bool(nulls_equal)), int32_array(lhs_out), len(lhs_out), int32_array(rhs_out),
+                     len(rhs_out), ctypes.byref(alloc.struct), stream_ptr(), ctypes.byref(out)))
+    return _adopt_columns(alloc, out)
+
+
+def groupby(table: Sequence[DeviceColumn], keys, aggs, alloc: Optional[TorchAllocator] = None):
+    """aggs: list of (in_col, kind_name, count_mode_name)."""
+    L = load()
+    alloc = alloc or TorchAllocator()
+    tv, _ta = make_table_view([c.view() for c in table])
+    arr = (Agg * max(1, len(aggs)))(*[Agg(-1 if c is None else c, AGG[k], COUNT_MODE[m or "only_valid"], 0)
+                                      for c, k, m in aggs])
+    ncols = len(keys) + len(aggs)
+    cols = (Column * ncols)()
+    out = TableOut(ncols, cols)
+    check(L.ldf_groupby(ctypes.byref(tv), int32_array(keys), len(keys), arr, len(aggs), ctypes.byref(alloc.struct),
+                        stream_ptr(), ctypes.byref(out)))
+    return _adopt_columns(alloc, out)
+
+
+def hash_partition(table: Sequence[DeviceColumn], cols, nparts: int, seed: int = 0,
+                   alloc: Optional[TorchAllocator] = None):
+    L = load()
+    alloc = alloc or TorchAllocator()
+    tv, _ta = make_table_view([c.view() for c in table])
+    outc = (Column * max(1, len(table)))()
+    out = TableOut(len(table), outc)
+    offs = (ctypes.c_int64 * (nparts + 1))()
+    check(L.ldf_hash_partition(ctypes.byref(tv), int32_array(cols), len(cols), nparts, seed,
+                               ctypes.byref(alloc.struct), stream_ptr(), ctypes.byref(out), offs))
+    return _adopt_columns(alloc, out), list(offs)
+
+
+def murmur3_hash(table: Sequence[DeviceColumn], cols, seed: int = 0):
+    import torch
+
+    L = load()
+    n = table[0].length if table else 0
+    tv, _ta = make_table_view([c.view() for c in table])
+    out = torch.empty(max(1, n), dtype=torch.int32, device=table[0].data.device)
+    check(L.ldf_murmur3_hash(ctypes.byref(tv), int32_array(cols), len(cols), seed, out.data_ptr(), stream_ptr()))
+    return out[:n]
