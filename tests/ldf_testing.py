@@ -1,0 +1,115 @@
+"""Helpers shared by the tests: golden fixtures, oracle columns, comparisons."""
+
+from __future__ import annotations
+
+import json
+import math
+from functools import lru_cache
+from pathlib import Path
+
+import numpy as np
+import pyarrow as pa
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+ARROW_TYPE = {"int8": pa.int8(), "int16": pa.int16(), "int32": pa.int32(), "int64": pa.int64(),
+              "uint8": pa.uint8(), "uint16": pa.uint16(), "uint32": pa.uint32(), "uint64": pa.uint64(),
+              "float": pa.float32(), "double": pa.float64(), "bool": pa.bool_(), "string": pa.string(),
+              "large_string": pa.large_string()}
+TYPE_ID = {"bool": 1, "uint8": 2, "int8": 3, "uint16": 4, "int16": 5, "uint32": 6, "int32": 7, "uint64": 8,
+           "int64": 9, "float": 11, "double": 12, "string": 34, "large_string": 34}
+
+
+@lru_cache(maxsize=None)
+def join_fixture():
+    return json.loads((GOLDEN / "join_cases.json").read_text())
+
+
+@lru_cache(maxsize=None)
+def groupby_fixture():
+    return json.loads((GOLDEN / "groupby_cases.json").read_text())
+
+
+def _vals(c):
+    out = []
+    for v in c["values"]:
+        out.append(float("nan") if v == "nan" else v)
+    return out
+
+
+def fixture_table(tj) -> pa.Table:
+    arrays = [pa.array(_vals(c), type=ARROW_TYPE[c["type"]]) for c in tj["columns"]]
+    return pa.Table.from_arrays(arrays, names=tj["names"])
+
+
+def ocol(arr: pa.Array):
+    """pyarrow array -> oracle column"""
+    from oracle import OCol
+
+    arr = arr.combine_chunks() if isinstance(arr, pa.ChunkedArray) else arr
+    tname = str(arr.type)
+    tid = TYPE_ID[tname]
+    valid = None if arr.null_count == 0 else np.asarray(arr.is_valid())
+    if tid == 34:
+        return OCol(tid, arr.to_pylist(), valid)
+    if tid == 1:
+        vals = np.asarray(arr.fill_null(False), dtype=np.uint8)
+    else:
+        vals = arr.fill_null(0).to_numpy(zero_copy_only=False) if arr.null_count else arr.to_numpy(zero_copy_only=False)
+    return OCol(tid, vals, valid)
+
+
+def rows_of(t: pa.Table):
+    """Rows as tuples with NaN made comparable, for sorted multiset comparison."""
+    cols = [t.column(i).to_pylist() for i in range(t.num_columns)]
+    out = []
+    for r in zip(*cols):
+        out.append(tuple(("<nan>" if isinstance(v, float) and math.isnan(v) else v) for v in r))
+    return out
+
+
+def sort_key(row):
+    return tuple((0, "") if v is None else (1, str(type(v)), v) for v in row)
+
+
+def sorted_rows(t: pa.Table):
+    return sorted(rows_of(t), key=sort_key)
+
+
+def join_via_oracle(lhs: pa.Table, rhs: pa.Table, lk, rk, how, nulls_equal, lhs_out=None, rhs_out=None) -> pa.Table:
+    """Materialises the oracle's pair list into the reference's output table."""
+    import oracle
+
+    lm, rm = oracle.join([ocol(lhs.column(k)) for k in lk], [ocol(rhs.column(k)) for k in rk], how, nulls_equal)
+    lhs_out = lhs.column_names if lhs_out is None else lhs_out
+    rhs_out = ([] if how in ("semi", "anti") else rhs.column_names) if rhs_out is None else rhs_out
+    cols, names = [], []
+    for side, table, m, outs in (("l", lhs, lm, lhs_out), ("r", rhs, rm, rhs_out)):
+        idx = pa.array(np.where(m < 0, 0, m).astype(np.int64))
+        mask = pa.array(m < 0)
+        for c in outs:
+            src = table.column(c).combine_chunks()
+            if table.num_rows == 0:
+                g = pa.nulls(len(m), type=src.type)
+            else:
+                g = src.take(idx)
+                g = pa.compute.if_else(mask, pa.scalar(None, type=src.type), g)
+            cols.append(g)
+            names.append(c)
+    return pa.Table.from_arrays(cols, names=names)
+
+
+def assert_groupby_equal(got: pa.Table, exp: pa.Table, keys, rel=1e-12):
+    """Groups matched by key; integers/strings exactly, floats within `rel` relative error."""
+    assert got.num_rows == exp.num_rows, (got.num_rows, exp.num_rows)
+    assert got.column_names == exp.column_names
+    g = {tuple(r[: len(keys)]): r for r in rows_of(got)}
+    e = {tuple(r[: len(keys)]): r for r in rows_of(exp)}
+    assert set(g) == set(e)
+    for k, er in e.items():
+        gr = g[k]
+        for name, a, b in zip(exp.column_names, gr, er):
+            if isinstance(b, float) and isinstance(a, float):
+                assert abs(a - b) <= rel * max(abs(b), 1e-300) or a == b, (name, k, a, b)
+            else:
+                assert a == b, (name, k, a, b)
