@@ -1,0 +1,298 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X hash join / groupby / hash-partition path.
+
+Default (the line the driver records): the 100M-row groupby on one GPU -- int32 key Zipf(1.1)
+over 1M values; float64 U[-1,1) (SUM, MEAN); int64 U[-2^40, 2^40] (SUM, MIN, MAX, COUNT); int32
+U[0, 10^7) (NUNIQUE); 1% nulls in each value column -- one plain groupby call per step, inputs
+resident in HBM, outputs allocated inside the timed region.  `value` = rows/s.
+
+roofline: the dominant kernel (gb_aggregate) timed with HIP events on its own stream during the
+timed steps; achieved = its algorithmic bytes (input columns read once) / its mean duration.
+cpu_baseline: the CPU oracle (single-threaded C restatement of the reference CPU variant) on a
+10M-row sample of the same workload; Arrow Acero (the reference CPU variant's engine, pyarrow,
+use_threads=False) on the same sample is reported beside it.
+
+Other workloads (for profiling): --workload join (1M-build x 10M-probe inner join),
+--workload partition (1B int64 keys -> 256 partitions), --copy-calibration (a plain copy of
+known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "legate-dataframe_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "join/groupby rows/s + % HBM roofline; batched-vs-plain speedup; p99 ms"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# synthetic inputs (device resident)
+# ---------------------------------------------------------------------------------------------
+def zipf_keys(torch, n, n_keys, s, gen, dtype):
+    ranks = torch.arange(1, n_keys + 1, device="cuda", dtype=torch.float64)
+    cdf = torch.cumsum(ranks.pow(-s), 0)
+    cdf /= cdf[-1].clone()
+    out = torch.empty(n, dtype=dtype, device="cuda")
+    chunk = 1 << 26
+    for i in range(0, n, chunk):
+        m = min(chunk, n - i)
+        u = torch.rand(m, device="cuda", generator=gen, dtype=torch.float64)
+        out[i:i + m] = torch.searchsorted(cdf, u).clamp_(max=n_keys - 1).to(dtype)
+    return out
+
+
+def bitmap(torch, valid_bool):
+    """LSB-first validity bitmap (uint8) of a bool tensor, padded to whole 64-bit words."""
+    n = valid_bool.numel()
+    pad = (-n) % 64
+    v = torch.cat([valid_bool, torch.zeros(pad, dtype=torch.bool, device=valid_bool.device)]) if pad else valid_bool
+    w = torch.tensor([1, 2, 4, 8, 16, 32, 64, 128], dtype=torch.uint8, device=v.device)
+    return (v.view(-1, 8).to(torch.uint8) * w).sum(1, dtype=torch.uint8)
+
+
+def make_groupby_inputs(torch, C, n, seed=2):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    k = zipf_keys(torch, n, 1_000_000, 1.1, g, torch.int32)
+    f = torch.rand(n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1
+    i = torch.randint(-(2**40), 2**40 + 1, (n,), device="cuda", generator=g, dtype=torch.int64)
+    u = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int32)
+    masks = [torch.rand(n, device="cuda", generator=g) >= 0.01 for _ in range(3)]
+    cols = [
+        C.DeviceColumn(C.INT32, n, k.view(torch.uint8)),
+        C.DeviceColumn(C.DOUBLE, n, f.view(torch.uint8), bitmap(torch, masks[0])),
+        C.DeviceColumn(C.INT64, n, i.view(torch.uint8), bitmap(torch, masks[1])),
+        C.DeviceColumn(C.INT32, n, u.view(torch.uint8), bitmap(torch, masks[2])),
+    ]
+    host = dict(k=k, f=f, i=i, u=u, masks=masks)
+    return cols, host
+
+
+GB_AGGS = [(1, "sum", None), (1, "mean", None), (2, "sum", None), (2, "min", None), (2, "max", None),
+           (2, "count", None), (3, "count_distinct", None)]
+
+
+def groupby_alg_bytes(n, ngroups):
+    """Inputs read once (values + 1 bit/row validity of the 3 nullable columns) and outputs
+    written once (key i32 + 2 f64 + 4 i64 + 1 i64 per group + 6 validity bitmaps)."""
+    inp = n * (4 + 8 + 8 + 4) + 3 * (n // 8)
+    out = ngroups * (4 + 8 * 7) + 6 * ((ngroups + 7) // 8)
+    return inp, out
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baseline on a sample
+# ---------------------------------------------------------------------------------------------
+def cpu_baseline_groupby(torch, host, sample):
+    import numpy as np
+    import pyarrow as pa
+
+    import oracle
+
+    k = host["k"][:sample].cpu().numpy()
+    f = host["f"][:sample].cpu().numpy()
+    i = host["i"][:sample].cpu().numpy()
+    u = host["u"][:sample].cpu().numpy()
+    m = [x[:sample].cpu().numpy() for x in host["masks"]]
+    keys = [oracle.OCol(oracle.T_INT32, k)]
+    aggs = [(oracle.OCol(oracle.T_DOUBLE, f, m[0]), "sum", None), (oracle.OCol(oracle.T_DOUBLE, f, m[0]), "mean", None),
+            (oracle.OCol(oracle.T_INT64, i, m[1]), "sum", None), (oracle.OCol(oracle.T_INT64, i, m[1]), "min", None),
+            (oracle.OCol(oracle.T_INT64, i, m[1]), "max", None), (oracle.OCol(oracle.T_INT64, i, m[1]), "count", None),
+            (oracle.OCol(oracle.T_INT32, u, m[2]), "count_distinct", None)]
+    t0 = time.perf_counter()
+    oracle.groupby(keys, aggs)
+    t_oracle = time.perf_counter() - t0
+    tab = pa.table({"k": k, "f": pa.array(f, mask=~m[0]), "i": pa.array(i, mask=~m[1]), "u": pa.array(u, mask=~m[2])})
+    t0 = time.perf_counter()
+    pa.TableGroupBy(tab, ["k"], use_threads=False).aggregate(
+        [("f", "sum"), ("f", "mean"), ("i", "sum"), ("i", "min"), ("i", "max"), ("i", "count"), ("u", "count_distinct")])
+    t_acero = time.perf_counter() - t0
+    return sample / t_oracle, sample / t_acero, t_oracle
+
+
+def profiles_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a copy kernel of known size), recorded under profiles/ for this kernel/workload."""
+    p = ROOT / "profiles" / f"{name}_traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+# ---------------------------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------------------------
+def run_groupby(args, torch, C):
+    n = args.rows or 100_000_000
+    cols, host = make_groupby_inputs(torch, C, n)
+    torch.cuda.synchronize()
+    ngroups = None
+    for _ in range(args.warmup):
+        rt = C.groupby_raw(cols, [0], GB_AGGS)
+        ngroups = rt.cols[0].length
+        rt.free()
+    torch.cuda.synchronize()
+    C.profile_reset()
+    C.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rt = C.groupby_raw(cols, [0], GB_AGGS)
+        ngroups = rt.cols[0].length
+        rt.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    C.profile_enable(False)
+    launches, kms = C.profile_get("gb_aggregate")
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    inp, outb = groupby_alg_bytes(n, ngroups)
+    k_avg_ms = kms / max(1, launches)
+    achieved = inp / (k_avg_ms * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": n / (ms_step * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64+f64", "data": "synthetic (seeded torch generator, device resident)",
+        "config": {"workload": "groupby 100M rows: int32 key Zipf(1.1) over 1M; f64 U[-1,1) SUM,MEAN; int64 "
+                               "U[-2^40,2^40] SUM,MIN,MAX,COUNT; int32 U[0,1e7) NUNIQUE; 1% nulls per value column",
+                   "rows": n, "groups": ngroups, "form": "plain, one call per step"},
+        "roofline": {"bound": "hbm", "kernel": "gb_aggregate", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiles_traffic("gb_aggregate_groupby100M"),
+                     "kernel_ms": k_avg_ms, "launches": launches, "alg_bytes_per_launch": inp,
+                     "whole_step_frac": (inp + outb) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu:
+        sample = min(n, 10_000_000)
+        o_rps, a_rps, t_o = cpu_baseline_groupby(torch, host, sample)
+        res["cpu_baseline"] = {"value": o_rps, "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"first {sample} rows of the same workload, all 7 aggregations, "
+                                         f"{t_o:.1f} s single-threaded",
+                               "acero_value": a_rps, "acero_note": "pyarrow TableGroupBy use_threads=False"}
+    return res
+
+
+def run_join(args, torch, C):
+    m, n = 1_000_000, args.rows or 10_000_000
+    g = torch.Generator(device="cuda").manual_seed(1)
+    bkeys = torch.randperm(m, device="cuda", generator=g).to(torch.int64)
+    bpay = torch.randint(-(2**62), 2**62, (m,), device="cuda", generator=g, dtype=torch.int64)
+    pkeys = torch.randint(0, 2 * m, (n,), device="cuda", generator=g, dtype=torch.int64)
+    ppay = torch.rand(n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1
+    dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
+    L = [dc(pkeys, C.INT64), dc(ppay, C.DOUBLE)]
+    R = [dc(bkeys, C.INT64), dc(bpay, C.INT64)]
+    for _ in range(args.warmup):
+        C.join_raw(L, R, [0], [0], 0).free()
+    torch.cuda.synchronize()
+    C.profile_reset()
+    C.profile_enable(True)
+    t0 = time.perf_counter()
+    nout = 0
+    for _ in range(args.steps):
+        rt = C.join_raw(L, R, [0], [0], 0)
+        nout = rt.cols[0].length
+        rt.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    C.profile_enable(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    alg = (n + m) * 16 + nout * 32
+    kern = {k: C.profile_get(k) for k in ("join_build", "join_count", "join_emit", "gather")}
+    return {"workload": "inner join build 1M (permutation keys) x probe 10M (uniform [0,2M)), i64 keys, "
+                        "i64/f64 payloads", "rows_per_s": (n + m) / (ms_step * 1e-3), "ms_per_step": ms_step,
+            "out_rows": nout, "alg_bytes": alg, "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+
+
+def run_partition(args, torch, C):
+    n, P = args.rows or 1_000_000_000, 256
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randint(-(2**63), 2**63 - 1, (n,), device="cuda", generator=g, dtype=torch.int64)
+    col = [C.DeviceColumn(C.INT64, n, x.view(torch.uint8))]
+    for _ in range(args.warmup):
+        rt, _ = C.hash_partition_raw(col, [0], P)
+        rt.free()
+    torch.cuda.synchronize()
+    C.profile_reset()
+    C.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rt, _ = C.hash_partition_raw(col, [0], P)
+        rt.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    C.profile_enable(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern = {k: C.profile_get(k) for k in ("hp_hist", "hp_scatter")}
+    alg = 16 * n
+    return {"workload": f"hash_partition {n} int64 keys -> {P} partitions (murmur3, seed 0)",
+            "keys_per_s": n / (ms_step * 1e-3), "ms_per_step": ms_step, "alg_bytes": alg,
+            "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+
+
+def copy_calibration(torch, nbytes=4 << 30, reps=5):
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--workload", choices=["groupby", "join", "partition"], default="groupby")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the path on a single MI355X (no collectives); "
+              "multi-GPU runs are not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from legate_dataframe import _capi as C
+
+    C.load()
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
+    if args.workload == "groupby":
+        res = run_groupby(args, torch, C)
+        if not args.no_secondary:
+            sa = argparse.Namespace(**vars(args))
+            sa.rows, sa.steps, sa.warmup = 0, max(3, args.steps // 2), 2
+            res["secondary"] = {"join_1Mx10M": run_join(sa, torch, C)}
+            sa.rows = 200_000_000
+            res["secondary"]["partition_200M"] = run_partition(sa, torch, C)
+    elif args.workload == "join":
+        res = run_join(args, torch, C)
+    else:
+        res = run_partition(args, torch, C)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

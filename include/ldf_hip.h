@@ -209,6 +209,14 @@ int ldf_ranges_to_offsets(const int64_t* ranges /* 2 per row */, int64_t n, int6
 int ldf_offsets_to_ranges(const int64_t* offsets, int64_t n, int64_t global_base, int64_t* ranges,
                           void* stream);
 
+/* Per-kernel timing with HIP events recorded on the call's stream around the main kernels of
+ * each entry point (off by default; no counterpart in the reference, whose only tracing is
+ * Legion Prof provenance labels, python/legate_dataframe/utils.py:35-46).  ldf_profile_get
+ * returns the launches and summed milliseconds recorded for `kernel` since the last reset. */
+void ldf_profile_enable(int on);
+void ldf_profile_reset(void);
+int ldf_profile_get(const char* kernel, int64_t* launches, double* total_ms);
+
 /* Frees every buffer of an output table through `alloc` (NULL = library default). */
 void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream);
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "ldf_common.hpp"
@@ -14,6 +16,73 @@ namespace ldf {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------------------------------------
+// default pool + kernel timing
+// ------------------------------------------------------------------------------------------
+void configure_default_pool()
+{
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev             = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  std::lock_guard<std::mutex> lk(mu);
+  if (done[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t thr = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  (void)hipGetLastError();
+  done[dev] = true;
+}
+
+namespace {
+struct ProfState {
+  std::mutex mu;
+  bool on = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<int64_t, double>> acc;
+  void drain()
+  {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        auto& e = acc[p.name];
+        e.first += 1;
+        e.second += ms;
+      }
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
+  }
+};
+ProfState& prof()
+{
+  static ProfState s;
+  return s;
+}
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), s_(s)
+{
+  if (!prof().on) return;
+  if (hipEventCreate(&a_) != hipSuccess || hipEventRecord(a_, s_) != hipSuccess) a_ = nullptr;
+}
+
+ProfScope::~ProfScope()
+{
+  if (a_ == nullptr) return;
+  hipEvent_t b;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s_) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(prof().mu);
+  prof().pending.push_back({name_, a_, b});
+}
 
 void validate_view(const ldf_column_view& c, int64_t nrows, const char* what)
 {
@@ -276,6 +345,7 @@ void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out
   } else {
     alloc_column(src.type_id, n_out, nullable, 0, a, out);
     if (n_out > 0) {
+      ProfScope prof("gather", s);
       switch (w) {
         case 1:
           hipLaunchKernelGGL(k_gather_fixed<uint8_t>, dim3(grid), dim3(256), 0, s, (const uint8_t*)src.data, src.validity, map,
@@ -355,6 +425,29 @@ using namespace ldf;
 extern "C" {
 
 const char* ldf_last_error(void) { return g_last_error.c_str(); }
+
+void ldf_profile_enable(int on)
+{
+  std::lock_guard<std::mutex> lk(prof().mu);
+  prof().on = on != 0;
+}
+
+void ldf_profile_reset(void)
+{
+  std::lock_guard<std::mutex> lk(prof().mu);
+  prof().drain();
+  prof().acc.clear();
+}
+
+int ldf_profile_get(const char* kernel, int64_t* launches, double* total_ms)
+{
+  std::lock_guard<std::mutex> lk(prof().mu);
+  prof().drain();
+  auto it = prof().acc.find(kernel ? kernel : "");
+  if (launches) *launches = it == prof().acc.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == prof().acc.end() ? 0.0 : it->second.second;
+  return it == prof().acc.end() ? LDF_E_INVALID : LDF_OK;
+}
 const char* ldf_version(void) { return "ldf-hip 0.1.0 (gfx950)"; }
 
 void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream)

@@ -688,6 +688,7 @@ struct AggState {
 template <bool GENERIC>
 static void launch_aggregate(const DKeys& keys, int64_t n, const KeyTab& t, const AggBatch& b, hipStream_t s)
 {
+  ProfScope prof("gb_aggregate", s);
   hipLaunchKernelGGL(k_gb_aggregate<GENERIC>, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, s, keys, n, t, b);
   LDF_HIP_TRY(hipGetLastError());
 }

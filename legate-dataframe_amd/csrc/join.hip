@@ -326,6 +326,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
 
   uint64_t cap  = next_pow2((uint64_t)(nb > 32 ? 2 * nb : 64));
   uint64_t mask = cap - 1;
+  LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "join: build side too large for one call (2^29 rows)");
   int64_t nslot = (int64_t)cap + 2;
   Buf<JSlot> tab(a, nslot);
   hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
@@ -335,6 +336,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   Buf<int32_t> slot_of_row(a, nb > 0 ? nb : 1);
   Buf<uint32_t> rows(a, nb > 0 ? nb : 1);
   if (nb > 0) {
+    ProfScope prof("join_build", s);
     if (generic)
       hipLaunchKernelGGL(k_join_build<true>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, cap,
                          nulls_equal, slot_of_row.get());
@@ -368,6 +370,8 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     matched = Buf<uint8_t>(a, nslot);
     fill_u8(matched.get(), 0, nslot, s);
   }
+  {
+  ProfScope prof("join_count", s);
   if (generic)
     hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
                        nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
@@ -375,6 +379,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
                        nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
   LDF_HIP_TRY(hipGetLastError());
+  }
   exclusive_scan_u64(btot.get(), boff.get(), nblk, a);
 
   int64_t ublk = (nb + kJRows - 1) / kJRows;
@@ -398,6 +403,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   Buf<int32_t> rmap;
   if (with_rmap) rmap = Buf<int32_t>(a, n_out > 0 ? n_out : 1);
   if (n1 > 0) {
+    ProfScope prof("join_emit", s);
     hipLaunchKernelGGL(k_join_emit, dim3(nblk), dim3(kJBlock), 0, s, (const uint64_t*)info.get(), np, (int)jt,
                        (const uint32_t*)rows.get(), (const uint64_t*)boff.get(), lmap.get(), rmap.get());
     LDF_HIP_TRY(hipGetLastError());

@@ -64,6 +64,24 @@ int guarded(F&& f)
 // ------------------------------------------------------------------------------------------
 // allocation (all device memory of a call goes through here)
 // ------------------------------------------------------------------------------------------
+// Keeps freed memory in the current device's default pool (release threshold = max) so that
+// stream-ordered allocations after a synchronisation do not go back to the driver.
+void configure_default_pool();
+
+// Records HIP events on `s` around a kernel launch when ldf_profile_enable(1) is set.
+class ProfScope {
+ public:
+  ProfScope(const char* name, hipStream_t s);
+  ~ProfScope();
+  ProfScope(const ProfScope&)            = delete;
+  ProfScope& operator=(const ProfScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr;
+};
+
 class Alloc {
  public:
   Alloc(const ldf_allocator* a, hipStream_t s) : a_(a), s_(s) {}
@@ -79,6 +97,7 @@ class Alloc {
       p = a_->alloc(a_->ctx, bytes, (void*)s_);
       if (p == nullptr) throw Error(LDF_E_OOM, "allocator returned NULL for " + std::to_string(bytes) + " bytes");
     } else {
+      configure_default_pool();
       hipError_t e = hipMallocAsync(&p, bytes, s_);
       if (e != hipSuccess || p == nullptr) {
         (void)hipGetLastError();

@@ -296,9 +296,12 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
 
     Buf<uint32_t> counts(a, (size_t)nparts * nblk);
     Buf<uint64_t> scan(a, (size_t)nparts * nblk + 1);
+    {
+    ProfScope prof("hp_hist", s);
     hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), nparts * sizeof(uint32_t), s, hc, n, rpb,
                        (uint32_t)nparts, counts.get());
     LDF_HIP_TRY(hipGetLastError());
+    }
     exclusive_scan_u32_to_u64(counts.get(), scan.get(), (int64_t)nparts * nblk, a);
 
     ScatterCols sc;
@@ -317,6 +320,13 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       perm = Buf<int64_t>(a, n);
     }
     size_t lds = scatter_lds_bytes((uint32_t)nparts);
+    static bool lds_attr = false;
+    if (!lds_attr) {
+      LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)scatter_lds_bytes(kHpMaxParts)));
+      lds_attr = true;
+    }
+    ProfScope prof("hp_scatter", s);
     hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, pbits,
                        (const uint64_t*)scan.get(), sc, perm.get());
     LDF_HIP_TRY(hipGetLastError());

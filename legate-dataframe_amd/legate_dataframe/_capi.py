@@ -30,6 +30,7 @@ EXPORTED = [
     "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash",
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
+    "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get",
 ]
 
 
@@ -106,6 +107,9 @@ def load(require: bool = True):
         L.ldf_offsets_to_ranges.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p]
         L.ldf_free_table.argtypes = [P(TableOut), P(Allocator), ctypes.c_void_p]
+        L.ldf_profile_enable.argtypes = [ctypes.c_int]
+        L.ldf_profile_get.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_double)]
+        L.ldf_profile_get.restype = ctypes.c_int
         L.ldf_last_error.restype = ctypes.c_char_p
         L.ldf_version.restype = ctypes.c_char_p
         for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_bools_to_bitmask",
@@ -337,3 +341,69 @@ def murmur3_hash(table: Sequence[DeviceColumn], cols, seed: int = 0):
     out = torch.empty(max(1, n), dtype=torch.int32, device=table[0].data.device)
     check(L.ldf_murmur3_hash(ctypes.byref(tv), int32_array(cols), len(cols), seed, out.data_ptr(), stream_ptr()))
     return out[:n]
+
+
+# ---------------------------------------------------------------------------------------------
+# library-default allocator (stream-ordered pool, no Python in the allocation path)
+# ---------------------------------------------------------------------------------------------
+class RawTable:
+    """Output table whose buffers belong to the library's default pool; free() returns them."""
+
+    def __init__(self, ncols: int):
+        self.cols = (Column * max(1, ncols))()
+        self.out = TableOut(ncols, self.cols)
+
+    def free(self):
+        load().ldf_free_table(ctypes.byref(self.out), None, stream_ptr())
+        self.out.ncols = 0
+
+
+def groupby_raw(table: Sequence[DeviceColumn], keys, aggs) -> RawTable:
+    L = load()
+    tv, _ta = make_table_view([c.view() for c in table])
+    arr = (Agg * max(1, len(aggs)))(*[Agg(-1 if c is None else c, AGG[k], COUNT_MODE[m or "only_valid"], 0)
+                                      for c, k, m in aggs])
+    rt = RawTable(len(keys) + len(aggs))
+    check(L.ldf_groupby(ctypes.byref(tv), int32_array(keys), len(keys), arr, len(aggs), None, stream_ptr(),
+                        ctypes.byref(rt.out)))
+    return rt
+
+
+def join_raw(lhs, rhs, lhs_keys, rhs_keys, join_type: int, nulls_equal=True, lhs_out=None, rhs_out=None) -> RawTable:
+    L = load()
+    lhs_out = list(range(len(lhs))) if lhs_out is None else list(lhs_out)
+    if rhs_out is None:
+        rhs_out = [] if join_type in (3, 4) else list(range(len(rhs)))
+    lv, _la = make_table_view([c.view() for c in lhs])
+    rv, _ra = make_table_view([c.view() for c in rhs])
+    rt = RawTable(len(lhs_out) + len(rhs_out))
+    check(L.ldf_join(ctypes.byref(lv), ctypes.byref(rv), int32_array(lhs_keys), int32_array(rhs_keys), len(lhs_keys),
+                     join_type, int(bool(nulls_equal)), int32_array(lhs_out), len(lhs_out), int32_array(rhs_out),
+                     len(rhs_out), None, stream_ptr(), ctypes.byref(rt.out)))
+    return rt
+
+
+def hash_partition_raw(table, cols, nparts: int, seed: int = 0):
+    L = load()
+    tv, _ta = make_table_view([c.view() for c in table])
+    rt = RawTable(len(table))
+    offs = (ctypes.c_int64 * (nparts + 1))()
+    check(L.ldf_hash_partition(ctypes.byref(tv), int32_array(cols), len(cols), nparts, seed, None, stream_ptr(),
+                               ctypes.byref(rt.out), offs))
+    return rt, list(offs)
+
+
+def profile_enable(on: bool = True) -> None:
+    load().ldf_profile_enable(int(on))
+
+
+def profile_reset() -> None:
+    load().ldf_profile_reset()
+
+
+def profile_get(kernel: str):
+    """(launches, total ms) recorded with HIP events for `kernel`."""
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    load().ldf_profile_get(kernel.encode(), ctypes.byref(n), ctypes.byref(ms))
+    return n.value, ms.value

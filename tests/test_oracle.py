@@ -13,8 +13,8 @@ import pytest
 
 import oracle
 from oracle import OCol, T_INT32, T_INT64, T_DOUBLE, T_LARGE_STRING, T_UINT8
-from ldf_testing import (assert_groupby_equal, fixture_table, groupby_fixture, join_fixture, join_via_oracle, ocol,
-                         sorted_rows)
+from ldf_testing import (assert_groupby_equal, fixture_table, groupby_fixture, groupby_via_oracle, join_fixture,
+                         join_via_oracle, ocol, sorted_rows)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -168,39 +168,6 @@ def test_join_oracle_closed_form_permutation_build():
 # ---------------------------------------------------------------------------------------------
 # groupby
 # ---------------------------------------------------------------------------------------------
-def groupby_via_oracle(t: pa.Table, keys, aggs) -> pa.Table:
-    """Materialises the oracle's groupby into the reference's output table (Arrow types)."""
-    kcols = [ocol(t.column(k)) for k in keys]
-    spec = []
-    for col, kind, _ in aggs:
-        mode = None
-        if isinstance(kind, (list, tuple)):
-            kind, mode = kind
-        spec.append((ocol(t.column(col)) if col is not None else None, kind, mode))
-    reps, res = oracle.groupby(kcols, spec)
-    cols = [t.column(k).combine_chunks().take(pa.array(reps)) for k in keys]
-    for (col, kind, _), (bits, f64, valid) in zip(aggs, res):
-        kind = kind[0] if isinstance(kind, (list, tuple)) else kind
-        vt = t.column(col).type if col is not None else pa.int64()
-        if kind in ("count", "count_all", "count_distinct"):
-            arr = pa.array(bits.view(np.int64))
-        elif kind in ("mean", "variance", "stddev") or (kind in ("sum", "product") and pa.types.is_floating(vt)):
-            arr = pa.array(f64, mask=~valid)
-        elif kind in ("sum", "product"):
-            arr = pa.array(bits.view(np.uint64 if pa.types.is_unsigned_integer(vt) or vt == pa.bool_() else np.int64),
-                           mask=~valid)
-        else:  # min / max: raw bits of the input type
-            npt = {pa.float32(): np.float32, pa.float64(): np.float64}.get(vt, None)
-            if vt == pa.bool_():
-                arr = pa.array(bits.astype(bool), mask=~valid)
-            else:
-                width = vt.bit_width // 8
-                raw = bits.astype(np.uint64).view(np.uint8).reshape(-1, 8)[:, :width].copy().reshape(-1)
-                arr = pa.array(raw.view(npt or vt.to_pandas_dtype()), type=vt, mask=~valid)
-        cols.append(arr)
-    return pa.Table.from_arrays(cols, names=list(keys) + [o for _, _, o in aggs])
-
-
 @pytest.mark.parametrize("case", range(len(groupby_fixture()["cases"])))
 def test_groupby_oracle_matches_acero(case):
     fx = groupby_fixture()
