@@ -278,6 +278,8 @@ def _adopt_columns(alloc: TorchAllocator, out: TableOut) -> List[DeviceColumn]:
     for i in range(out.ncols):
         c = out.cols[i]
         data = alloc.adopt(c.data) if c.data else None
+        if data is not None:  # allocations are rounded up; expose the exact byte length
+            data = data[: c.data_bytes if c.type_id == LARGE_STRING else c.length * WIDTH[c.type_id]]
         validity = alloc.adopt(c.validity) if c.validity else None
         offsets = None
         if c.type_id == LARGE_STRING and c.offsets:
