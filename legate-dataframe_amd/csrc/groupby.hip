@@ -9,24 +9,34 @@
 // of a group without valid values is null; MIN/MAX ignore NaN unless every valid value is
 // NaN; COUNT / COUNT_DISTINCT honour CountOptions modes; integer SUM wraps (order-free).
 //
-// Structure (one pass over the rows, everything keyed by hash-table slot):
-//   key table : open addressing, 8-byte slots installed by 64-bit atomicCAS.  A single
-//               fixed-width key stores its bits (two extra slots: the key equal to the empty
-//               marker, and the null key); composite / string keys store
-//               (fingerprint32 << 32 | representative row).
-//   state     : per aggregation, arrays indexed by slot.  Floating SUM / MEAN / VARIANCE
-//               accumulate x * 2^96 exactly in a 192-bit two's-complement integer spread over
-//               three 64-bit words; the carry out of each word is read from the returned old
-//               value of the atomic add, so the sum is exact and independent of the order of
-//               the atomics and the final double is the correctly rounded exact sum.  Values
-//               with |x| >= 2^62 or non-finite values divert that aggregation to a FP64 atomic
-//               fallback pass.  Bits below 2^-96 are truncated (|error| <= n * 2^-96).
-//   distinct  : COUNT_DISTINCT inserts (slot, value) into a second open-addressing set; a
-//               successful CAS counts a new distinct value for the slot.
-// The table size comes from a GEE distinct-value estimate over a 65536-row sample; if the
-// table still fills beyond 3/4 the call is re-run with four times the capacity.
+// Structure:
+//   accumulators  the aggregations are lowered to deduplicated accumulators per input column
+//                 (row count, valid count, wrapping integer sum, exact FP sum, min, max,
+//                 product, distinct count): SUM and MEAN of a column share one exact sum, and
+//                 every aggregation of a column shares its valid count.
+//   key table     open addressing, 8-byte slots installed by 64-bit atomicCAS.  A single
+//                 fixed-width key stores its bits (two extra slots: the key equal to the empty
+//                 marker, and the null key); composite / string keys store
+//                 (fingerprint32 << 32 | representative row).  Global accumulator arrays are
+//                 indexed by slot.
+//   hot keys      a 65536-row sample gives both the table size (GEE distinct estimate) and the
+//                 most frequent keys.  Every block of k_gb_main preloads those keys into an LDS
+//                 hash table with LDS accumulators, aggregates its rows of hot keys with LDS
+//                 atomics, and flushes once at the end; rows of other keys update the global
+//                 arrays directly.  Skewed keys (Zipf) therefore never serialise on global
+//                 atomics.
+//   exact FP sum  x * 2^96 accumulated exactly in a 192-bit two's-complement integer over three
+//                 64-bit words; the carry out of each word is read from the old value the
+//                 atomic add returns, so the sum is exact and independent of the order of the
+//                 atomics, and the final double is the correctly rounded exact sum.  Values with
+//                 |x| >= 2^62 or non-finite values divert that aggregation to an FP64 atomic
+//                 fallback pass.  Bits below 2^-96 are truncated (|error| <= n * 2^-96).
+//   distinct      COUNT_DISTINCT inserts (slot, value) into an open-addressing set; a
+//                 successful CAS counts a new distinct value for the group.
+// If the key table still fills beyond 3/4, the call is re-run with four times the capacity.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "ldf_common.hpp"
@@ -35,31 +45,65 @@
 namespace ldf {
 
 constexpr unsigned long long kGEmpty = ~0ull;
-constexpr int kMaxAggsPerLaunch       = 12;
+constexpr int kGbThreads             = 1024;
+constexpr int kGbRowsPerThread       = 4;
+constexpr int kMaxAcc                = 16;
+constexpr int kMaxCols               = 4;  // distinct value columns read per launch
+constexpr int kMaxOut                = 16;
+constexpr int kSampleRows            = 65536;
 
-struct AggDesc {
+enum AccKind : int32_t {
+  ACC_ROWS     = 0,
+  ACC_VCOUNT   = 1,
+  ACC_ISUM     = 2,
+  ACC_FSUM     = 3,
+  ACC_MIN      = 4,
+  ACC_MAX      = 5,
+  ACC_PROD     = 6,
+  ACC_DISTINCT = 7
+};
+
+struct AccDev {
   int32_t kind;
-  int32_t in_type;
-  int32_t width;  // 0 = string
-  int32_t mode;
+  int32_t col;       // index into GbPlan::col (-1 for ACC_ROWS)
+  int32_t lds_off;   // byte offset of this accumulator's LDS array, -1 when global only
+  int32_t flag_bit;  // MIN/MAX: bit in the NaN-flag word
+  unsigned long long* g0;  // count / sum / min / max / product / limb 0
+  unsigned long long* g1;  // limb 1
+  unsigned long long* g2;  // limb 2
+  uint32_t* gflag;         // MIN/MAX: NaN seen
+  unsigned long long* pset;
+  uint64_t pmask;
+  int32_t* err;  // FSUM: a value left the fixed-point range
+};
+
+struct ColDev {
   const void* data;
   const uint8_t* valid;
   const int64_t* offsets;
-  unsigned long long* a0;   // SUM/PRODUCT/MIN/MAX accumulator or limb 0
-  unsigned long long* a1;   // limb 1
-  unsigned long long* a2;   // limb 2
-  unsigned long long* cnt;  // valid-value / row / distinct count
-  uint8_t* flag;            // NaN seen (MIN/MAX) or null seen (COUNT_DISTINCT)
-  unsigned long long* pset; // COUNT_DISTINCT pair set
-  uint64_t pmask;
-  double* fsum;             // fallback FP64 sum / second-pass accumulator
-  const double* mean;       // VARIANCE second pass
-  int32_t* err;             // set when the exact path cannot represent a value
+  int32_t type;
+  int32_t width;
 };
 
-struct AggBatch {
-  AggDesc d[kMaxAggsPerLaunch];
-  int32_t n;
+struct GbPlan {
+  AccDev a[kMaxAcc];
+  ColDev col[kMaxCols];
+  int32_t nacc;
+  int32_t ncol;
+  int32_t hot_cap;     // LDS hot slots (power of two); 0 = no LDS table
+  int32_t tab_off;     // LDS: u64 keys [2*hot_cap]
+  int32_t tidx_off;    // LDS: u16 hot index [2*hot_cap]
+  int32_t flags_off;   // LDS: u32 NaN flags [hot_cap]
+  int32_t rows_off;    // LDS: u32 rows [hot_cap]
+  int32_t lds_bytes;
+  int64_t rows_per_block;
+};
+
+struct HotList {
+  unsigned long long* keys;  // key bits
+  int32_t* special;          // 1 = key equal to the empty marker, 2 = null key
+  int32_t* gslot;            // global slot
+  unsigned int* count;
 };
 
 struct KeyTab {
@@ -159,8 +203,8 @@ __device__ __forceinline__ int64_t gb_find(const DKeys& k, int64_t r, const KeyT
 // Returns false for values the fixed-point range cannot hold (non-finite or |x| >= 2^62).
 __device__ __forceinline__ bool to_fixed192(double x, uint64_t& l0, uint64_t& l1, uint64_t& l2)
 {
-  uint64_t b   = (uint64_t)__double_as_longlong(x);
-  int ex       = (int)((b >> 52) & 0x7ff);
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  int ex     = (int)((b >> 52) & 0x7ff);
   l0 = l1 = l2 = 0;
   if (ex == 0x7ff || ex >= 1023 + 62) return false;
   if (ex == 0) return true;  // zero or subnormal: below 2^-96, contributes 0
@@ -179,7 +223,7 @@ __device__ __forceinline__ bool to_fixed192(double x, uint64_t& l0, uint64_t& l1
     l0 = (-shift < 64) ? (m >> (-shift)) : 0;
   }
   if (b >> 63) {  // negate
-    l0 = ~l0 + 1;
+    l0         = ~l0 + 1;
     uint64_t c = (l0 == 0);
     l1         = ~l1 + c;
     c          = c && (l1 == 0);
@@ -188,6 +232,7 @@ __device__ __forceinline__ bool to_fixed192(double x, uint64_t& l0, uint64_t& l1
   return true;
 }
 
+// 192-bit add of (l0, l1, l2) into three words; works on LDS and global pointers alike.
 __device__ __forceinline__ void add_fixed192(unsigned long long* a0, unsigned long long* a1, unsigned long long* a2,
                                              uint64_t l0, uint64_t l1, uint64_t l2)
 {
@@ -195,9 +240,12 @@ __device__ __forceinline__ void add_fixed192(unsigned long long* a0, unsigned lo
   uint64_t c0           = (o0 + l0) < o0;
   uint64_t y1           = l1 + c0;
   uint64_t c1a          = (y1 < l1);
-  unsigned long long o1 = atomicAdd(a1, (unsigned long long)y1);
-  uint64_t c1           = c1a | (uint64_t)((o1 + y1) < o1);
-  uint64_t y2           = l2 + c1;
+  uint64_t c1           = c1a;
+  if (y1 != 0) {
+    unsigned long long o1 = atomicAdd(a1, (unsigned long long)y1);
+    c1 |= (uint64_t)((o1 + y1) < o1);
+  }
+  uint64_t y2 = l2 + c1;
   if (y2 != 0) atomicAdd(a2, (unsigned long long)y2);
 }
 
@@ -206,7 +254,7 @@ __device__ inline double fixed192_to_double(uint64_t l0, uint64_t l1, uint64_t l
 {
   bool neg = (l2 >> 63) != 0;
   if (neg) {
-    l0 = ~l0 + 1;
+    l0         = ~l0 + 1;
     uint64_t c = (l0 == 0);
     l1         = ~l1 + c;
     c          = c && (l1 == 0);
@@ -217,7 +265,6 @@ __device__ inline double fixed192_to_double(uint64_t l0, uint64_t l1, uint64_t l
   else if (l1) lz = 64 + __clzll(l1);
   else if (l0) lz = 128 + __clzll(l0);
   else return 0.0;
-  // normalise so the top set bit is bit 191
   uint64_t t2, t1, t0;
   if (lz >= 128) {
     t2 = l0 << (lz - 128);
@@ -234,10 +281,10 @@ __device__ inline double fixed192_to_double(uint64_t l0, uint64_t l1, uint64_t l
     t1    = s ? (l1 << s) | (l0 >> (64 - s)) : l1;
     t0    = s ? (l0 << s) : l0;
   }
-  int p         = 191 - lz;  // exponent of the top bit (value = t * 2^(p-191) * 2^-96)
-  uint64_t q    = t2 >> 11;
-  uint64_t rem  = t2 & 0x7ff;
-  bool sticky   = (t1 | t0) != 0;
+  int p        = 191 - lz;  // exponent of the top bit
+  uint64_t q   = t2 >> 11;
+  uint64_t rem = t2 & 0x7ff;
+  bool sticky  = (t1 | t0) != 0;
   if (rem > 0x400 || (rem == 0x400 && (sticky || (q & 1)))) {
     ++q;
     if (q == (1ull << 53)) {
@@ -250,36 +297,46 @@ __device__ inline double fixed192_to_double(uint64_t l0, uint64_t l1, uint64_t l
 }
 
 // ---- value helpers ----------------------------------------------------------------------
-__device__ __forceinline__ double load_as_double(const AggDesc& a, int64_t r)
-{
-  switch (a.in_type) {
-    case LDF_TYPE_DOUBLE: return static_cast<const double*>(a.data)[r];
-    case LDF_TYPE_FLOAT: return (double)static_cast<const float*>(a.data)[r];
-    case LDF_TYPE_BOOL:
-    case LDF_TYPE_UINT8:
-    case LDF_TYPE_UINT16:
-    case LDF_TYPE_UINT32:
-    case LDF_TYPE_UINT64: return (double)load_bits(a.data, a.width, r);
-    default: return (double)load_signed(a.data, a.width, r);
-  }
-}
-
 __device__ __forceinline__ bool is_float_type(int t) { return t == LDF_TYPE_DOUBLE || t == LDF_TYPE_FLOAT; }
 __device__ __forceinline__ bool is_unsigned_type(int t)
 {
   return t == LDF_TYPE_BOOL || t == LDF_TYPE_UINT8 || t == LDF_TYPE_UINT16 || t == LDF_TYPE_UINT32 || t == LDF_TYPE_UINT64;
 }
 
-// order-preserving map to uint64 (NaN excluded by the caller)
-__device__ __forceinline__ uint64_t ordered_bits(const AggDesc& a, int64_t r)
+// value bits -> double / wrapping integer / order-preserving uint64 (NaN excluded by callers)
+__device__ __forceinline__ double bits_as_double(uint64_t b, int type, int width)
 {
-  if (is_float_type(a.in_type)) {
-    double d   = load_as_double(a, r);
-    uint64_t b = (uint64_t)__double_as_longlong(d);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+  if (type == LDF_TYPE_DOUBLE) return __longlong_as_double((long long)b);
+  if (type == LDF_TYPE_FLOAT) return (double)__uint_as_float((uint32_t)b);
+  if (is_unsigned_type(type)) return (double)b;
+  switch (width) {
+    case 1: return (double)(int8_t)b;
+    case 2: return (double)(int16_t)b;
+    case 4: return (double)(int32_t)b;
+    default: return (double)(int64_t)b;
   }
-  if (is_unsigned_type(a.in_type)) return load_bits(a.data, a.width, r);
-  return (uint64_t)load_signed(a.data, a.width, r) ^ 0x8000000000000000ull;
+}
+
+__device__ __forceinline__ uint64_t bits_as_int(uint64_t b, int type, int width)
+{
+  if (is_unsigned_type(type)) return b;
+  switch (width) {
+    case 1: return (uint64_t)(int64_t)(int8_t)b;
+    case 2: return (uint64_t)(int64_t)(int16_t)b;
+    case 4: return (uint64_t)(int64_t)(int32_t)b;
+    default: return b;
+  }
+}
+
+__device__ __forceinline__ uint64_t ordered_of(uint64_t b, int type, int width)
+{
+  if (is_float_type(type)) {
+    double d    = bits_as_double(b, type, width);
+    uint64_t db = (uint64_t)__double_as_longlong(d);
+    return (db >> 63) ? ~db : (db | 0x8000000000000000ull);
+  }
+  if (is_unsigned_type(type)) return b;
+  return bits_as_int(b, type, width) ^ 0x8000000000000000ull;
 }
 
 __device__ __forceinline__ void atomic_mul_u64(unsigned long long* p, uint64_t v)
@@ -301,166 +358,359 @@ __device__ __forceinline__ void atomic_mul_f64(unsigned long long* p, double v)
   } while (old != assumed);
 }
 
-// value bits used by COUNT_DISTINCT (floating point compared bitwise, as Arrow does)
-__device__ __forceinline__ uint64_t distinct_bits(const AggDesc& a, int64_t r) { return load_bits(a.data, a.width, r); }
-
-template <bool GENERIC>
-__device__ inline void distinct_insert(const AggDesc& a, const DKeys& keys, int64_t r, int64_t slot)
+// Inserts (slot, value) into the distinct set; true when the pair is new.
+__device__ inline bool distinct_insert(const AccDev& a, const ColDev& c, const DKeys& keys, int64_t r, int64_t slot,
+                                       uint64_t vbits)
 {
-  if (a.width > 0 && a.width <= 4) {
-    unsigned long long pk = ((unsigned long long)(uint32_t)slot << 32) | (uint32_t)distinct_bits(a, r);
+  if (c.width > 0 && c.width <= 4) {
+    unsigned long long pk = ((unsigned long long)(uint32_t)slot << 32) | (uint32_t)vbits;
     uint64_t i            = mix64(pk) & a.pmask;
     while (true) {
       unsigned long long cur = atomic_load_relaxed(&a.pset[i]);
-      if (cur == pk) return;
+      if (cur == pk) return false;
       if (cur == kGEmpty) {
         unsigned long long old = atomicCAS(&a.pset[i], kGEmpty, pk);
-        if (old == kGEmpty) {
-          atomicAdd(&a.cnt[slot], 1ull);
-          return;
-        }
-        if (old == pk) return;
+        if (old == kGEmpty) return true;
+        if (old == pk) return false;
       }
       i = (i + 1) & a.pmask;
     }
+  }
+  // (fingerprint, representative row): equality re-checks the key tuple and the value
+  uint64_t vh;
+  if (c.width == 0) {
+    int64_t b = c.offsets[r], e = c.offsets[r + 1];
+    vh        = string_hash64(static_cast<const uint8_t*>(c.data) + b, e - b);
   } else {
-    // (fingerprint, representative row); equality re-checks the key tuple and the value
-    uint64_t vh;
-    if (a.width == 0) {
-      int64_t b = a.offsets[r], e = a.offsets[r + 1];
-      vh        = string_hash64(static_cast<const uint8_t*>(a.data) + b, e - b);
-    } else {
-      vh = mix64(distinct_bits(a, r) ^ 0x51afd7ed558ccdffull);
+    vh = mix64(vbits ^ 0x51afd7ed558ccdffull);
+  }
+  uint64_t h                = mix64(vh ^ mix64((uint64_t)slot + 0x9e3779b97f4a7c15ull));
+  uint32_t fp               = (uint32_t)(h >> 32);
+  unsigned long long packed = ((unsigned long long)fp << 32) | (uint32_t)r;
+  uint64_t i                = h & a.pmask;
+  while (true) {
+    unsigned long long cur = atomic_load_relaxed(&a.pset[i]);
+    if (cur == kGEmpty) {
+      unsigned long long old = atomicCAS(&a.pset[i], kGEmpty, packed);
+      if (old == kGEmpty) return true;
+      cur = old;
     }
-    uint64_t h                = mix64(vh ^ mix64((uint64_t)slot + 0x9e3779b97f4a7c15ull));
-    uint32_t fp               = (uint32_t)(h >> 32);
-    unsigned long long packed = ((unsigned long long)fp << 32) | (uint32_t)r;
-    uint64_t i                = h & a.pmask;
-    while (true) {
-      unsigned long long cur = atomic_load_relaxed(&a.pset[i]);
-      if (cur == kGEmpty) {
-        unsigned long long old = atomicCAS(&a.pset[i], kGEmpty, packed);
-        if (old == kGEmpty) {
-          atomicAdd(&a.cnt[slot], 1ull);
-          return;
+    if ((uint32_t)(cur >> 32) == fp) {
+      int64_t o = (int64_t)(uint32_t)cur;
+      bool same = rows_equal(keys, r, keys, o);
+      if (same) {
+        if (c.width == 0) {
+          DCol d;
+          d.data    = c.data;
+          d.offsets = c.offsets;
+          d.valid   = nullptr;
+          d.width   = 0;
+          same      = strings_equal(d, r, d, o);
+        } else {
+          same = load_bits(c.data, c.width, o) == vbits;
         }
-        cur = old;
       }
-      if ((uint32_t)(cur >> 32) == fp) {
-        int64_t o = (int64_t)(uint32_t)cur;
-        bool same = rows_equal(keys, r, keys, o);
-        if (same) {
-          if (a.width == 0) {
-            DCol c;
-            c.data    = a.data;
-            c.offsets = a.offsets;
-            c.valid   = nullptr;
-            c.width   = 0;
-            same      = strings_equal(c, r, c, o);
-          } else {
-            same = distinct_bits(a, r) == distinct_bits(a, o);
+      if (same) return false;
+    }
+    i = (i + 1) & a.pmask;
+  }
+}
+
+// ---- per-row accumulation ---------------------------------------------------------------
+// Values of one row for the plan's (up to 4) input columns, passed by value so they stay in
+// registers.
+struct RowVals {
+  uint64_t b0, b1, b2, b3;
+  uint32_t valid;  // bit c = column c valid
+};
+
+__device__ __forceinline__ uint64_t pick(const RowVals v, int c)
+{
+  uint64_t x = v.b0;
+  x          = c == 1 ? v.b1 : x;
+  x          = c == 2 ? v.b2 : x;
+  x          = c == 3 ? v.b3 : x;
+  return x;
+}
+
+// global accumulators at slot s
+__device__ __forceinline__ void acc_global(const GbPlan& p, const DKeys& keys, int64_t r, int64_t s, const RowVals v)
+{
+  for (int i = 0; i < p.nacc; ++i) {
+    const AccDev& a = p.a[i];
+    const bool ok   = a.col < 0 || ((v.valid >> a.col) & 1);
+    if (a.kind == ACC_ROWS) {
+      if (a.g0) atomicAdd(&a.g0[s], 1ull);
+      continue;
+    }
+    if (!ok) continue;
+    const ColDev& c   = p.col[a.col];
+    const uint64_t vb = pick(v, a.col);
+    switch (a.kind) {
+      case ACC_VCOUNT: atomicAdd(&a.g0[s], 1ull); break;
+      case ACC_ISUM: atomicAdd(&a.g0[s], (unsigned long long)bits_as_int(vb, c.type, c.width)); break;
+      case ACC_FSUM: {
+        uint64_t l0, l1, l2;
+        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) add_fixed192(&a.g0[s], &a.g1[s], &a.g2[s], l0, l1, l2);
+        else *a.err = 1;
+        break;
+      }
+      case ACC_MIN:
+      case ACC_MAX: {
+        if (is_float_type(c.type) && isnan(bits_as_double(vb, c.type, c.width))) {
+          atomicOr(&a.gflag[s], 1u);
+          break;
+        }
+        uint64_t o = ordered_of(vb, c.type, c.width);
+        if (a.kind == ACC_MIN) atomicMin(&a.g0[s], (unsigned long long)o);
+        else atomicMax(&a.g0[s], (unsigned long long)o);
+        break;
+      }
+      case ACC_PROD:
+        if (is_float_type(c.type)) atomic_mul_f64(&a.g0[s], bits_as_double(vb, c.type, c.width));
+        else atomic_mul_u64(&a.g0[s], bits_as_int(vb, c.type, c.width));
+        break;
+      case ACC_DISTINCT:
+        if (distinct_insert(a, c, keys, r, s, vb)) atomicAdd(&a.g0[s], 1ull);
+        break;
+      default: break;
+    }
+  }
+}
+
+// LDS accumulators of hot slot h (global slot s for the global-only accumulators)
+__device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, const DKeys& keys, int64_t r, int h,
+                                        int64_t s, const RowVals v)
+{
+  uint32_t* rows = reinterpret_cast<uint32_t*>(lds + p.rows_off);
+  atomicAdd(&rows[h], 1u);
+  for (int i = 0; i < p.nacc; ++i) {
+    const AccDev& a = p.a[i];
+    if (a.kind == ACC_ROWS) continue;
+    const bool ok = (v.valid >> a.col) & 1;
+    if (!ok) continue;
+    const ColDev& c   = p.col[a.col];
+    const uint64_t vb = pick(v, a.col);
+    switch (a.kind) {
+      case ACC_VCOUNT: atomicAdd(&reinterpret_cast<uint32_t*>(lds + a.lds_off)[h], 1u); break;
+      case ACC_ISUM:
+        atomicAdd(&reinterpret_cast<unsigned long long*>(lds + a.lds_off)[h],
+                  (unsigned long long)bits_as_int(vb, c.type, c.width));
+        break;
+      case ACC_FSUM: {
+        uint64_t l0, l1, l2;
+        unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
+        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2))
+          add_fixed192(&base[3 * h], &base[3 * h + 1], &base[3 * h + 2], l0, l1, l2);
+        else *a.err = 1;
+        break;
+      }
+      case ACC_MIN:
+      case ACC_MAX: {
+        if (is_float_type(c.type) && isnan(bits_as_double(vb, c.type, c.width))) {
+          atomicOr(&reinterpret_cast<uint32_t*>(lds + p.flags_off)[h], 1u << a.flag_bit);
+          break;
+        }
+        uint64_t o               = ordered_of(vb, c.type, c.width);
+        unsigned long long* slot = &reinterpret_cast<unsigned long long*>(lds + a.lds_off)[h];
+        if (a.kind == ACC_MIN) atomicMin(slot, (unsigned long long)o);
+        else atomicMax(slot, (unsigned long long)o);
+        break;
+      }
+      case ACC_PROD:
+        if (is_float_type(c.type)) atomic_mul_f64(&a.g0[s], bits_as_double(vb, c.type, c.width));
+        else atomic_mul_u64(&a.g0[s], bits_as_int(vb, c.type, c.width));
+        break;
+      case ACC_DISTINCT:
+        if (distinct_insert(a, c, keys, r, s, vb)) atomicAdd(&reinterpret_cast<uint32_t*>(lds + a.lds_off)[h], 1u);
+        break;
+      default: break;
+    }
+  }
+}
+
+template <bool GENERIC>
+__global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, KeyTab t, GbPlan p, HotList hot)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_null_hot, s_sent_hot;
+  const int H = p.hot_cap;
+  int nhot    = 0;
+  if (!GENERIC && H > 0) {
+    nhot = (int)min(*hot.count, (unsigned int)H);
+    const int T                 = 2 * H;
+    unsigned long long* tkeys   = reinterpret_cast<unsigned long long*>(lds + p.tab_off);
+    uint16_t* tidx              = reinterpret_cast<uint16_t*>(lds + p.tidx_off);
+    for (int i = threadIdx.x; i < T; i += blockDim.x) tkeys[i] = kGEmpty;
+    // accumulator arrays: zero, MIN = all ones
+    for (int i = threadIdx.x; i < H; i += blockDim.x) {
+      reinterpret_cast<uint32_t*>(lds + p.rows_off)[i]  = 0;
+      reinterpret_cast<uint32_t*>(lds + p.flags_off)[i] = 0;
+    }
+    for (int j = 0; j < p.nacc; ++j) {
+      const AccDev& a = p.a[j];
+      if (a.lds_off < 0) continue;
+      if (a.kind == ACC_VCOUNT || a.kind == ACC_DISTINCT) {
+        for (int i = threadIdx.x; i < H; i += blockDim.x) reinterpret_cast<uint32_t*>(lds + a.lds_off)[i] = 0;
+      } else {
+        const int words          = a.kind == ACC_FSUM ? 3 * H : H;
+        const unsigned long long v = a.kind == ACC_MIN ? ~0ull : 0ull;
+        for (int i = threadIdx.x; i < words; i += blockDim.x) reinterpret_cast<unsigned long long*>(lds + a.lds_off)[i] = v;
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_null_hot = -1;
+      s_sent_hot = -1;
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+      int sp = hot.special[h];
+      if (sp == 1) s_sent_hot = h;
+      else if (sp == 2) s_null_hot = h;
+      else {
+        unsigned long long k = hot.keys[h];
+        uint32_t i           = (uint32_t)(mix64(k) & (uint64_t)(T - 1));
+        while (true) {
+          unsigned long long old = atomicCAS(&tkeys[i], kGEmpty, k);
+          if (old == kGEmpty) {
+            tidx[i] = (uint16_t)h;
+            break;
+          }
+          i = (i + 1) & (T - 1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
+  const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
+  const DCol& kc      = keys.c[0];
+  for (int64_t tile = begin; tile < end; tile += (int64_t)kGbRowsPerThread * blockDim.x) {
+    RowVals v[kGbRowsPerThread];
+    uint64_t kb[kGbRowsPerThread];
+    bool kv[kGbRowsPerThread];
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      const int64_t r = tile + (int64_t)j * blockDim.x + threadIdx.x;
+      v[j]  = RowVals{0, 0, 0, 0, 0};
+      kb[j] = 0;
+      kv[j] = false;
+      if (r < end) {
+        if (!GENERIC) {
+          kv[j] = valid_at(kc.valid, r);
+          kb[j] = load_bits(kc.data, kc.width, r);
+        }
+        uint32_t vm = 0;
+        uint64_t x[kMaxCols] = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < kMaxCols; ++c) {
+          if (c < p.ncol) {
+            const ColDev& cd = p.col[c];
+            if (valid_at(cd.valid, r)) vm |= 1u << c;
+            x[c] = cd.width > 0 ? load_bits(cd.data, cd.width, r) : 0;
           }
         }
-        if (same) return;
+        v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
       }
-      i = (i + 1) & a.pmask;
     }
-  }
-}
-
-template <bool GENERIC>
-__device__ __forceinline__ void apply_agg(const AggDesc& a, const DKeys& keys, int64_t r, int64_t s)
-{
-  const bool v = valid_at(a.valid, r);
-  switch (a.kind) {
-    case LDF_AGG_COUNT_ALL: atomicAdd(&a.cnt[s], 1ull); break;
-    case LDF_AGG_COUNT: {
-      bool c = (a.mode == LDF_COUNT_ALL) || (a.mode == LDF_COUNT_ONLY_NULL ? !v : v);
-      if (c) atomicAdd(&a.cnt[s], 1ull);
-      break;
-    }
-    case LDF_AGG_SUM:
-    case LDF_AGG_MEAN:
-    case LDF_AGG_VARIANCE:
-    case LDF_AGG_STDDEV: {
-      if (!v) break;
-      atomicAdd(&a.cnt[s], 1ull);
-      if (a.kind == LDF_AGG_SUM && !is_float_type(a.in_type)) {
-        uint64_t x = is_unsigned_type(a.in_type) ? load_bits(a.data, a.width, r) : (uint64_t)load_signed(a.data, a.width, r);
-        atomicAdd(&a.a0[s], (unsigned long long)x);
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      const int64_t r = tile + (int64_t)j * blockDim.x + threadIdx.x;
+      if (r >= end) continue;
+      int h = -1;
+      if (!GENERIC && nhot > 0) {
+        if (!kv[j]) h = s_null_hot;
+        else if (kb[j] == kGEmpty) h = s_sent_hot;
+        else {
+          const int T                     = 2 * H;
+          const unsigned long long* tkeys = reinterpret_cast<const unsigned long long*>(lds + p.tab_off);
+          uint32_t i                      = (uint32_t)(mix64(kb[j]) & (uint64_t)(T - 1));
+          while (true) {
+            unsigned long long cur = tkeys[i];
+            if (cur == kb[j]) {
+              h = reinterpret_cast<const uint16_t*>(lds + p.tidx_off)[i];
+              break;
+            }
+            if (cur == kGEmpty) break;
+            i = (i + 1) & (T - 1);
+          }
+        }
+      }
+      if (h >= 0) {
+        acc_lds(p, lds, keys, r, h, hot.gslot[h], v[j]);
       } else {
-        double x = load_as_double(a, r);
-        uint64_t l0, l1, l2;
-        if (to_fixed192(x, l0, l1, l2)) add_fixed192(&a.a0[s], &a.a1[s], &a.a2[s], l0, l1, l2);
-        else *a.err = 1;
+        int64_t s = gb_find_or_insert<GENERIC>(keys, r, t);
+        if (s >= 0) acc_global(p, keys, r, s, v[j]);
       }
-      break;
     }
-    case LDF_AGG_PRODUCT: {
-      if (!v) break;
-      atomicAdd(&a.cnt[s], 1ull);
-      if (is_float_type(a.in_type)) atomic_mul_f64(&a.a0[s], load_as_double(a, r));
-      else {
-        uint64_t x = is_unsigned_type(a.in_type) ? load_bits(a.data, a.width, r) : (uint64_t)load_signed(a.data, a.width, r);
-        atomic_mul_u64(&a.a0[s], x);
+  }
+
+  if (!GENERIC && H > 0) {
+    __syncthreads();
+    const uint32_t* rows = reinterpret_cast<const uint32_t*>(lds + p.rows_off);
+    const uint32_t* flg  = reinterpret_cast<const uint32_t*>(lds + p.flags_off);
+    for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+      const uint32_t nr = rows[h];
+      if (nr == 0) continue;
+      const int64_t s = hot.gslot[h];
+      for (int j = 0; j < p.nacc; ++j) {
+        const AccDev& a = p.a[j];
+        switch (a.kind) {
+          case ACC_ROWS:
+            if (a.g0) atomicAdd(&a.g0[s], (unsigned long long)nr);
+            break;
+          case ACC_VCOUNT:
+          case ACC_DISTINCT: {
+            uint32_t c = reinterpret_cast<const uint32_t*>(lds + a.lds_off)[h];
+            if (c) atomicAdd(&a.g0[s], (unsigned long long)c);
+            break;
+          }
+          case ACC_ISUM: {
+            unsigned long long x = reinterpret_cast<const unsigned long long*>(lds + a.lds_off)[h];
+            if (x) atomicAdd(&a.g0[s], x);
+            break;
+          }
+          case ACC_FSUM: {
+            const unsigned long long* b = reinterpret_cast<const unsigned long long*>(lds + a.lds_off) + 3 * h;
+            if (b[0] | b[1] | b[2]) add_fixed192(&a.g0[s], &a.g1[s], &a.g2[s], b[0], b[1], b[2]);
+            break;
+          }
+          case ACC_MIN:
+          case ACC_MAX: {
+            unsigned long long x = reinterpret_cast<const unsigned long long*>(lds + a.lds_off)[h];
+            if (a.kind == ACC_MIN && x != ~0ull) atomicMin(&a.g0[s], x);
+            if (a.kind == ACC_MAX && x != 0ull) atomicMax(&a.g0[s], x);
+            if ((flg[h] >> a.flag_bit) & 1) atomicOr(&a.gflag[s], 1u);
+            break;
+          }
+          default: break;
+        }
       }
-      break;
     }
-    case LDF_AGG_MIN:
-    case LDF_AGG_MAX: {
-      if (!v) break;
-      atomicAdd(&a.cnt[s], 1ull);
-      if (is_float_type(a.in_type) && isnan(load_as_double(a, r))) {
-        a.flag[s] = 1;
-        break;
-      }
-      uint64_t o = ordered_bits(a, r);
-      if (a.kind == LDF_AGG_MIN) atomicMin(&a.a0[s], (unsigned long long)o);
-      else atomicMax(&a.a0[s], (unsigned long long)o);
-      break;
-    }
-    case LDF_AGG_COUNT_DISTINCT: {
-      if (!v) {
-        if (a.mode != LDF_COUNT_ONLY_VALID) a.flag[s] = 1;
-        break;
-      }
-      if (a.mode == LDF_COUNT_ONLY_NULL) break;
-      distinct_insert<GENERIC>(a, keys, r, s);
-      break;
-    }
-    default: break;
   }
 }
 
+// FP64 fallback (sums that left the fixed-point range) and the VARIANCE second pass.
 template <bool GENERIC>
-__global__ __launch_bounds__(256) void k_gb_aggregate(DKeys keys, int64_t n, KeyTab t, AggBatch b)
+__global__ __launch_bounds__(256) void k_gb_second_pass(DKeys keys, int64_t n, KeyTab t, ColDev c, double* acc,
+                                                        const double* mean)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s = gb_find_or_insert<GENERIC>(keys, r, t);
-    if (s < 0) continue;
-    for (int i = 0; i < b.n; ++i) apply_agg<GENERIC>(b.d[i], keys, r, s);
-  }
-}
-
-// FP64 fallback (sums that left the fixed-point range) and VARIANCE second pass.
-template <bool GENERIC>
-__global__ __launch_bounds__(256) void k_gb_second_pass(DKeys keys, int64_t n, KeyTab t, AggDesc a, int second)
-{
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    if (!valid_at(a.valid, r)) continue;
+    if (!valid_at(c.valid, r)) continue;
     int64_t s = gb_find<GENERIC>(keys, r, t);
     if (s < 0) continue;
-    double x = load_as_double(a, r);
-    if (second) {
-      double d = x - a.mean[s];
-      atomicAdd(&a.fsum[s], d * d);
+    double x = bits_as_double(load_bits(c.data, c.width, r), c.type, c.width);
+    if (mean) {
+      double d = x - mean[s];
+      atomicAdd(&acc[s], d * d);
     } else {
-      atomicAdd(&a.fsum[s], x);
+      atomicAdd(&acc[s], x);
     }
   }
 }
 
-// ---- sampling: distinct-count estimate ---------------------------------------------------
+// ---- sampling: table size and hot keys ---------------------------------------------------
 template <bool GENERIC>
 __global__ void k_gb_sample(DKeys keys, int64_t n, int64_t S, int64_t stride, KeyTab t, unsigned int* cnt)
 {
@@ -472,18 +722,59 @@ __global__ void k_gb_sample(DKeys keys, int64_t n, int64_t S, int64_t stride, Ke
   }
 }
 
-__global__ void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* out3)
+// stats[0] = distinct, [1] = singletons, [2] = doubletons, [3 + c] = keys seen c times (c < 64)
+__global__ void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* stats)
 {
-  unsigned long long d = 0, f1 = 0, f2 = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     unsigned c = cnt[i];
-    d += c > 0;
-    f1 += c == 1;
-    f2 += c == 2;
+    if (c == 0) continue;
+    atomicAdd(&stats[0], 1ull);
+    if (c == 1) atomicAdd(&stats[1], 1ull);
+    if (c == 2) atomicAdd(&stats[2], 1ull);
+    atomicAdd(&stats[3 + (c < 63 ? c : 63)], 1ull);
   }
-  if (d) atomicAdd(&out3[0], d);
-  if (f1) atomicAdd(&out3[1], f1);
-  if (f2) atomicAdd(&out3[2], f2);
+}
+
+__global__ void k_gb_hot_select(const unsigned long long* sslots, const unsigned int* cnt, int64_t scap, unsigned int thr,
+                                int cap, HotList hot)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < scap + 2; i += (int64_t)gridDim.x * blockDim.x) {
+    if (cnt[i] < thr) continue;
+    unsigned int h = atomicAdd(hot.count, 1u);
+    if (h >= (unsigned)cap) continue;
+    hot.keys[h]    = sslots[i];
+    hot.special[h] = i == scap ? 1 : (i == scap + 1 ? 2 : 0);
+  }
+}
+
+__global__ void k_gb_hot_insert(KeyTab t, HotList hot, int cap)
+{
+  int nh = (int)min(*hot.count, (unsigned int)cap);
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+    int sp = hot.special[h];
+    int64_t s;
+    if (sp == 1) {
+      s             = (int64_t)t.cap;
+      t.slots[s]    = 0;
+    } else if (sp == 2) {
+      s             = (int64_t)t.cap + 1;
+      t.slots[s]    = 0;
+    } else {
+      unsigned long long key = hot.keys[h];
+      uint64_t i             = mix64(key) & t.mask;
+      while (true) {
+        unsigned long long old = atomicCAS(&t.slots[i], kGEmpty, key);
+        if (old == kGEmpty) {
+          atomicAdd(t.nkeys, 1ull);
+          break;
+        }
+        if (old == key) break;
+        i = (i + 1) & t.mask;
+      }
+      s = (int64_t)i;
+    }
+    hot.gslot[h] = (int32_t)s;
+  }
 }
 
 // ---- finalisation ------------------------------------------------------------------------
@@ -493,166 +784,174 @@ __global__ void k_gb_occupied(const unsigned long long* slots, int64_t nslot, ui
     occ[i] = slots[i] != kGEmpty;
 }
 
-// output position of every occupied slot; rep[g] = representative row (generic keys)
-__global__ void k_gb_compact(const unsigned long long* slots, int64_t nslot, uint64_t cap, const uint64_t* pos, bool generic,
+__global__ void k_gb_compact(const unsigned long long* slots, int64_t nslot, const uint64_t* pos, bool generic,
                              int32_t* slot_of_group, int32_t* rep)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
     if (slots[i] == kGEmpty) continue;
-    uint64_t g        = pos[i];
-    slot_of_group[g]  = (int32_t)i;
+    uint64_t g       = pos[i];
+    slot_of_group[g] = (int32_t)i;
     if (generic) rep[g] = (int32_t)(uint32_t)slots[i];
   }
 }
 
-// single fixed-width key: write the key bits of every group
-__global__ void k_gb_write_key(const unsigned long long* slots, const int32_t* slot_of_group, int64_t ng, uint64_t cap,
-                               int width, void* out, uint8_t* valid)
-{
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s = slot_of_group[g];
-    uint64_t k = (s == (int64_t)cap) ? kGEmpty : (s == (int64_t)cap + 1 ? 0 : slots[s]);
-    bool ok    = s != (int64_t)cap + 1;
-    switch (width) {
-      case 1: static_cast<uint8_t*>(out)[g] = (uint8_t)k; break;
-      case 2: static_cast<uint16_t*>(out)[g] = (uint16_t)k; break;
-      case 4: static_cast<uint32_t*>(out)[g] = (uint32_t)k; break;
-      default: static_cast<uint64_t*>(out)[g] = k; break;
-    }
-    if (valid) valid[g] = ok;
-  }
-}
-
-struct FinalDesc {
-  int32_t kind;
+struct OutDev {
+  int32_t kind;  // LDF_AGG_* ; -1 = the single fixed-width key column
+  int32_t mode;
   int32_t in_type;
   int32_t out_type;
-  int32_t mode;
-  const unsigned long long* a0;
-  const unsigned long long* a1;
-  const unsigned long long* a2;
-  const unsigned long long* cnt;
-  const uint8_t* flag;
-  const double* fsum;
+  const unsigned long long* main;  // sum / min / max / product / distinct count
+  const unsigned long long* l1;
+  const unsigned long long* l2;
+  const unsigned long long* vcount;
+  const unsigned long long* rows;
+  const uint32_t* flag;
+  const double* fsum;  // fallback sum or M2
   int32_t use_fsum;
+  void* out;
+  uint64_t* valid;
+  unsigned long long* nulls;
 };
 
-__device__ __forceinline__ double fixed_sum(const FinalDesc& f, int64_t s)
+struct OutPlan {
+  OutDev o[kMaxOut];
+  int32_t n;
+  uint64_t cap;
+  const unsigned long long* slots;
+};
+
+__device__ __forceinline__ void store_bits(void* out, int type, int64_t g, uint64_t bits)
 {
-  if (f.use_fsum) return f.fsum[s];
-  return fixed192_to_double(f.a0[s], f.a1[s], f.a2[s]);
+  switch (type) {
+    case LDF_TYPE_BOOL:
+    case LDF_TYPE_UINT8:
+    case LDF_TYPE_INT8: static_cast<uint8_t*>(out)[g] = (uint8_t)bits; break;
+    case LDF_TYPE_UINT16:
+    case LDF_TYPE_INT16: static_cast<uint16_t*>(out)[g] = (uint16_t)bits; break;
+    case LDF_TYPE_UINT32:
+    case LDF_TYPE_INT32:
+    case LDF_TYPE_FLOAT:
+    case LDF_TYPE_DATE32: static_cast<uint32_t*>(out)[g] = (uint32_t)bits; break;
+    default: static_cast<uint64_t*>(out)[g] = bits; break;
+  }
 }
 
-__global__ void k_gb_finalize(FinalDesc f, const int32_t* slot_of_group, int64_t ng, void* out, uint8_t* valid)
+// One wave covers 64 consecutive groups: each output column's validity is one ballot.
+__global__ __launch_bounds__(256) void k_gb_finalize(OutPlan fp, const int32_t* __restrict__ slot_of_group, int64_t ng)
 {
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s          = slot_of_group[g];
-    unsigned long long c = f.cnt ? f.cnt[s] : 0;
-    bool ok            = true;
-    switch (f.kind) {
-      case LDF_AGG_COUNT:
-      case LDF_AGG_COUNT_ALL: static_cast<int64_t*>(out)[g] = (int64_t)c; break;
-      case LDF_AGG_COUNT_DISTINCT: {
-        int64_t v = (int64_t)c;
-        if (f.mode == LDF_COUNT_ALL) v += f.flag[s] ? 1 : 0;
-        if (f.mode == LDF_COUNT_ONLY_NULL) v = f.flag[s] ? 1 : 0;
-        static_cast<int64_t*>(out)[g] = v;
-        break;
-      }
-      case LDF_AGG_SUM:
-        ok = c > 0;
-        if (f.out_type == LDF_TYPE_DOUBLE) static_cast<double*>(out)[g] = ok ? fixed_sum(f, s) : 0.0;
-        else static_cast<uint64_t*>(out)[g] = ok ? f.a0[s] : 0;
-        break;
-      case LDF_AGG_MEAN:
-        ok                             = c > 0;
-        static_cast<double*>(out)[g] = ok ? fixed_sum(f, s) / (double)c : 0.0;
-        break;
-      case LDF_AGG_VARIANCE:
-      case LDF_AGG_STDDEV: {
-        ok       = c > 0;
-        double v = ok ? f.fsum[s] / (double)c : 0.0;
-        static_cast<double*>(out)[g] = (f.kind == LDF_AGG_STDDEV) ? sqrt(v) : v;
-        break;
-      }
-      case LDF_AGG_PRODUCT:
-        ok = c > 0;
-        static_cast<uint64_t*>(out)[g] = ok ? f.a0[s] : 0;
-        break;
-      case LDF_AGG_MIN:
-      case LDF_AGG_MAX: {
-        ok         = c > 0;
-        uint64_t o = f.a0[s];
-        uint64_t ident = (f.kind == LDF_AGG_MIN) ? ~0ull : 0ull;
-        bool fl    = f.in_type == LDF_TYPE_DOUBLE || f.in_type == LDF_TYPE_FLOAT;
-        uint64_t bits;
-        if (fl) {
-          double d;
-          if (o == ident) {
-            d = __longlong_as_double(0x7ff8000000000000ll);  // only NaN seen
-          } else {
-            uint64_t b = (o >> 63) ? (o & 0x7fffffffffffffffull) : ~o;
-            d          = __longlong_as_double((long long)b);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < ng;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g      = base + lane;
+    const bool in        = g < ng;
+    const int64_t s      = in ? slot_of_group[g] : 0;
+    const int64_t nrange = ng - base < 64 ? ng - base : 64;
+    for (int k = 0; k < fp.n; ++k) {
+      const OutDev& o = fp.o[k];
+      bool ok         = true;
+      if (in) {
+        uint64_t c = o.vcount ? o.vcount[s] : 0;
+        switch (o.kind) {
+          case -1: {  // key
+            uint64_t key = (s == (int64_t)fp.cap) ? kGEmpty : (s == (int64_t)fp.cap + 1 ? 0 : fp.slots[s]);
+            ok           = s != (int64_t)fp.cap + 1;
+            store_bits(o.out, o.out_type, g, key);
+            break;
           }
-          if (f.in_type == LDF_TYPE_FLOAT) {
-            float x = (float)d;
-            bits    = (uint32_t)__float_as_uint(x);
-          } else {
-            bits = (uint64_t)__double_as_longlong(d);
+          case LDF_AGG_COUNT: {
+            uint64_t v = o.mode == LDF_COUNT_ALL ? o.rows[s] : (o.mode == LDF_COUNT_ONLY_NULL ? o.rows[s] - c : c);
+            static_cast<int64_t*>(o.out)[g] = (int64_t)v;
+            break;
           }
-        } else if (f.in_type == LDF_TYPE_BOOL || f.in_type == LDF_TYPE_UINT8 || f.in_type == LDF_TYPE_UINT16 ||
-                   f.in_type == LDF_TYPE_UINT32 || f.in_type == LDF_TYPE_UINT64) {
-          bits = o;
-        } else {
-          bits = o ^ 0x8000000000000000ull;
+          case LDF_AGG_COUNT_ALL: static_cast<int64_t*>(o.out)[g] = (int64_t)o.rows[s]; break;
+          case LDF_AGG_COUNT_DISTINCT: {
+            bool has_null = o.rows && o.rows[s] > c;
+            int64_t v     = (int64_t)o.main[s];
+            if (o.mode == LDF_COUNT_ALL) v += has_null ? 1 : 0;
+            if (o.mode == LDF_COUNT_ONLY_NULL) v = has_null ? 1 : 0;
+            static_cast<int64_t*>(o.out)[g] = v;
+            break;
+          }
+          case LDF_AGG_SUM:
+            ok = c > 0;
+            if (o.out_type == LDF_TYPE_DOUBLE) {
+              double v = !ok ? 0.0 : (o.use_fsum ? o.fsum[s] : fixed192_to_double(o.main[s], o.l1[s], o.l2[s]));
+              static_cast<double*>(o.out)[g] = v;
+            } else {
+              static_cast<uint64_t*>(o.out)[g] = ok ? o.main[s] : 0;
+            }
+            break;
+          case LDF_AGG_MEAN: {
+            ok       = c > 0;
+            double v = !ok ? 0.0 : (o.use_fsum ? o.fsum[s] : fixed192_to_double(o.main[s], o.l1[s], o.l2[s]));
+            static_cast<double*>(o.out)[g] = ok ? v / (double)c : 0.0;
+            break;
+          }
+          case LDF_AGG_VARIANCE:
+          case LDF_AGG_STDDEV: {
+            ok                             = c > 0;
+            double v                       = ok ? o.fsum[s] / (double)c : 0.0;
+            static_cast<double*>(o.out)[g] = o.kind == LDF_AGG_STDDEV ? sqrt(v) : v;
+            break;
+          }
+          case LDF_AGG_PRODUCT:
+            ok                               = c > 0;
+            static_cast<uint64_t*>(o.out)[g] = ok ? o.main[s] : 0;
+            break;
+          case LDF_AGG_MIN:
+          case LDF_AGG_MAX: {
+            ok              = c > 0;
+            uint64_t ord    = o.main[s];
+            uint64_t ident  = o.kind == LDF_AGG_MIN ? ~0ull : 0ull;
+            const bool fl   = o.in_type == LDF_TYPE_DOUBLE || o.in_type == LDF_TYPE_FLOAT;
+            uint64_t bits;
+            if (fl) {
+              double d;
+              if (ord == ident) d = __longlong_as_double(0x7ff8000000000000ll);  // only NaN seen
+              else d = __longlong_as_double((long long)((ord >> 63) ? (ord & 0x7fffffffffffffffull) : ~ord));
+              bits = o.in_type == LDF_TYPE_FLOAT ? (uint64_t)__float_as_uint((float)d) : (uint64_t)__double_as_longlong(d);
+            } else if (is_unsigned_type(o.in_type)) {
+              bits = ord;
+            } else {
+              bits = ord ^ 0x8000000000000000ull;
+            }
+            store_bits(o.out, o.out_type, g, ok ? bits : 0);
+            break;
+          }
+          default: break;
         }
-        if (!ok) bits = 0;
-        switch (f.out_type) {
-          case LDF_TYPE_BOOL:
-          case LDF_TYPE_UINT8:
-          case LDF_TYPE_INT8: static_cast<uint8_t*>(out)[g] = (uint8_t)bits; break;
-          case LDF_TYPE_UINT16:
-          case LDF_TYPE_INT16: static_cast<uint16_t*>(out)[g] = (uint16_t)bits; break;
-          case LDF_TYPE_UINT32:
-          case LDF_TYPE_INT32:
-          case LDF_TYPE_FLOAT:
-          case LDF_TYPE_DATE32: static_cast<uint32_t*>(out)[g] = (uint32_t)bits; break;
-          default: static_cast<uint64_t*>(out)[g] = bits; break;
-        }
-        break;
+      } else {
+        ok = false;
       }
-      default: break;
+      if (o.valid != nullptr) {
+        uint64_t b = __ballot(ok);
+        if (lane == 0) {
+          o.valid[base >> 6] = b;
+          uint64_t z         = (uint64_t)nrange - (uint64_t)__popcll(b);
+          if (z) atomicAdd(o.nulls, (unsigned long long)z);
+        }
+      }
     }
-    if (valid) valid[g] = ok;
   }
 }
 
-__global__ void k_gb_means(const unsigned long long* a0, const unsigned long long* a1, const unsigned long long* a2,
-                           const unsigned long long* cnt, const double* fsum, int use_fsum, int64_t nslot, double* mean)
-{
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
-    unsigned long long c = cnt[i];
-    double s             = use_fsum ? fsum[i] : fixed192_to_double(a0[i], a1[i], a2[i]);
-    mean[i]              = c ? s / (double)c : 0.0;
-  }
-}
+// segment fill: seg[i] = (offset, length, value) over one u64 buffer
+struct FillSeg {
+  uint64_t off, len, val;
+};
+constexpr int kMaxFillSeg = 40;
+struct FillPlan {
+  FillSeg s[kMaxFillSeg];
+  int32_t n;
+};
 
-__global__ void k_pack_bools(const uint8_t* __restrict__ b, int64_t n, uint8_t* __restrict__ mask, unsigned long long* nulls)
+__global__ void k_fill_segments(unsigned long long* buf, FillPlan fp)
 {
-  uint64_t z     = 0;
-  int64_t nbytes = (n + 7) / 8;
-  for (int64_t byte = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; byte < nbytes; byte += (int64_t)gridDim.x * blockDim.x) {
-    uint8_t m = 0;
-    for (int k = 0; k < 8; ++k) {
-      int64_t i = byte * 8 + k;
-      if (i < n) {
-        if (b[i]) m |= (uint8_t)(1u << k);
-        else ++z;
-      }
-    }
-    mask[byte] = m;
-  }
-  if (z) atomicAdd(nulls, (unsigned long long)z);
+  const int seg = blockIdx.y;
+  if (seg >= fp.n) return;
+  const FillSeg f = fp.s[seg];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < f.len; i += (uint64_t)gridDim.x * blockDim.x)
+    buf[f.off + i] = f.val;
 }
 
 // ---- host --------------------------------------------------------------------------------
@@ -676,63 +975,115 @@ static int32_t agg_out_type(int32_t kind, int32_t in_type)
   }
 }
 
-struct AggState {
-  Buf<unsigned long long> a0, a1, a2, cnt, pset;
-  Buf<uint8_t> flag;
-  Buf<double> fsum, mean;
-  Buf<int32_t> err;
-  uint64_t pmask = 0;
-  int use_fsum   = 0;
+struct HostAcc {
+  int32_t kind;
+  int32_t col;  // input column of the table (-1 for ACC_ROWS)
+  bool global_rows = true;
 };
 
-template <bool GENERIC>
-static void launch_aggregate(const DKeys& keys, int64_t n, const KeyTab& t, const AggBatch& b, hipStream_t s)
+struct Lowered {
+  std::vector<HostAcc> acc;
+  struct AggRef {
+    int main = -1, vcount = -1, rows = -1;
+  };
+  std::vector<AggRef> ref;
+};
+
+static int find_or_add(std::vector<HostAcc>& v, int32_t kind, int32_t col)
 {
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i].kind == kind && v[i].col == col) return (int)i;
+  v.push_back({kind, col});
+  return (int)v.size() - 1;
+}
+
+static Lowered lower(const ldf_table_view* tv, const ldf_agg* aggs, int32_t naggs)
+{
+  Lowered L;
+  L.ref.resize(naggs);
+  for (int i = 0; i < naggs; ++i) {
+    const ldf_agg& g = aggs[i];
+    auto& R          = L.ref[i];
+    const int c      = g.in_col;
+    switch (g.kind) {
+      case LDF_AGG_COUNT_ALL: R.rows = find_or_add(L.acc, ACC_ROWS, -1); break;
+      case LDF_AGG_COUNT:
+        if (g.count_mode != LDF_COUNT_ONLY_VALID) R.rows = find_or_add(L.acc, ACC_ROWS, -1);
+        if (g.count_mode != LDF_COUNT_ALL) R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        break;
+      case LDF_AGG_SUM:
+        R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        R.main   = find_or_add(L.acc, is_float(tv->cols[c].type_id) ? ACC_FSUM : ACC_ISUM, c);
+        break;
+      case LDF_AGG_MEAN:
+      case LDF_AGG_VARIANCE:
+      case LDF_AGG_STDDEV:
+        R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        R.main   = find_or_add(L.acc, ACC_FSUM, c);
+        break;
+      case LDF_AGG_MIN:
+      case LDF_AGG_MAX:
+        R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        R.main   = find_or_add(L.acc, g.kind == LDF_AGG_MIN ? ACC_MIN : ACC_MAX, c);
+        break;
+      case LDF_AGG_PRODUCT:
+        R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        R.main   = find_or_add(L.acc, ACC_PROD, c);
+        break;
+      case LDF_AGG_COUNT_DISTINCT:
+        R.main = find_or_add(L.acc, ACC_DISTINCT, c);
+        if (g.count_mode != LDF_COUNT_ONLY_VALID) {
+          R.rows   = find_or_add(L.acc, ACC_ROWS, -1);
+          R.vcount = find_or_add(L.acc, ACC_VCOUNT, c);
+        }
+        break;
+      default: break;
+    }
+  }
+  return L;
+}
+
+template <bool GENERIC>
+static void launch_main(const DKeys& keys, int64_t n, const KeyTab& t, const GbPlan& p, const HotList& hot, int nblk,
+                        hipStream_t s)
+{
+  static bool attr = false;
+  if (!attr) {
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main<GENERIC>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    attr = true;
+  }
   ProfScope prof("gb_aggregate", s);
-  hipLaunchKernelGGL(k_gb_aggregate<GENERIC>, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, s, keys, n, t, b);
+  hipLaunchKernelGGL(k_gb_main<GENERIC>, dim3(nblk), dim3(kGbThreads), p.lds_bytes, s, keys, n, t, p, hot);
   LDF_HIP_TRY(hipGetLastError());
 }
 
 template <bool GENERIC>
-static void launch_second(const DKeys& keys, int64_t n, const KeyTab& t, const AggDesc& d, int second, hipStream_t s)
+static void launch_second(const DKeys& keys, int64_t n, const KeyTab& t, const ColDev& c, double* acc, const double* mean,
+                          hipStream_t s)
 {
-  hipLaunchKernelGGL(k_gb_second_pass<GENERIC>, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, s, keys, n, t, d, second);
+  hipLaunchKernelGGL(k_gb_second_pass<GENERIC>, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, s, keys, n, t, c, acc, mean);
   LDF_HIP_TRY(hipGetLastError());
 }
 
-static uint64_t estimate_groups(const DKeys& keys, int64_t n, bool generic, Alloc& a)
+__global__ void k_gb_means(const unsigned long long* a0, const unsigned long long* a1, const unsigned long long* a2,
+                           const unsigned long long* cnt, const double* fsum, int use_fsum, int64_t nslot, double* mean)
 {
-  hipStream_t s = a.stream();
-  const int64_t S = n < 65536 ? n : 65536;
-  if (S == 0) return 1;
-  int64_t stride = n / S;
-  uint64_t scap  = next_pow2((uint64_t)S * 2);
-  Buf<unsigned long long> slots(a, scap + 2);
-  Buf<unsigned int> cnt(a, scap + 2);
-  Buf<unsigned long long> misc(a, 5);  // nkeys, overflow(int), d, f1, f2
-  fill_u64((uint64_t*)slots.get(), kGEmpty, scap + 2, s);
-  fill_u32(cnt.get(), 0, scap + 2, s);
-  fill_u64((uint64_t*)misc.get(), 0, 5, s);
-  KeyTab t{slots.get(), scap - 1, scap, misc.get(), reinterpret_cast<int32_t*>(misc.get() + 1)};
-  if (generic)
-    hipLaunchKernelGGL(k_gb_sample<true>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, t, cnt.get());
-  else
-    hipLaunchKernelGGL(k_gb_sample<false>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, t, cnt.get());
-  LDF_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_gb_sample_stats, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, cnt.get(), (int64_t)scap + 2,
-                     misc.get() + 2);
-  LDF_HIP_TRY(hipGetLastError());
-  unsigned long long h[3];
-  LDF_HIP_TRY(hipMemcpyAsync(h, misc.get() + 2, sizeof(h), hipMemcpyDeviceToHost, s));
-  LDF_HIP_TRY(hipStreamSynchronize(s));
-  double d = (double)h[0], f1 = (double)h[1];
-  if (S == n) return (uint64_t)d;
-  double est;
-  if (f1 > 0.5 * (double)S) est = d * (double)n / (double)S;  // mostly singletons: near-unique keys
-  else est = std::sqrt((double)n / (double)S) * f1 + (d - f1);  // GEE
-  if (est > (double)n) est = (double)n;
-  if (est < d) est = d;
-  return (uint64_t)est;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
+    unsigned long long c = cnt[i];
+    double s             = use_fsum ? fsum[i] : fixed192_to_double(a0[i], a1[i], a2[i]);
+    mean[i]              = c ? s / (double)c : 0.0;
+  }
+}
+
+static ColDev to_coldev(const ldf_column_view& c)
+{
+  ColDev d;
+  d.data    = c.data;
+  d.valid   = c.validity;
+  d.offsets = c.offsets;
+  d.type    = c.type_id;
+  d.width   = type_width(c.type_id);
+  return d;
 }
 
 static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys, const ldf_agg* aggs, int32_t naggs,
@@ -754,161 +1105,233 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     keys.c[i] = to_dcol(tv->cols[key_idx[i]]);
   }
   const bool generic = !(nkeys == 1 && keys.c[0].width > 0);
-
-  // validate aggregations
   for (int i = 0; i < naggs; ++i) {
     const ldf_agg& g = aggs[i];
     LDF_EXPECTS(g.kind >= LDF_AGG_SUM && g.kind <= LDF_AGG_COUNT_ALL, "Unsupported groupby aggregation");
     if (g.kind == LDF_AGG_APPROX_MEDIAN) throw Error(LDF_E_UNSUPPORTED, "approximate_median is not implemented");
     if (g.kind == LDF_AGG_COUNT_ALL) continue;
     LDF_EXPECTS(g.in_col >= 0 && g.in_col < tv->ncols, "groupby: aggregation input column out of range");
-    int32_t t = tv->cols[g.in_col].type_id;
     if (g.kind == LDF_AGG_COUNT || g.kind == LDF_AGG_COUNT_DISTINCT) {
       LDF_EXPECTS(g.count_mode >= LDF_COUNT_ONLY_VALID && g.count_mode <= LDF_COUNT_ALL, "Unsupported count mode");
       continue;
     }
-    if (type_width(t) == 0) throw Error(LDF_E_UNSUPPORTED, "groupby: numeric aggregation of a string column");
+    if (type_width(tv->cols[g.in_col].type_id) == 0)
+      throw Error(LDF_E_UNSUPPORTED, "groupby: numeric aggregation of a string column");
+  }
+  Lowered low = lower(tv, aggs, naggs);
+  const int nacc = (int)low.acc.size();
+
+  // ---- sample: table size estimate and hot keys ----
+  const int64_t S  = n < kSampleRows ? n : kSampleRows;
+  uint64_t scap    = next_pow2((uint64_t)std::max<int64_t>(2 * S, 64));
+  Buf<unsigned long long> sslots(a, scap + 2);
+  Buf<unsigned int> scnt(a, scap + 2);
+  Buf<unsigned long long> smisc(a, 2 + 3 + 64);  // nkeys, overflow, stats
+  fill_u64((uint64_t*)sslots.get(), kGEmpty, scap + 2, s);
+  fill_u32(scnt.get(), 0, scap + 2, s);
+  fill_u64((uint64_t*)smisc.get(), 0, 2 + 3 + 64, s);
+  {
+    KeyTab st{sslots.get(), scap - 1, scap, smisc.get(), reinterpret_cast<int32_t*>(smisc.get() + 1)};
+    if (S > 0) {
+      int64_t stride = n / S;
+      if (generic)
+        hipLaunchKernelGGL(k_gb_sample<true>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, st, scnt.get());
+      else
+        hipLaunchKernelGGL(k_gb_sample<false>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, st, scnt.get());
+      LDF_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_gb_sample_stats, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, scnt.get(), (int64_t)scap + 2,
+                         smisc.get() + 2);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+  }
+  unsigned long long stats[3 + 64] = {};
+  LDF_HIP_TRY(hipMemcpyAsync(stats, smisc.get() + 2, sizeof(stats), hipMemcpyDeviceToHost, s));
+  LDF_HIP_TRY(hipStreamSynchronize(s));
+  uint64_t est;
+  {
+    double d = (double)stats[0], f1 = (double)stats[1];
+    if (S == n) est = (uint64_t)d;
+    else {
+      double e = (f1 > 0.5 * (double)S) ? d * (double)n / (double)S : std::sqrt((double)n / (double)S) * f1 + (d - f1);
+      e        = std::min(std::max(e, d), (double)n);
+      est      = (uint64_t)e;
+    }
   }
 
-  uint64_t est = estimate_groups(keys, n, generic, a);
-  uint64_t cap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(2.5 * (double)est)));
-  uint64_t maxcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n + 2));
-  if (cap > maxcap) cap = maxcap;
-
-  for (int attempt = 0;; ++attempt) {
-    LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "groupby: too many groups");
-    const int64_t nslot = (int64_t)cap + 2;
-    Buf<unsigned long long> slots(a, nslot);
-    Buf<unsigned long long> misc(a, 2);
-    fill_u64((uint64_t*)slots.get(), kGEmpty, nslot, s);
-    fill_u64((uint64_t*)misc.get(), 0, 2, s);
-    KeyTab t{slots.get(), cap - 1, cap, misc.get(), reinterpret_cast<int32_t*>(misc.get() + 1)};
-
-    std::vector<AggState> st(naggs);
-    std::vector<AggDesc> desc(naggs);
-    Buf<int32_t> errs(a, naggs > 0 ? naggs : 1);
-    fill_u32(reinterpret_cast<uint32_t*>(errs.get()), 0, naggs > 0 ? naggs : 1, s);
-    for (int i = 0; i < naggs; ++i) {
-      const ldf_agg& g = aggs[i];
-      AggDesc& d       = desc[i];
-      std::memset(&d, 0, sizeof(d));
-      d.kind = g.kind;
-      d.mode = g.count_mode;
-      if (g.kind != LDF_AGG_COUNT_ALL) {
-        const ldf_column_view& c = tv->cols[g.in_col];
-        d.in_type                = c.type_id;
-        d.width                  = type_width(c.type_id);
-        d.data                   = c.data;
-        d.valid                  = c.validity;
-        d.offsets                = c.offsets;
+  // ---- LDS plan ----
+  GbPlan plan;
+  std::memset(&plan, 0, sizeof(plan));
+  std::vector<int32_t> colmap;  // plan column -> table column
+  for (auto& h : low.acc)
+    if (h.col >= 0 && std::find(colmap.begin(), colmap.end(), h.col) == colmap.end()) colmap.push_back(h.col);
+  LDF_EXPECTS(nacc <= kMaxAcc && (int)colmap.size() <= kMaxCols,
+              "groupby: more than 16 accumulators or 4 distinct input columns per call is not supported");
+  int acc_bytes = 8;  // rows + flags
+  for (auto& h : low.acc) {
+    switch (h.kind) {
+      case ACC_VCOUNT:
+      case ACC_DISTINCT: acc_bytes += 4; break;
+      case ACC_ISUM:
+      case ACC_MIN:
+      case ACC_MAX: acc_bytes += 8; break;
+      case ACC_FSUM: acc_bytes += 24; break;
+      default: break;
+    }
+  }
+  int hot_cap = 0;
+  if (!generic) {
+    const int budget = 140 * 1024;
+    for (int h = 2048; h >= 64; h >>= 1)
+      if (h * (acc_bytes + 20) + 64 <= budget) {
+        hot_cap = h;
+        break;
       }
-      d.err = errs.get() + i;
-      AggState& S = st[i];
-      S.cnt       = Buf<unsigned long long>(a, nslot);
-      fill_u64((uint64_t*)S.cnt.get(), 0, nslot, s);
-      d.cnt = S.cnt.get();
-      switch (g.kind) {
-        case LDF_AGG_SUM:
-        case LDF_AGG_MEAN:
-        case LDF_AGG_VARIANCE:
-        case LDF_AGG_STDDEV:
-          S.a0 = Buf<unsigned long long>(a, nslot);
-          fill_u64((uint64_t*)S.a0.get(), 0, nslot, s);
-          d.a0 = S.a0.get();
-          if (!(g.kind == LDF_AGG_SUM && !is_float(d.in_type))) {
-            S.a1 = Buf<unsigned long long>(a, nslot);
-            S.a2 = Buf<unsigned long long>(a, nslot);
-            fill_u64((uint64_t*)S.a1.get(), 0, nslot, s);
-            fill_u64((uint64_t*)S.a2.get(), 0, nslot, s);
-            d.a1 = S.a1.get();
-            d.a2 = S.a2.get();
-          }
-          break;
-        case LDF_AGG_PRODUCT:
-          S.a0 = Buf<unsigned long long>(a, nslot);
-          fill_u64((uint64_t*)S.a0.get(), is_float(d.in_type) ? 0x3ff0000000000000ull : 1ull, nslot, s);
-          d.a0 = S.a0.get();
-          break;
-        case LDF_AGG_MIN:
-        case LDF_AGG_MAX:
-          S.a0 = Buf<unsigned long long>(a, nslot);
-          fill_u64((uint64_t*)S.a0.get(), g.kind == LDF_AGG_MIN ? ~0ull : 0ull, nslot, s);
-          d.a0   = S.a0.get();
-          S.flag = Buf<uint8_t>(a, nslot);
-          fill_u8(S.flag.get(), 0, nslot, s);
-          d.flag = S.flag.get();
-          break;
-        case LDF_AGG_COUNT_DISTINCT: {
-          S.flag = Buf<uint8_t>(a, nslot);
-          fill_u8(S.flag.get(), 0, nslot, s);
-          d.flag        = S.flag.get();
-          uint64_t pcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
-          S.pset        = Buf<unsigned long long>(a, pcap);
-          fill_u64((uint64_t*)S.pset.get(), kGEmpty, pcap, s);
-          d.pset  = S.pset.get();
-          d.pmask = pcap - 1;
-          S.pmask = pcap - 1;
-          break;
-        }
+  }
+  unsigned int thr = 2;
+  {
+    uint64_t above = 0;
+    for (int c = 63; c >= 2; --c) {
+      above += stats[3 + c];
+      if (above > (uint64_t)hot_cap) {
+        thr = (unsigned)c + 1;
+        break;
+      }
+    }
+  }
+  int32_t off = 0;
+  auto take   = [&](int bytes) {
+    int32_t o = off;
+    off += (bytes + 15) & ~15;
+    return o;
+  };
+  if (hot_cap > 0) {
+    plan.tab_off   = take(2 * hot_cap * 8);
+    plan.tidx_off  = take(2 * hot_cap * 2);
+    plan.rows_off  = take(hot_cap * 4);
+    plan.flags_off = take(hot_cap * 4);
+  }
+  int flag_bits = 0;
+  for (int i = 0; i < nacc; ++i) {
+    AccDev& d = plan.a[i];
+    d.kind    = low.acc[i].kind;
+    d.col     = low.acc[i].col < 0 ? -1 : (int32_t)(std::find(colmap.begin(), colmap.end(), low.acc[i].col) - colmap.begin());
+    d.lds_off = -1;
+    if (hot_cap > 0) {
+      switch (d.kind) {
+        case ACC_VCOUNT:
+        case ACC_DISTINCT: d.lds_off = take(hot_cap * 4); break;
+        case ACC_ISUM:
+        case ACC_MIN:
+        case ACC_MAX: d.lds_off = take(hot_cap * 8); break;
+        case ACC_FSUM: d.lds_off = take(hot_cap * 24); break;
         default: break;
       }
     }
+    if (d.kind == ACC_MIN || d.kind == ACC_MAX) d.flag_bit = flag_bits++;
+  }
+  LDF_EXPECTS(flag_bits <= 32, "groupby: too many MIN/MAX aggregations");
+  plan.nacc      = nacc;
+  plan.ncol      = (int32_t)colmap.size();
+  plan.hot_cap   = hot_cap;
+  plan.lds_bytes = off;
+  for (size_t c = 0; c < colmap.size(); ++c) plan.col[c] = to_coldev(tv->cols[colmap[c]]);
 
-    // one pass over the rows (aggregations in batches of kMaxAggsPerLaunch)
+  // hot keys from the sample
+  Buf<unsigned long long> hot_keys(a, hot_cap > 0 ? hot_cap : 1);
+  Buf<int32_t> hot_special(a, hot_cap > 0 ? hot_cap : 1);
+  Buf<int32_t> hot_gslot(a, hot_cap > 0 ? hot_cap : 1);
+  Buf<unsigned int> hot_count(a, 1);
+  fill_u32(hot_count.get(), 0, 1, s);
+  HotList hot{hot_keys.get(), hot_special.get(), hot_gslot.get(), hot_count.get()};
+  if (hot_cap > 0 && S > 0) {
+    hipLaunchKernelGGL(k_gb_hot_select, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, sslots.get(), scnt.get(),
+                       (int64_t)scap, thr, hot_cap, hot);
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  sslots.reset();
+  scnt.reset();
+
+  uint64_t cap    = next_pow2(std::max<uint64_t>(1024, (uint64_t)(2.5 * (double)est)));
+  uint64_t maxcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n + 2));
+  if (cap > maxcap) cap = maxcap;
+
+  for (;;) {
+    LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "groupby: too many groups");
+    const int64_t nslot = (int64_t)cap + 2;
+    Buf<unsigned long long> slots(a, nslot);
+    fill_u64((uint64_t*)slots.get(), kGEmpty, nslot, s);
+    // status: [0] nkeys, [1] overflow (int), [2.. 2+nacc) err flags (int), then null counts
+    const int nstatus = 2 + nacc + nkeys + naggs + 1;
+    Buf<unsigned long long> status(a, nstatus);
+    fill_u64((uint64_t*)status.get(), 0, nstatus, s);
+    KeyTab t{slots.get(), cap - 1, cap, status.get(), reinterpret_cast<int32_t*>(status.get() + 1)};
+
+    // global accumulator arrays: one buffer, filled with one launch
+    std::vector<uint64_t> off0(nacc), off1(nacc, 0), off2(nacc, 0), offp(nacc, 0), offf(nacc, 0);
+    FillPlan fp;
+    fp.n            = 0;
+    uint64_t total  = 0;
+    auto seg        = [&](uint64_t len, uint64_t val) {
+      uint64_t o = total;
+      total += len;
+      if (fp.n > 0 && fp.s[fp.n - 1].val == val && fp.s[fp.n - 1].off + fp.s[fp.n - 1].len == o) fp.s[fp.n - 1].len += len;
+      else fp.s[fp.n++] = FillSeg{o, len, val};
+      return o;
+    };
+    uint64_t pcap_total = 0;
+    std::vector<uint64_t> pcap(nacc, 0);
+    for (int i = 0; i < nacc; ++i) {
+      const HostAcc& h = low.acc[i];
+      const bool fl    = h.col >= 0 && is_float(tv->cols[h.col].type_id);
+      uint64_t init    = 0;
+      if (h.kind == ACC_MIN) init = ~0ull;
+      if (h.kind == ACC_PROD) init = fl ? 0x3ff0000000000000ull : 1ull;
+      off0[i] = seg(nslot, init);
+      if (h.kind == ACC_FSUM) {
+        off1[i] = seg(nslot, 0);
+        off2[i] = seg(nslot, 0);
+      }
+      if (h.kind == ACC_MIN || h.kind == ACC_MAX) offf[i] = seg((nslot + 1) / 2, 0);  // u32 flags
+      if (h.kind == ACC_DISTINCT) {
+        pcap[i] = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
+        offp[i] = seg(pcap[i], kGEmpty);
+        pcap_total += pcap[i];
+      }
+      LDF_EXPECTS(fp.n < kMaxFillSeg, "groupby: too many accumulator segments");
+    }
+    Buf<unsigned long long> state(a, total > 0 ? total : 1);
+    if (total > 0) {
+      unsigned gx = (unsigned)std::min<uint64_t>(2048, (total / std::max(1, fp.n) + 255) / 256 + 1);
+      hipLaunchKernelGGL(k_fill_segments, dim3(gx, fp.n), dim3(256), 0, s, state.get(), fp);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+    unsigned long long* st = state.get();
+    for (int i = 0; i < nacc; ++i) {
+      AccDev& d = plan.a[i];
+      d.g0      = st + off0[i];
+      d.g1      = low.acc[i].kind == ACC_FSUM ? st + off1[i] : nullptr;
+      d.g2      = low.acc[i].kind == ACC_FSUM ? st + off2[i] : nullptr;
+      d.gflag   = (d.kind == ACC_MIN || d.kind == ACC_MAX) ? reinterpret_cast<uint32_t*>(st + offf[i]) : nullptr;
+      d.pset    = d.kind == ACC_DISTINCT ? st + offp[i] : nullptr;
+      d.pmask   = d.kind == ACC_DISTINCT ? pcap[i] - 1 : 0;
+      d.err     = reinterpret_cast<int32_t*>(status.get() + 2 + i);
+    }
+
+    // ---- the pass over the rows ----
+    if (hot_cap > 0) {
+      hipLaunchKernelGGL(k_gb_hot_insert, dim3(grid_for(hot_cap, 256)), dim3(256), 0, s, t, hot, hot_cap);
+      LDF_HIP_TRY(hipGetLastError());
+    }
     if (n > 0) {
-      if (naggs == 0) {
-        AggBatch b;
-        b.n = 0;
-        if (generic) launch_aggregate<true>(keys, n, t, b, s);
-        else launch_aggregate<false>(keys, n, t, b, s);
-      }
-      for (int i0 = 0; i0 < naggs; i0 += kMaxAggsPerLaunch) {
-        AggBatch b;
-        b.n = std::min(kMaxAggsPerLaunch, naggs - i0);
-        for (int j = 0; j < b.n; ++j) b.d[j] = desc[i0 + j];
-        if (generic) launch_aggregate<true>(keys, n, t, b, s);
-        else launch_aggregate<false>(keys, n, t, b, s);
-      }
-    }
-    int32_t ov = 0;
-    LDF_HIP_TRY(hipMemcpyAsync(&ov, misc.get() + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    std::vector<int32_t> herr(naggs > 0 ? naggs : 1);
-    LDF_HIP_TRY(hipMemcpyAsync(herr.data(), errs.get(), herr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipStreamSynchronize(s));
-    if (ov) {
-      LDF_EXPECTS(cap < maxcap, "groupby: key table overflow");
-      cap = std::min(maxcap, cap * 4);
-      continue;
-    }
-
-    // fallback FP64 sums and VARIANCE/STDDEV second passes
-    for (int i = 0; i < naggs; ++i) {
-      const ldf_agg& g = aggs[i];
-      AggDesc& d       = desc[i];
-      AggState& S      = st[i];
-      bool fpsum = (g.kind == LDF_AGG_SUM && is_float(d.in_type)) || g.kind == LDF_AGG_MEAN || g.kind == LDF_AGG_VARIANCE ||
-                   g.kind == LDF_AGG_STDDEV;
-      if (fpsum && herr[i]) {
-        S.fsum = Buf<double>(a, nslot);
-        fill_u64(reinterpret_cast<uint64_t*>(S.fsum.get()), 0, nslot, s);
-        d.fsum = S.fsum.get();
-        if (generic) launch_second<true>(keys, n, t, d, 0, s);
-        else launch_second<false>(keys, n, t, d, 0, s);
-        S.use_fsum = 1;
-      }
-      if (g.kind == LDF_AGG_VARIANCE || g.kind == LDF_AGG_STDDEV) {
-        S.mean = Buf<double>(a, nslot);
-        hipLaunchKernelGGL(k_gb_means, dim3(grid_for(nslot, 256)), dim3(256), 0, s, d.a0, d.a1, d.a2, d.cnt, d.fsum,
-                           S.use_fsum, nslot, S.mean.get());
-        LDF_HIP_TRY(hipGetLastError());
-        Buf<double> m2(a, nslot);
-        fill_u64(reinterpret_cast<uint64_t*>(m2.get()), 0, nslot, s);
-        d.fsum = m2.get();
-        d.mean = S.mean.get();
-        if (generic) launch_second<true>(keys, n, t, d, 1, s);
-        else launch_second<false>(keys, n, t, d, 1, s);
-        S.fsum = std::move(m2);
-      }
+      // one wave of resident blocks (256 CUs; two 1024-thread blocks per CU when LDS allows)
+      int per_cu = plan.lds_bytes > 0 ? std::max(1, std::min(2, (160 * 1024) / plan.lds_bytes)) : 2;
+      int nblk   = 256 * per_cu;
+      int64_t rpb = (n + nblk - 1) / nblk;
+      rpb         = (rpb + 1023) / 1024 * 1024;
+      nblk        = (int)((n + rpb - 1) / rpb);
+      plan.rows_per_block = rpb;
+      if (generic) launch_main<true>(keys, n, t, plan, hot, nblk, s);
+      else launch_main<false>(keys, n, t, plan, hot, nblk, s);
     }
 
     // compact occupied slots into groups
@@ -917,71 +1340,123 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     hipLaunchKernelGGL(k_gb_occupied, dim3(grid_for(nslot, 256)), dim3(256), 0, s, slots.get(), nslot, occ.get());
     LDF_HIP_TRY(hipGetLastError());
     exclusive_scan_u32_to_u64(occ.get(), pos.get(), nslot, a);
-    int64_t ng = (int64_t)read_u64(pos.get() + nslot, s);
+    LDF_HIP_TRY(hipMemcpyAsync(status.get() + nstatus - 1, pos.get() + nslot, 8, hipMemcpyDeviceToDevice, s));
+    std::vector<unsigned long long> hs(nstatus);
+    LDF_HIP_TRY(hipMemcpyAsync(hs.data(), status.get(), nstatus * 8, hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    if ((int32_t)hs[1]) {
+      LDF_EXPECTS(cap < maxcap, "groupby: key table overflow");
+      cap = std::min(maxcap, cap * 4);
+      continue;
+    }
+    const int64_t ng = (int64_t)hs[nstatus - 1];
+
+    // FP64 fallback for sums that left the fixed-point range; VARIANCE/STDDEV second pass
+    std::vector<Buf<double>> fsum(nacc);
+    std::vector<int> use_fsum(nacc, 0);
+    for (int i = 0; i < nacc; ++i) {
+      if (low.acc[i].kind != ACC_FSUM || !(int32_t)hs[2 + i]) continue;
+      fsum[i] = Buf<double>(a, nslot);
+      fill_u64(reinterpret_cast<uint64_t*>(fsum[i].get()), 0, nslot, s);
+      ColDev c = to_coldev(tv->cols[low.acc[i].col]);
+      if (generic) launch_second<true>(keys, n, t, c, fsum[i].get(), nullptr, s);
+      else launch_second<false>(keys, n, t, c, fsum[i].get(), nullptr, s);
+      use_fsum[i] = 1;
+    }
+    std::vector<Buf<double>> m2(naggs);
+    for (int i = 0; i < naggs; ++i) {
+      if (aggs[i].kind != LDF_AGG_VARIANCE && aggs[i].kind != LDF_AGG_STDDEV) continue;
+      const int fi = low.ref[i].main, vi = low.ref[i].vcount;
+      Buf<double> mean(a, nslot);
+      hipLaunchKernelGGL(k_gb_means, dim3(grid_for(nslot, 256)), dim3(256), 0, s, plan.a[fi].g0, plan.a[fi].g1,
+                         plan.a[fi].g2, plan.a[vi].g0, fsum[fi].get(), use_fsum[fi], nslot, mean.get());
+      LDF_HIP_TRY(hipGetLastError());
+      m2[i] = Buf<double>(a, nslot);
+      fill_u64(reinterpret_cast<uint64_t*>(m2[i].get()), 0, nslot, s);
+      ColDev c = to_coldev(tv->cols[aggs[i].in_col]);
+      if (generic) launch_second<true>(keys, n, t, c, m2[i].get(), mean.get(), s);
+      else launch_second<false>(keys, n, t, c, m2[i].get(), mean.get(), s);
+    }
+
     Buf<int32_t> sog(a, ng > 0 ? ng : 1);
     Buf<int32_t> rep;
     if (generic) rep = Buf<int32_t>(a, ng > 0 ? ng : 1);
-    hipLaunchKernelGGL(k_gb_compact, dim3(grid_for(nslot, 256)), dim3(256), 0, s, slots.get(), nslot, cap, pos.get(),
-                       generic, sog.get(), rep.get());
+    hipLaunchKernelGGL(k_gb_compact, dim3(grid_for(nslot, 256)), dim3(256), 0, s, slots.get(), nslot, pos.get(), generic,
+                       sog.get(), rep.get());
     LDF_HIP_TRY(hipGetLastError());
 
     out->ncols = nkeys + naggs;
     int done   = 0;
     try {
-      Buf<uint8_t> vbytes(a, ng > 0 ? ng : 1);
-      Buf<unsigned long long> nulls(a, 1);
-      auto pack_validity = [&](ldf_column* c) {
-        fill_u64(reinterpret_cast<uint64_t*>(nulls.get()), 0, 1, s);
-        if (ng > 0) {
-          hipLaunchKernelGGL(k_pack_bools, dim3(grid_for((ng + 7) / 8, 256)), dim3(256), 0, s, vbytes.get(), ng, c->validity,
-                             nulls.get());
+      OutPlan op;
+      std::memset(&op, 0, sizeof(op));
+      op.cap                      = cap;
+      op.slots                    = slots.get();
+      unsigned long long* nullcnt = status.get() + 2 + nacc;  // one per output column
+      auto flush_plan             = [&]() {
+        if (op.n > 0 && ng > 0) {
+          hipLaunchKernelGGL(k_gb_finalize, dim3(grid_for(ng, 256, 256 * 16)), dim3(256), 0, s, op, sog.get(), ng);
           LDF_HIP_TRY(hipGetLastError());
         }
-        c->null_count = (int64_t)read_u64(reinterpret_cast<const uint64_t*>(nulls.get()), s);
+        op.n = 0;
       };
-      // keys
+      auto add_out = [&](const OutDev& o) {
+        if (op.n == kMaxOut) flush_plan();
+        op.o[op.n++] = o;
+      };
       if (generic) {
-        for (int i = 0; i < nkeys; ++i, ++done)
-          gather_column(tv->cols[key_idx[i]], rep.get(), ng, false, a, &out->cols[done]);
+        for (int i = 0; i < nkeys; ++i, ++done) gather_column(tv->cols[key_idx[i]], rep.get(), ng, false, a, &out->cols[done]);
       } else {
         const ldf_column_view& kc = tv->cols[key_idx[0]];
-        bool nullable             = kc.validity != nullptr;
+        const bool nullable       = kc.validity != nullptr;
         alloc_column(kc.type_id, ng, nullable, 0, a, &out->cols[0]);
         done = 1;
-        if (ng > 0) {
-          hipLaunchKernelGGL(k_gb_write_key, dim3(grid_for(ng, 256)), dim3(256), 0, s, slots.get(), sog.get(), ng, cap,
-                             type_width(kc.type_id), out->cols[0].data, nullable ? vbytes.get() : (uint8_t*)nullptr);
-          LDF_HIP_TRY(hipGetLastError());
-        }
-        if (nullable) pack_validity(&out->cols[0]);
+        OutDev o;
+        std::memset(&o, 0, sizeof(o));
+        o.kind     = -1;
+        o.out_type = kc.type_id;
+        o.out      = out->cols[0].data;
+        o.valid    = nullable ? reinterpret_cast<uint64_t*>(out->cols[0].validity) : nullptr;
+        o.nulls    = nullcnt + 0;
+        add_out(o);
       }
-      // aggregations
       for (int i = 0; i < naggs; ++i, ++done) {
-        const ldf_agg& g = aggs[i];
-        int32_t in_type  = g.kind == LDF_AGG_COUNT_ALL ? LDF_TYPE_INT64 : tv->cols[g.in_col].type_id;
-        int32_t ot       = agg_out_type(g.kind, in_type);
-        bool nullable    = !(g.kind == LDF_AGG_COUNT_ALL || g.kind == LDF_AGG_COUNT_DISTINCT);
+        const ldf_agg& g   = aggs[i];
+        const int32_t in_t = g.kind == LDF_AGG_COUNT_ALL ? LDF_TYPE_INT64 : tv->cols[g.in_col].type_id;
+        const int32_t ot   = agg_out_type(g.kind, in_t);
+        const bool nullable = !(g.kind == LDF_AGG_COUNT_ALL || g.kind == LDF_AGG_COUNT_DISTINCT);
         alloc_column(ot, ng, nullable, 0, a, &out->cols[done]);
-        FinalDesc f;
-        std::memset(&f, 0, sizeof(f));
-        f.kind     = g.kind;
-        f.in_type  = in_type;
-        f.out_type = ot;
-        f.mode     = g.count_mode;
-        f.a0       = desc[i].a0;
-        f.a1       = desc[i].a1;
-        f.a2       = desc[i].a2;
-        f.cnt      = desc[i].cnt;
-        f.flag     = desc[i].flag;
-        f.fsum     = st[i].fsum.get();
-        f.use_fsum = st[i].use_fsum;
-        if (ng > 0) {
-          hipLaunchKernelGGL(k_gb_finalize, dim3(grid_for(ng, 256)), dim3(256), 0, s, f, sog.get(), ng,
-                             out->cols[done].data, nullable ? vbytes.get() : (uint8_t*)nullptr);
-          LDF_HIP_TRY(hipGetLastError());
+        const auto& R = low.ref[i];
+        OutDev o;
+        std::memset(&o, 0, sizeof(o));
+        o.kind     = g.kind;
+        o.mode     = g.count_mode;
+        o.in_type  = in_t;
+        o.out_type = ot;
+        if (R.main >= 0) {
+          o.main = plan.a[R.main].g0;
+          o.l1   = plan.a[R.main].g1;
+          o.l2   = plan.a[R.main].g2;
+          o.flag = plan.a[R.main].gflag;
+          if (low.acc[R.main].kind == ACC_FSUM) {
+            o.use_fsum = use_fsum[R.main];
+            o.fsum     = fsum[R.main].get();
+          }
         }
-        if (nullable) pack_validity(&out->cols[done]);
+        if (g.kind == LDF_AGG_VARIANCE || g.kind == LDF_AGG_STDDEV) o.fsum = m2[i].get();
+        o.vcount = R.vcount >= 0 ? plan.a[R.vcount].g0 : nullptr;
+        o.rows   = R.rows >= 0 ? plan.a[R.rows].g0 : nullptr;
+        o.out    = out->cols[done].data;
+        o.valid  = nullable ? reinterpret_cast<uint64_t*>(out->cols[done].validity) : nullptr;
+        o.nulls  = nullcnt + done;
+        add_out(o);
       }
+      flush_plan();
+      std::vector<unsigned long long> nc(nkeys + naggs);
+      LDF_HIP_TRY(hipMemcpyAsync(nc.data(), nullcnt, nc.size() * 8, hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      for (int i = generic ? nkeys : 0; i < nkeys + naggs; ++i)
+        out->cols[i].null_count = out->cols[i].validity ? (int64_t)nc[i] : 0;
     } catch (...) {
       for (int i = 0; i < done; ++i) free_column(&out->cols[i], a);
       out->ncols = 0;
