@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
@@ -75,6 +76,8 @@ struct AccDev {
   unsigned long long* pset;
   uint64_t pmask;
   int32_t* err;  // FSUM: a value left the fixed-point range
+  unsigned long long* pset2;  // DISTINCT pairs of keys outside the hot set (partitioned path)
+  uint64_t pmask2;
 };
 
 struct ColDev {
@@ -83,6 +86,13 @@ struct ColDev {
   const int64_t* offsets;
   int32_t type;
   int32_t width;
+};
+
+// Where the per-slot LDS accumulators of a table of `cap` slots live (byte offsets).
+struct LdsLayout {
+  int32_t rows_off;           // u32 rows [cap]
+  int32_t flags_off;          // u32 NaN flags [cap]
+  int32_t acc_off[kMaxAcc];   // per accumulator, -1 = none
 };
 
 struct GbPlan {
@@ -115,6 +125,36 @@ struct KeyTab {
 };
 
 // ---- key table --------------------------------------------------------------------------
+// slot of a single fixed-width key given by its bits (kv = the key is valid)
+__device__ __forceinline__ int64_t gb_insert_bits(unsigned long long key, bool kv, const KeyTab& t)
+{
+  if (!kv) {
+    if (t.slots[t.cap + 1] == kGEmpty) t.slots[t.cap + 1] = 0;  // mark used (idempotent)
+    return (int64_t)t.cap + 1;
+  }
+  if (key == kGEmpty) {
+    if (t.slots[t.cap] == kGEmpty) t.slots[t.cap] = 0;
+    return (int64_t)t.cap;
+  }
+  uint64_t i = mix64(key) & t.mask;
+  for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+    unsigned long long cur = atomic_load_relaxed(&t.slots[i]);
+    if (cur == key) return (int64_t)i;
+    if (cur == kGEmpty) {
+      unsigned long long old = atomicCAS(&t.slots[i], kGEmpty, key);
+      if (old == kGEmpty) {
+        unsigned long long nk = atomicAdd(t.nkeys, 1ull);
+        if (nk + 1 > (t.cap / 4) * 3) *t.overflow = 1;
+        return (int64_t)i;
+      }
+      if (old == key) return (int64_t)i;
+    }
+    i = (i + 1) & t.mask;
+  }
+  *t.overflow = 1;
+  return -1;
+}
+
 template <bool GENERIC>
 __device__ __forceinline__ int64_t gb_find_or_insert(const DKeys& k, int64_t r, const KeyTab& t)
 {
@@ -249,6 +289,41 @@ __device__ __forceinline__ void add_fixed192(unsigned long long* a0, unsigned lo
   if (y2 != 0) atomicAdd(a2, (unsigned long long)y2);
 }
 
+// LDS form of an exact sum: six u64 words, word k accumulating 32-bit chunk k of the 192-bit
+// two's-complement addends.  The adds need no return value (2^32 addends cannot overflow a
+// word); lds_fsum_value folds the words back into three 64-bit limbs once, at flush time.
+__device__ __forceinline__ void lds_fsum_add(unsigned long long* w, uint64_t l0, uint64_t l1, uint64_t l2)
+{
+  if ((uint32_t)l0) atomicAdd(&w[0], (unsigned long long)(uint32_t)l0);
+  if (l0 >> 32) atomicAdd(&w[1], (unsigned long long)(l0 >> 32));
+  if ((uint32_t)l1) atomicAdd(&w[2], (unsigned long long)(uint32_t)l1);
+  if (l1 >> 32) atomicAdd(&w[3], (unsigned long long)(l1 >> 32));
+  if ((uint32_t)l2) atomicAdd(&w[4], (unsigned long long)(uint32_t)l2);
+  if (l2 >> 32) atomicAdd(&w[5], (unsigned long long)(l2 >> 32));
+}
+
+__device__ __forceinline__ void add192(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t a0, uint64_t a1, uint64_t a2)
+{
+  uint64_t t0 = r0 + a0, c0 = t0 < r0;
+  uint64_t t1 = r1 + a1, c1 = t1 < r1;
+  uint64_t t1c = t1 + c0;
+  c1 |= t1c < t1;
+  r0 = t0;
+  r1 = t1c;
+  r2 = r2 + a2 + c1;
+}
+
+__device__ __forceinline__ void lds_fsum_value(const unsigned long long* w, uint64_t& r0, uint64_t& r1, uint64_t& r2)
+{
+  r0 = r1 = r2 = 0;
+  add192(r0, r1, r2, w[0], 0, 0);
+  add192(r0, r1, r2, w[1] << 32, w[1] >> 32, 0);
+  add192(r0, r1, r2, 0, w[2], 0);
+  add192(r0, r1, r2, 0, w[3] << 32, w[3] >> 32);
+  add192(r0, r1, r2, 0, 0, w[4]);
+  add192(r0, r1, r2, 0, 0, w[5] << 32);
+}
+
 // Correctly rounded double of a 192-bit two's-complement integer times 2^-96.
 __device__ inline double fixed192_to_double(uint64_t l0, uint64_t l1, uint64_t l2)
 {
@@ -365,14 +440,10 @@ __device__ inline bool distinct_insert(const AccDev& a, const ColDev& c, const D
   if (c.width > 0 && c.width <= 4) {
     unsigned long long pk = ((unsigned long long)(uint32_t)slot << 32) | (uint32_t)vbits;
     uint64_t i            = mix64(pk) & a.pmask;
-    while (true) {
-      unsigned long long cur = atomic_load_relaxed(&a.pset[i]);
-      if (cur == pk) return false;
-      if (cur == kGEmpty) {
-        unsigned long long old = atomicCAS(&a.pset[i], kGEmpty, pk);
-        if (old == kGEmpty) return true;
-        if (old == pk) return false;
-      }
+    while (true) {  // most pairs are new: try the CAS first, one round trip
+      unsigned long long old = atomicCAS(&a.pset[i], kGEmpty, pk);
+      if (old == kGEmpty) return true;
+      if (old == pk) return false;
       i = (i + 1) & a.pmask;
     }
   }
@@ -483,6 +554,7 @@ __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, con
                                         int64_t s, const RowVals v)
 {
   uint32_t* rows = reinterpret_cast<uint32_t*>(lds + p.rows_off);
+  // DISTINCT pairs are keyed by the global slot here
   atomicAdd(&rows[h], 1u);
   for (int i = 0; i < p.nacc; ++i) {
     const AccDev& a = p.a[i];
@@ -500,8 +572,7 @@ __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, con
       case ACC_FSUM: {
         uint64_t l0, l1, l2;
         unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
-        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2))
-          add_fixed192(&base[3 * h], &base[3 * h + 1], &base[3 * h + 2], l0, l1, l2);
+        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) lds_fsum_add(&base[6 * h], l0, l1, l2);
         else *a.err = 1;
         break;
       }
@@ -553,7 +624,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
       if (a.kind == ACC_VCOUNT || a.kind == ACC_DISTINCT) {
         for (int i = threadIdx.x; i < H; i += blockDim.x) reinterpret_cast<uint32_t*>(lds + a.lds_off)[i] = 0;
       } else {
-        const int words          = a.kind == ACC_FSUM ? 3 * H : H;
+        const int words          = a.kind == ACC_FSUM ? 6 * H : H;
         const unsigned long long v = a.kind == ACC_MIN ? ~0ull : 0ull;
         for (int i = threadIdx.x; i < words; i += blockDim.x) reinterpret_cast<unsigned long long*>(lds + a.lds_off)[i] = v;
       }
@@ -672,8 +743,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
             break;
           }
           case ACC_FSUM: {
-            const unsigned long long* b = reinterpret_cast<const unsigned long long*>(lds + a.lds_off) + 3 * h;
-            if (b[0] | b[1] | b[2]) add_fixed192(&a.g0[s], &a.g1[s], &a.g2[s], b[0], b[1], b[2]);
+            uint64_t b0, b1, b2;
+            lds_fsum_value(reinterpret_cast<const unsigned long long*>(lds + a.lds_off) + 6 * h, b0, b1, b2);
+            if (b0 | b1 | b2) add_fixed192(&a.g0[s], &a.g1[s], &a.g2[s], b0, b1, b2);
             break;
           }
           case ACC_MIN:
@@ -686,6 +758,558 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
           }
           default: break;
         }
+      }
+    }
+  }
+}
+
+// ---- partitioned path for keys outside the hot set ---------------------------------------
+// Rows whose key is not hot are not aggregated with global atomics: the main pass scatters them
+// (key, value bits, a flags byte) into kNumParts hash partitions, LDS-staged so each partition
+// receives contiguous runs, and k_gb_part3 aggregates every (partition, sub-pass) in one block's
+// LDS table.  Each key therefore lives in exactly one place -- a hot LDS slot, a partition
+// block's LDS slot, or (only when a partition block's table is full) the global table -- and
+// the final values are stored, not added, into the global slot arrays.
+constexpr int kPartBits = 10;
+constexpr int kNumParts = 1 << kPartBits;
+constexpr int kSub3     = kGbThreads * kGbRowsPerThread;  // rows staged per step
+constexpr int kFlagKeyValid = 1 << 4;
+constexpr int kFlagDistinct = 5;  // bits 5..7: the row's (key, value) pair was new, per DISTINCT
+constexpr int kPartProbes   = 64;
+
+struct PartBufs {
+  void* key;
+  void* val[kMaxCols];
+  uint8_t* flags;           // bits 0-3 value valid, bit 4 key valid, bits 5-7 new distinct pair
+  const uint64_t* offsets;  // exclusive scan of counts[part][blk]
+  int32_t key_width;
+  int32_t nblk;
+};
+
+struct HotParts {  // per-block partials of the hot slots, index blk * H + h (FSUM: 3 words)
+  uint32_t* rows;
+  uint32_t* flags;
+  unsigned long long* acc[kMaxAcc];
+};
+
+struct PartPlan {
+  int32_t tcap;       // LDS table slots (power of two)
+  int32_t S;          // sub-passes per partition
+  int32_t key_off;    // u64 keys [tcap]
+  int32_t rows_off;   // u32 [tcap]
+  int32_t flags_off;  // u32 [tcap]
+  int32_t acc_off[kMaxAcc];
+  int32_t lds_bytes;
+};
+
+__device__ __forceinline__ uint32_t cold_part(uint64_t key)
+{
+  return (uint32_t)(mix64(key ^ 0x5bd1e9955bd1e995ull) >> (64 - kPartBits));
+}
+__device__ __forceinline__ uint32_t cold_sub(uint64_t key, int S)
+{
+  return (uint32_t)(mix64(key + 0x2545f4914f6cdd1dull) % (uint64_t)S);
+}
+
+// Builds the LDS table of hot keys (and clears the hot accumulators when `accs`).
+__device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
+                                 int* s_null_hot, int* s_sent_hot)
+{
+  const int H               = p.hot_cap;
+  const int T               = 2 * H;
+  unsigned long long* tkeys = reinterpret_cast<unsigned long long*>(lds + p.tab_off);
+  uint16_t* tidx            = reinterpret_cast<uint16_t*>(lds + p.tidx_off);
+  for (int i = threadIdx.x; i < T; i += blockDim.x) tkeys[i] = kGEmpty;
+  if (accs) {
+    for (int i = threadIdx.x; i < H; i += blockDim.x) {
+      reinterpret_cast<uint32_t*>(lds + p.rows_off)[i]  = 0;
+      reinterpret_cast<uint32_t*>(lds + p.flags_off)[i] = 0;
+    }
+    for (int j = 0; j < p.nacc; ++j) {
+      const AccDev& a = p.a[j];
+      if (a.lds_off < 0) continue;
+      if (a.kind == ACC_VCOUNT || a.kind == ACC_DISTINCT) {
+        for (int i = threadIdx.x; i < H; i += blockDim.x) reinterpret_cast<uint32_t*>(lds + a.lds_off)[i] = 0;
+      } else {
+        const int words            = a.kind == ACC_FSUM ? 6 * H : H;
+        const unsigned long long v = a.kind == ACC_MIN ? ~0ull : 0ull;
+        for (int i = threadIdx.x; i < words; i += blockDim.x) reinterpret_cast<unsigned long long*>(lds + a.lds_off)[i] = v;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    *s_null_hot = -1;
+    *s_sent_hot = -1;
+  }
+  __syncthreads();
+  for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+    int sp = hot.special[h];
+    if (sp == 1) *s_sent_hot = h;
+    else if (sp == 2) *s_null_hot = h;
+    else {
+      unsigned long long k = hot.keys[h];
+      uint32_t i           = (uint32_t)(mix64(k) & (uint64_t)(T - 1));
+      while (true) {
+        unsigned long long old = atomicCAS(&tkeys[i], kGEmpty, k);
+        if (old == kGEmpty) {
+          tidx[i] = (uint16_t)h;
+          break;
+        }
+        i = (i + 1) & (T - 1);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* lds, uint64_t k, bool kv, int null_hot,
+                                          int sent_hot)
+{
+  if (!kv) return null_hot;
+  if (k == kGEmpty) return sent_hot;
+  const int T                     = 2 * p.hot_cap;
+  const unsigned long long* tkeys = reinterpret_cast<const unsigned long long*>(lds + p.tab_off);
+  uint32_t i                      = (uint32_t)(mix64(k) & (uint64_t)(T - 1));
+  while (true) {
+    unsigned long long cur = tkeys[i];
+    if (cur == k) return reinterpret_cast<const uint16_t*>(lds + p.tidx_off)[i];
+    if (cur == kGEmpty) return -1;
+    i = (i + 1) & (T - 1);
+  }
+}
+
+__device__ __forceinline__ void store_w(void* base, int width, int64_t i, uint64_t bits)
+{
+  switch (width) {
+    case 1: static_cast<uint8_t*>(base)[i] = (uint8_t)bits; break;
+    case 2: static_cast<uint16_t*>(base)[i] = (uint16_t)bits; break;
+    case 4: static_cast<uint32_t*>(base)[i] = (uint32_t)bits; break;
+    default: static_cast<uint64_t*>(base)[i] = bits; break;
+  }
+}
+
+// counts[part * nblk + blk] of the rows whose key is not hot
+__global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_null_hot, s_sent_hot;
+  __shared__ uint32_t hist[kNumParts];
+  const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
+  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) hist[i] = 0;
+  hot_setup(p, lds, hot, nhot, false, &s_null_hot, &s_sent_hot);
+  const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
+  const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
+  const DCol& kc      = keys.c[0];
+  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
+    const bool kv = valid_at(kc.valid, r);
+    uint64_t k    = kv ? load_bits(kc.data, kc.width, r) : 0;
+    if (hot_lookup(p, lds, k, kv, s_null_hot, s_sent_hot) < 0) atomicAdd(&hist[cold_part(k)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) counts[(int64_t)i * gridDim.x + blockIdx.x] = hist[i];
+}
+
+// Main pass: hot rows into LDS accumulators, other rows written straight into their partition:
+// each takes the next position of its partition's region with an LDS atomic on the region
+// cursor (no barrier per tile; order inside a partition is irrelevant because every
+// accumulator is exact and order-independent).  LDS: [0, p.lds_bytes) hot table +
+// accumulators, then cursor u64[kNumParts].
+__global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
+                                                         HotParts hp)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_null_hot, s_sent_hot;
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + p.lds_bytes);
+
+  const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
+  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
+  hot_setup(p, lds, hot, nhot, true, &s_null_hot, &s_sent_hot);
+
+  const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
+  const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
+  const DCol& kc      = keys.c[0];
+  for (int64_t sub = begin; sub < end; sub += kSub3) {
+    RowVals v[kGbRowsPerThread];
+    uint64_t kb[kGbRowsPerThread];
+    bool kv[kGbRowsPerThread];
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      v[j]            = RowVals{0, 0, 0, 0, 0};
+      kb[j]           = 0;
+      kv[j]           = false;
+      if (r < end) {
+        kv[j]                = valid_at(kc.valid, r);
+        kb[j]                = kv[j] ? load_bits(kc.data, kc.width, r) : 0;
+        uint32_t vm          = 0;
+        uint64_t x[kMaxCols] = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < kMaxCols; ++c) {
+          if (c < p.ncol) {
+            const ColDev& cd = p.col[c];
+            if (valid_at(cd.valid, r)) vm |= 1u << c;
+            x[c] = load_bits(cd.data, cd.width, r);
+          }
+        }
+        v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      if (r >= end) continue;
+      const int h = nhot > 0 ? hot_lookup(p, lds, kb[j], kv[j], s_null_hot, s_sent_hot) : -1;
+      if (h >= 0) {
+        acc_lds(p, lds, keys, r, h, hot.gslot[h], v[j]);
+        continue;
+      }
+      uint32_t fl = (v[j].valid & 0xF) | (kv[j] ? kFlagKeyValid : 0);
+      int di      = 0;
+      for (int i = 0; i < p.nacc; ++i) {
+        const AccDev& a = p.a[i];
+        if (a.kind != ACC_DISTINCT) continue;
+        if ((v[j].valid >> a.col) & 1) {
+          // (key, value) pair of a key outside the hot set: both fit 32 bits on this path
+          unsigned long long pk = ((unsigned long long)(uint32_t)kb[j] << 32) | (uint32_t)pick(v[j], a.col);
+          uint64_t q            = mix64(pk) & a.pmask2;
+          while (true) {  // try the CAS first: one round trip for a new pair
+            unsigned long long old = atomicCAS(&a.pset2[q], kGEmpty, pk);
+            if (old == kGEmpty) {
+              fl |= 1u << (kFlagDistinct + di);
+              break;
+            }
+            if (old == pk) break;
+            q = (q + 1) & a.pmask2;
+          }
+        }
+        ++di;
+      }
+      const int64_t dest = (int64_t)atomicAdd(&cursor[cold_part(kb[j])], 1ull);
+      store_w(pb.key, pb.key_width, dest, kb[j]);
+      if (p.ncol > 0) store_w(pb.val[0], p.col[0].width, dest, v[j].b0);
+      if (p.ncol > 1) store_w(pb.val[1], p.col[1].width, dest, v[j].b1);
+      if (p.ncol > 2) store_w(pb.val[2], p.col[2].width, dest, v[j].b2);
+      if (p.ncol > 3) store_w(pb.val[3], p.col[3].width, dest, v[j].b3);
+      pb.flags[dest] = (uint8_t)fl;
+    }
+  }
+  __syncthreads();
+
+  // hot partials of this block
+  const int H = p.hot_cap;
+  for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+    const int64_t o = (int64_t)blockIdx.x * H + h;
+    hp.rows[o]      = reinterpret_cast<const uint32_t*>(lds + p.rows_off)[h];
+    hp.flags[o]     = reinterpret_cast<const uint32_t*>(lds + p.flags_off)[h];
+    for (int j = 0; j < p.nacc; ++j) {
+      const AccDev& a = p.a[j];
+      if (a.lds_off < 0) continue;
+      if (a.kind == ACC_VCOUNT || a.kind == ACC_DISTINCT) {
+        hp.acc[j][o] = reinterpret_cast<const uint32_t*>(lds + a.lds_off)[h];
+      } else if (a.kind == ACC_FSUM) {
+        uint64_t b0, b1, b2;
+        lds_fsum_value(reinterpret_cast<const unsigned long long*>(lds + a.lds_off) + 6 * h, b0, b1, b2);
+        hp.acc[j][3 * o]     = b0;
+        hp.acc[j][3 * o + 1] = b1;
+        hp.acc[j][3 * o + 2] = b2;
+      } else {
+        hp.acc[j][o] = reinterpret_cast<const unsigned long long*>(lds + a.lds_off)[h];
+      }
+    }
+  }
+}
+
+// Combines the per-block partials of each hot key (one block per key) and stores the totals in
+// its global slot.
+__global__ __launch_bounds__(256) void k_gb_hot_reduce(GbPlan p, HotList hot, HotParts hp, int nblk)
+{
+  __shared__ unsigned long long red[3][256];
+  const int H    = p.hot_cap;
+  const int nhot = (int)min(*hot.count, (unsigned int)H);
+  for (int h = blockIdx.x; h < nhot; h += gridDim.x) {
+    const int64_t s = hot.gslot[h];
+    for (int j = -1; j < p.nacc; ++j) {
+      const int kind = j < 0 ? -1 : p.a[j].kind;
+      if (j >= 0 && (kind == ACC_PROD || p.a[j].lds_off < 0) && kind != ACC_ROWS) continue;
+      uint64_t x0 = 0, x1 = 0, x2 = 0;
+      if (kind == ACC_MIN) x0 = ~0ull;
+      for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const int64_t o = (int64_t)b * H + h;
+        switch (kind) {
+          case -1: x0 |= hp.flags[o]; break;
+          case ACC_ROWS: x0 += hp.rows[o]; break;
+          case ACC_FSUM: {
+            uint64_t a0 = hp.acc[j][3 * o], a1 = hp.acc[j][3 * o + 1], a2 = hp.acc[j][3 * o + 2];
+            uint64_t t0 = x0 + a0, c0 = t0 < x0;
+            uint64_t t1 = x1 + a1, c1 = t1 < x1;
+            uint64_t t1c = t1 + c0;
+            c1 |= t1c < t1;
+            x0 = t0;
+            x1 = t1c;
+            x2 = x2 + a2 + c1;
+            break;
+          }
+          case ACC_MIN: x0 = min(x0, (uint64_t)hp.acc[j][o]); break;
+          case ACC_MAX: x0 = max(x0, (uint64_t)hp.acc[j][o]); break;
+          default: x0 += hp.acc[j][o]; break;
+        }
+      }
+      red[0][threadIdx.x] = x0;
+      red[1][threadIdx.x] = x1;
+      red[2][threadIdx.x] = x2;
+      __syncthreads();
+      for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+          uint64_t a0 = red[0][threadIdx.x], b0 = red[0][threadIdx.x + w];
+          switch (kind) {
+            case -1: red[0][threadIdx.x] = a0 | b0; break;
+            case ACC_MIN: red[0][threadIdx.x] = min(a0, b0); break;
+            case ACC_MAX: red[0][threadIdx.x] = max(a0, b0); break;
+            case ACC_FSUM: {
+              uint64_t a1 = red[1][threadIdx.x], b1 = red[1][threadIdx.x + w];
+              uint64_t t0 = a0 + b0, c0 = t0 < a0;
+              uint64_t t1 = a1 + b1, c1 = t1 < a1;
+              uint64_t t1c = t1 + c0;
+              c1 |= t1c < t1;
+              red[0][threadIdx.x] = t0;
+              red[1][threadIdx.x] = t1c;
+              red[2][threadIdx.x] = red[2][threadIdx.x] + red[2][threadIdx.x + w] + c1;
+              break;
+            }
+            default: red[0][threadIdx.x] = a0 + b0; break;
+          }
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        if (kind == -1) {
+          for (int q = 0; q < p.nacc; ++q)
+            if ((p.a[q].kind == ACC_MIN || p.a[q].kind == ACC_MAX) && ((red[0][0] >> p.a[q].flag_bit) & 1))
+              p.a[q].gflag[s] = 1u;
+        } else {
+          p.a[j].g0[s] = red[0][0];
+          if (kind == ACC_FSUM) {
+            p.a[j].g1[s] = red[1][0];
+            p.a[j].g2[s] = red[2][0];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// accumulate a partition row into LDS slot `slot` (DISTINCT from the row's flag bits)
+__device__ __forceinline__ void acc_part(const GbPlan& p, const PartPlan& q, unsigned char* lds, int slot, const RowVals v,
+                                         uint32_t fl)
+{
+  atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.rows_off)[slot], 1u);
+  int di = 0;
+  for (int i = 0; i < p.nacc; ++i) {
+    const AccDev& a = p.a[i];
+    if (a.kind == ACC_ROWS) continue;
+    if (a.kind == ACC_DISTINCT) {
+      if ((fl >> (kFlagDistinct + di)) & 1) atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.acc_off[i])[slot], 1u);
+      ++di;
+      continue;
+    }
+    if (!((v.valid >> a.col) & 1)) continue;
+    const ColDev& c   = p.col[a.col];
+    const uint64_t vb = pick(v, a.col);
+    switch (a.kind) {
+      case ACC_VCOUNT: atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.acc_off[i])[slot], 1u); break;
+      case ACC_ISUM:
+        atomicAdd(&reinterpret_cast<unsigned long long*>(lds + q.acc_off[i])[slot],
+                  (unsigned long long)bits_as_int(vb, c.type, c.width));
+        break;
+      case ACC_FSUM: {
+        uint64_t l0, l1, l2;
+        unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
+        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) lds_fsum_add(&base[6 * slot], l0, l1, l2);
+        else *a.err = 1;
+        break;
+      }
+      case ACC_MIN:
+      case ACC_MAX: {
+        if (is_float_type(c.type) && isnan(bits_as_double(vb, c.type, c.width))) {
+          atomicOr(&reinterpret_cast<uint32_t*>(lds + q.flags_off)[slot], 1u << a.flag_bit);
+          break;
+        }
+        uint64_t o               = ordered_of(vb, c.type, c.width);
+        unsigned long long* cell = &reinterpret_cast<unsigned long long*>(lds + q.acc_off[i])[slot];
+        if (a.kind == ACC_MIN) atomicMin(cell, (unsigned long long)o);
+        else atomicMax(cell, (unsigned long long)o);
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+// the same into the global arrays at slot s (rows a full partition table could not take)
+__device__ __forceinline__ void acc_spill(const GbPlan& p, int64_t s, const RowVals v, uint32_t fl)
+{
+  int di = 0;
+  for (int i = 0; i < p.nacc; ++i) {
+    const AccDev& a = p.a[i];
+    if (a.kind == ACC_ROWS) {
+      atomicAdd(&a.g0[s], 1ull);
+      continue;
+    }
+    if (a.kind == ACC_DISTINCT) {
+      if ((fl >> (kFlagDistinct + di)) & 1) atomicAdd(&a.g0[s], 1ull);
+      ++di;
+      continue;
+    }
+    if (!((v.valid >> a.col) & 1)) continue;
+    const ColDev& c   = p.col[a.col];
+    const uint64_t vb = pick(v, a.col);
+    switch (a.kind) {
+      case ACC_VCOUNT: atomicAdd(&a.g0[s], 1ull); break;
+      case ACC_ISUM: atomicAdd(&a.g0[s], (unsigned long long)bits_as_int(vb, c.type, c.width)); break;
+      case ACC_FSUM: {
+        uint64_t l0, l1, l2;
+        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) add_fixed192(&a.g0[s], &a.g1[s], &a.g2[s], l0, l1, l2);
+        else *a.err = 1;
+        break;
+      }
+      case ACC_MIN:
+      case ACC_MAX: {
+        if (is_float_type(c.type) && isnan(bits_as_double(vb, c.type, c.width))) {
+          atomicOr(&a.gflag[s], 1u);
+          break;
+        }
+        uint64_t o = ordered_of(vb, c.type, c.width);
+        if (a.kind == ACC_MIN) atomicMin(&a.g0[s], (unsigned long long)o);
+        else atomicMax(&a.g0[s], (unsigned long long)o);
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+// One block per (partition, sub-pass); the S blocks of a partition share an XCD (block % 8) so
+// the partition's rows are re-read from that XCD's L2.
+__global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, PartBufs pb, KeyTab t)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int xcd   = blockIdx.x & 7;
+  const int rest  = blockIdx.x >> 3;
+  const int sub   = rest % q.S;
+  const int part  = (rest / q.S) * 8 + xcd;
+  if (part >= kNumParts) return;
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(lds + q.key_off);
+  for (int i = threadIdx.x; i < q.tcap; i += blockDim.x) {
+    keys[i]                                           = kGEmpty;
+    reinterpret_cast<uint32_t*>(lds + q.rows_off)[i]  = 0;
+    reinterpret_cast<uint32_t*>(lds + q.flags_off)[i] = 0;
+  }
+  for (int j = 0; j < p.nacc; ++j) {
+    const AccDev& a = p.a[j];
+    if (q.acc_off[j] < 0) continue;
+    if (a.kind == ACC_VCOUNT || a.kind == ACC_DISTINCT) {
+      for (int i = threadIdx.x; i < q.tcap; i += blockDim.x) reinterpret_cast<uint32_t*>(lds + q.acc_off[j])[i] = 0;
+    } else {
+      const int words            = a.kind == ACC_FSUM ? 6 * q.tcap : q.tcap;
+      const unsigned long long v = a.kind == ACC_MIN ? ~0ull : 0ull;
+      for (int i = threadIdx.x; i < words; i += blockDim.x) reinterpret_cast<unsigned long long*>(lds + q.acc_off[j])[i] = v;
+    }
+  }
+  __syncthreads();
+
+  const int64_t begin = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
+  const int64_t end   = (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
+  const uint32_t mask = (uint32_t)q.tcap - 1;
+  constexpr int R      = 4;
+  for (int64_t i0 = begin + threadIdx.x; i0 < end; i0 += (int64_t)R * blockDim.x) {
+    uint64_t kr[R];
+    uint32_t fr[R];
+    RowVals vr[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {  // issue the loads of R rows before using any
+      const int64_t i = i0 + (int64_t)j * blockDim.x;
+      kr[j]           = 0;
+      fr[j]           = 0;
+      vr[j]           = RowVals{0, 0, 0, 0, 0};
+      if (i < end) {
+        kr[j]                = load_bits(pb.key, pb.key_width, i);
+        fr[j]                = pb.flags[i];
+        uint64_t x[kMaxCols] = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < kMaxCols; ++c)
+          if (c < p.ncol) x[c] = load_bits(pb.val[c], p.col[c].width, i);
+        vr[j] = RowVals{x[0], x[1], x[2], x[3], fr[j] & 0xF};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int64_t i = i0 + (int64_t)j * blockDim.x;
+      if (i >= end) continue;
+      const uint64_t k = kr[j];
+      if (q.S > 1 && (int)cold_sub(k, q.S) != sub) continue;
+      const uint32_t fl = fr[j];
+      const bool kv     = fl & kFlagKeyValid;
+      // A key spills when the first kPartProbes slots of its probe sequence hold other keys.
+      // Slots never become empty again, so every thread reaches the same verdict for a key and a
+      // key is either in this table or spilled, never both.
+      int slot = -1;
+      if (kv && k != kGEmpty) {
+        uint32_t h = (uint32_t)(mix64(k) & mask);
+        for (int probe = 0; probe < kPartProbes; ++probe) {
+          unsigned long long cur = keys[h];
+          if (cur == k) {
+            slot = (int)h;
+            break;
+          }
+          if (cur == kGEmpty) {
+            unsigned long long old = atomicCAS(&keys[h], kGEmpty, (unsigned long long)k);
+            if (old == kGEmpty || old == k) {
+              slot = (int)h;
+              break;
+            }
+          }
+          h = (h + 1) & mask;
+        }
+      }
+      if (slot >= 0) {
+        acc_part(p, q, lds, slot, vr[j], fl);
+      } else {
+        const int64_t s = gb_insert_bits(k, kv, t);
+        if (s >= 0) acc_spill(p, s, vr[j], fl);
+      }
+    }
+  }
+  __syncthreads();
+  // store this block's groups in their global slots
+  for (int h = threadIdx.x; h < q.tcap; h += blockDim.x) {
+    const unsigned long long k = keys[h];
+    if (k == kGEmpty) continue;
+    const int64_t s = gb_insert_bits(k, true, t);
+    if (s < 0) continue;
+    const uint32_t nr = reinterpret_cast<const uint32_t*>(lds + q.rows_off)[h];
+    const uint32_t fg = reinterpret_cast<const uint32_t*>(lds + q.flags_off)[h];
+    for (int j = 0; j < p.nacc; ++j) {
+      const AccDev& a = p.a[j];
+      switch (a.kind) {
+        case ACC_ROWS: a.g0[s] = nr; break;
+        case ACC_VCOUNT:
+        case ACC_DISTINCT: a.g0[s] = reinterpret_cast<const uint32_t*>(lds + q.acc_off[j])[h]; break;
+        case ACC_ISUM: a.g0[s] = reinterpret_cast<const unsigned long long*>(lds + q.acc_off[j])[h]; break;
+        case ACC_FSUM: {
+          uint64_t b0, b1, b2;
+          lds_fsum_value(reinterpret_cast<const unsigned long long*>(lds + q.acc_off[j]) + 6 * h, b0, b1, b2);
+          a.g0[s] = b0;
+          a.g1[s] = b1;
+          a.g2[s] = b2;
+          break;
+        }
+        case ACC_MIN:
+        case ACC_MAX:
+          a.g0[s] = reinterpret_cast<const unsigned long long*>(lds + q.acc_off[j])[h];
+          if ((fg >> a.flag_bit) & 1) a.gflag[s] = 1u;
+          break;
+        default: break;
       }
     }
   }
@@ -723,16 +1347,22 @@ __global__ void k_gb_sample(DKeys keys, int64_t n, int64_t S, int64_t stride, Ke
 }
 
 // stats[0] = distinct, [1] = singletons, [2] = doubletons, [3 + c] = keys seen c times (c < 64)
-__global__ void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* stats)
+__global__ __launch_bounds__(256) void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* stats)
 {
+  __shared__ unsigned int loc[3 + 64];
+  for (int i = threadIdx.x; i < 3 + 64; i += blockDim.x) loc[i] = 0;
+  __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     unsigned c = cnt[i];
     if (c == 0) continue;
-    atomicAdd(&stats[0], 1ull);
-    if (c == 1) atomicAdd(&stats[1], 1ull);
-    if (c == 2) atomicAdd(&stats[2], 1ull);
-    atomicAdd(&stats[3 + (c < 63 ? c : 63)], 1ull);
+    atomicAdd(&loc[0], 1u);
+    if (c == 1) atomicAdd(&loc[1], 1u);
+    if (c == 2) atomicAdd(&loc[2], 1u);
+    atomicAdd(&loc[3 + (c < 63 ? c : 63)], 1u);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 + 64; i += blockDim.x)
+    if (loc[i]) atomicAdd(&stats[i], (unsigned long long)loc[i]);
 }
 
 __global__ void k_gb_hot_select(const unsigned long long* sslots, const unsigned int* cnt, int64_t scap, unsigned int thr,
@@ -1174,13 +1804,13 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       case ACC_ISUM:
       case ACC_MIN:
       case ACC_MAX: acc_bytes += 8; break;
-      case ACC_FSUM: acc_bytes += 24; break;
+      case ACC_FSUM: acc_bytes += 48; break;
       default: break;
     }
   }
   int hot_cap = 0;
   if (!generic) {
-    const int budget = 140 * 1024;
+    const int budget = 124 * 1024;  // leaves room for the partition staging of k_gb_main3
     for (int h = 2048; h >= 64; h >>= 1)
       if (h * (acc_bytes + 20) + 64 <= budget) {
         hot_cap = h;
@@ -1223,7 +1853,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         case ACC_ISUM:
         case ACC_MIN:
         case ACC_MAX: d.lds_off = take(hot_cap * 8); break;
-        case ACC_FSUM: d.lds_off = take(hot_cap * 24); break;
+        case ACC_FSUM: d.lds_off = take(hot_cap * 48); break;
         default: break;
       }
     }
@@ -1254,6 +1884,25 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
   uint64_t cap    = next_pow2(std::max<uint64_t>(1024, (uint64_t)(2.5 * (double)est)));
   uint64_t maxcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n + 2));
   if (cap > maxcap) cap = maxcap;
+
+  // Partitioned path for the keys outside the hot set (single fixed-width key, large inputs,
+  // accumulators it handles; COUNT_DISTINCT needs 32-bit keys and values and no null keys).
+  bool use_v3 = !generic && hot_cap > 0 && n >= (int64_t(1) << 20);
+  {
+    const char* env = std::getenv("LDF_GROUPBY_PARTITIONED");
+    if (env != nullptr && env[0] == '0') use_v3 = false;
+  }
+  int ndistinct = 0;
+  for (auto& h : low.acc) {
+    if (h.kind == ACC_PROD) use_v3 = false;
+    if (h.kind == ACC_DISTINCT) {
+      ++ndistinct;
+      if (type_width(tv->cols[h.col].type_id) > 4 || type_width(tv->cols[h.col].type_id) == 0) use_v3 = false;
+    }
+  }
+  if (ndistinct > 0 && (keys.c[0].width > 4 || keys.c[0].valid != nullptr || ndistinct > 3)) use_v3 = false;
+  for (int i = 0; i < naggs; ++i)
+    if (aggs[i].kind == LDF_AGG_VARIANCE || aggs[i].kind == LDF_AGG_STDDEV) use_v3 = false;
 
   for (;;) {
     LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "groupby: too many groups");
@@ -1292,7 +1941,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         off2[i] = seg(nslot, 0);
       }
       if (h.kind == ACC_MIN || h.kind == ACC_MAX) offf[i] = seg((nslot + 1) / 2, 0);  // u32 flags
-      if (h.kind == ACC_DISTINCT) {
+      if (h.kind == ACC_DISTINCT && !use_v3) {
         pcap[i] = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
         offp[i] = seg(pcap[i], kGEmpty);
         pcap_total += pcap[i];
@@ -1312,8 +1961,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       d.g1      = low.acc[i].kind == ACC_FSUM ? st + off1[i] : nullptr;
       d.g2      = low.acc[i].kind == ACC_FSUM ? st + off2[i] : nullptr;
       d.gflag   = (d.kind == ACC_MIN || d.kind == ACC_MAX) ? reinterpret_cast<uint32_t*>(st + offf[i]) : nullptr;
-      d.pset    = d.kind == ACC_DISTINCT ? st + offp[i] : nullptr;
-      d.pmask   = d.kind == ACC_DISTINCT ? pcap[i] - 1 : 0;
+      d.pset    = (d.kind == ACC_DISTINCT && !use_v3) ? st + offp[i] : nullptr;
+      d.pmask   = (d.kind == ACC_DISTINCT && !use_v3) ? pcap[i] - 1 : 0;
+      d.pset2   = nullptr;
+      d.pmask2  = 0;
       d.err     = reinterpret_cast<int32_t*>(status.get() + 2 + i);
     }
 
@@ -1322,7 +1973,144 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       hipLaunchKernelGGL(k_gb_hot_insert, dim3(grid_for(hot_cap, 256)), dim3(256), 0, s, t, hot, hot_cap);
       LDF_HIP_TRY(hipGetLastError());
     }
-    if (n > 0) {
+    // buffers of the partitioned path (freed at the end of the attempt)
+    Buf<unsigned long long> psets;
+    Buf<uint32_t> pcounts;
+    Buf<uint64_t> poffs;
+    Buf<unsigned char> precs;
+    Buf<unsigned long long> hparts;
+    if (n > 0 && use_v3) {
+      const int stage_bytes = kNumParts * 8;
+      const int main_lds    = plan.lds_bytes + stage_bytes;
+      int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
+      int nblk              = 256 * per_cu;
+      int64_t rpb           = (n + nblk - 1) / nblk;
+      rpb                   = (rpb + kSub3 - 1) / kSub3 * kSub3;
+      nblk                  = (int)((n + rpb - 1) / rpb);
+      plan.rows_per_block   = rpb;
+      // 1. histogram of the non-hot rows per (partition, block)
+      pcounts = Buf<uint32_t>(a, (size_t)kNumParts * nblk);
+      poffs   = Buf<uint64_t>(a, (size_t)kNumParts * nblk + 1);
+      {
+        static bool attr = false;
+        if (!attr) {
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          attr = true;
+        }
+        ProfScope prof("gb_hist", s);
+        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk), dim3(kGbThreads), plan.rows_off, s, keys, n, plan, hot, pcounts.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)kNumParts * nblk, a);
+      const int64_t ncold = (int64_t)read_u64(poffs.get() + (size_t)kNumParts * nblk, s);
+      // 2. sets of (key, value) pairs for COUNT_DISTINCT: hot keys by global slot, others by key
+      uint64_t hcap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)(n - ncold))));
+      uint64_t ccap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)ncold)));
+      if (ndistinct > 0) {
+        psets = Buf<unsigned long long>(a, (size_t)ndistinct * (hcap + ccap));
+        fill_u64((uint64_t*)psets.get(), kGEmpty, (int64_t)ndistinct * (hcap + ccap), s);
+        int di = 0;
+        for (int i = 0; i < nacc; ++i) {
+          if (plan.a[i].kind != ACC_DISTINCT) continue;
+          plan.a[i].pset   = psets.get() + (size_t)di * (hcap + ccap);
+          plan.a[i].pmask  = hcap - 1;
+          plan.a[i].pset2  = plan.a[i].pset + hcap;
+          plan.a[i].pmask2 = ccap - 1;
+          ++di;
+        }
+      }
+      // 3. partition record buffers (SoA)
+      PartBufs pb;
+      std::memset(&pb, 0, sizeof(pb));
+      pb.key_width = keys.c[0].width;
+      pb.nblk      = nblk;
+      pb.offsets   = poffs.get();
+      size_t rec   = (size_t)pb.key_width + 1;
+      for (int c = 0; c < plan.ncol; ++c) rec += (size_t)plan.col[c].width;
+      precs             = Buf<unsigned char>(a, (size_t)std::max<int64_t>(1, ncold) * rec + 64 * (kMaxCols + 2));
+      unsigned char* rp = precs.get();
+      auto carve        = [&](size_t bytes) {
+        unsigned char* r0 = rp;
+        rp += (bytes + 63) & ~size_t(63);
+        return r0;
+      };
+      pb.key = carve((size_t)ncold * pb.key_width);
+      for (int c = 0; c < plan.ncol; ++c) pb.val[c] = carve((size_t)ncold * plan.col[c].width);
+      pb.flags = carve((size_t)ncold);
+      // 4. per-block partials of the hot slots
+      HotParts hp;
+      std::memset(&hp, 0, sizeof(hp));
+      size_t words = 2 * (size_t)nblk * hot_cap;  // rows + flags as u32 pairs
+      std::vector<size_t> aoff(nacc, 0);
+      for (int i = 0; i < nacc; ++i) {
+        if (plan.a[i].lds_off < 0) continue;
+        aoff[i] = words;
+        words += (size_t)nblk * hot_cap * (plan.a[i].kind == ACC_FSUM ? 3 : 1);
+      }
+      hparts   = Buf<unsigned long long>(a, words);
+      hp.rows  = reinterpret_cast<uint32_t*>(hparts.get());
+      hp.flags = hp.rows + (size_t)nblk * hot_cap;
+      for (int i = 0; i < nacc; ++i)
+        if (plan.a[i].lds_off >= 0) hp.acc[i] = hparts.get() + aoff[i];
+      // 5. the pass over the rows
+      {
+        ProfScope prof("gb_aggregate", s);
+        hipLaunchKernelGGL(k_gb_main3, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
+      LDF_HIP_TRY(hipGetLastError());
+      // 6. one block per (partition, sub-pass)
+      PartPlan q;
+      std::memset(&q, 0, sizeof(q));
+      int per_slot = 8 + 4 + 4;
+      for (int i = 0; i < nacc; ++i) {
+        switch (plan.a[i].kind) {
+          case ACC_VCOUNT:
+          case ACC_DISTINCT: per_slot += 4; break;
+          case ACC_ISUM:
+          case ACC_MIN:
+          case ACC_MAX: per_slot += 8; break;
+          case ACC_FSUM: per_slot += 48; break;
+          default: break;
+        }
+      }
+      q.tcap = 256;
+      while (q.tcap * 2 * per_slot + 64 * (nacc + 3) <= 150 * 1024 && q.tcap < 8192) q.tcap *= 2;
+      // the sample's estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load
+      const double keys_per_part = 1.25 * (double)est / kNumParts;
+      q.S          = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
+      int32_t loff = 0;
+      auto ltake   = [&](int bytes) {
+        int32_t o = loff;
+        loff += (bytes + 15) & ~15;
+        return o;
+      };
+      q.key_off   = ltake(q.tcap * 8);
+      q.rows_off  = ltake(q.tcap * 4);
+      q.flags_off = ltake(q.tcap * 4);
+      for (int i = 0; i < nacc; ++i) {
+        q.acc_off[i] = -1;
+        switch (plan.a[i].kind) {
+          case ACC_VCOUNT:
+          case ACC_DISTINCT: q.acc_off[i] = ltake(q.tcap * 4); break;
+          case ACC_ISUM:
+          case ACC_MIN:
+          case ACC_MAX: q.acc_off[i] = ltake(q.tcap * 8); break;
+          case ACC_FSUM: q.acc_off[i] = ltake(q.tcap * 48); break;
+          default: break;
+        }
+      }
+      q.lds_bytes = loff;
+      if (ncold > 0) {
+        ProfScope prof("gb_partitions", s);
+        const int grid = kNumParts * q.S;
+        hipLaunchKernelGGL(k_gb_part3, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t);
+        LDF_HIP_TRY(hipGetLastError());
+      }
+    } else if (n > 0) {
       // one wave of resident blocks (256 CUs; two 1024-thread blocks per CU when LDS allows)
       int per_cu = plan.lds_bytes > 0 ? std::max(1, std::min(2, (160 * 1024) / plan.lds_bytes)) : 2;
       int nblk   = 256 * per_cu;

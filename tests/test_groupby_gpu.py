@@ -79,6 +79,38 @@ def test_groupby_c2_mix_vs_oracle(gpu, seed):
     assert_groupby_equal(got, exp, ["k"], rel=1e-12)
 
 
+@pytest.mark.parametrize("keys_kind", ["uniform_300k", "zipf_1M", "few_16", "nullkeys_int64", "int64_wide"])
+def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatch):
+    """Inputs of >= 1M rows take the partitioned path (hot keys in LDS, the rest hash-partitioned
+    and aggregated per partition in LDS); LDF_GROUPBY_PARTITIONED=0 forces the global-table path.
+    Both must equal the oracle (integers exactly, FP64 sums within 1e-12)."""
+    rng = np.random.default_rng(21)
+    n = 1_500_000
+    if keys_kind == "uniform_300k":
+        k = pa.array(rng.integers(0, 300_000, n).astype(np.int32))
+    elif keys_kind == "zipf_1M":
+        k = _c2_like(n, 3, n_keys=1_000_000)["k"]
+    elif keys_kind == "few_16":
+        k = pa.array(rng.integers(0, 16, n).astype(np.int32))
+    elif keys_kind == "nullkeys_int64":
+        k = pa.array(rng.integers(0, 50_000, n), pa.int64(), mask=rng.random(n) < 0.02)
+    else:
+        k = pa.array(rng.integers(-(2**62), 2**62, n // 3).repeat(3)[:n])
+    t = pa.table({"k": k,
+                  "f": pa.array(rng.uniform(-1, 1, n), mask=rng.random(n) < 0.01),
+                  "i": pa.array(rng.integers(-(2**40), 2**40, n), mask=rng.random(n) < 0.01),
+                  "u": pa.array(rng.integers(0, 10_000, n).astype(np.int32), mask=rng.random(n) < 0.01)})
+    aggs = [("f", "sum", "fs"), ("f", "mean", "fm"), ("i", "sum", "is"), ("i", "min", "imin"), ("i", "max", "imax"),
+            ("i", "count", "ic"), ("f", "max", "fmax"), (None, "count_all", "n")]
+    if keys_kind in ("uniform_300k", "zipf_1M", "few_16"):
+        aggs.append(("u", "count_distinct", "nu"))
+    got = gpu_groupby(t, ["k"], aggs)
+    monkeypatch.setenv("LDF_GROUPBY_PARTITIONED", "0")
+    ref = gpu_groupby(t, ["k"], aggs)
+    assert_groupby_equal(got, ref, ["k"], rel=0.0)  # both paths are exact and order-independent
+    assert_groupby_equal(got, groupby_via_oracle(t, ["k"], aggs), ["k"], rel=1e-12)
+
+
 def test_groupby_float_sum_exact_under_cancellation(gpu):
     """Values spanning 24 orders of magnitude with heavy cancellation: the 192-bit fixed-point
     accumulation returns the correctly rounded exact sum (checked with rational arithmetic)."""
