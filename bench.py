@@ -12,9 +12,10 @@ cpu_baseline: the CPU oracle (single-threaded C restatement of the reference CPU
 10M-row sample of the same workload; Arrow Acero (the reference CPU variant's engine, pyarrow,
 use_threads=False) on the same sample is reported beside it.
 
-Other workloads (for profiling): --workload join (1M-build x 10M-probe inner join),
---workload partition (1B int64 keys -> 256 partitions), --copy-calibration (a plain copy of
-known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE).
+Other workloads (also reported as `secondary` entries of the default line): --workload join
+(1M-build x 10M-probe inner join), --workload join_left / join_full (50M-build x 500M-probe with
+string payloads), --workload partition (1B int64 keys -> 256 partitions), --copy-calibration (a
+plain copy of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE).
 """
 
 from __future__ import annotations
@@ -78,6 +79,71 @@ def make_groupby_inputs(torch, C, n, seed=2):
     ]
     host = dict(k=k, f=f, i=i, u=u, masks=masks)
     return cols, host
+
+
+TAG_ALPHABET = b"ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+
+
+def tagged_strings(torch, n, g):
+    """n ASCII strings of length U{8..32} (large_utf8: int64 offsets + chars).  The first 8
+    characters spell the row number in base 64 (low digit first), the rest are random printable
+    characters, so a gathered string names the input row it came from."""
+    lens = torch.randint(8, 33, (n,), device="cuda", generator=g, dtype=torch.int64)
+    offs = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    torch.cumsum(lens, 0, out=offs[1:])
+    del lens
+    chars = torch.randint(33, 127, (int(offs[-1]),), device="cuda", generator=g, dtype=torch.uint8)
+    alpha = torch.tensor(list(TAG_ALPHABET), dtype=torch.uint8, device="cuda")
+    chunk = 1 << 27
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        rows = torch.arange(a, b, device="cuda", dtype=torch.int64)
+        for j in range(8):
+            chars[offs[a:b] + j] = alpha[(rows >> (6 * j)) & 63]
+    return offs, chars
+
+
+def make_join_c3_inputs(torch, C, n_probe=500_000_000, n_build=50_000_000, seed=3):
+    """Left/full join workload at 50M x 500M: lhs (probe) key int64 U[0, 1.6 n_build) with 10%
+    null keys; rhs (build) key int64 = Zipf(0.8) rank - 1 over 0.8 n_build values; both sides
+    carry int32 U, f64 U[-1,1), timestamp[ms] U[2000-01-01, 2030-01-01) and large_utf8 payloads."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+
+    def side(n, key, key_valid):
+        i32 = torch.randint(-(2**31), 2**31, (n,), device="cuda", generator=g, dtype=torch.int32)
+        f64 = torch.rand(n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1
+        ts = torch.randint(946684800000, 1893456000000, (n,), device="cuda", generator=g, dtype=torch.int64)
+        offs, chars = tagged_strings(torch, n, g)
+        kv = bitmap(torch, key_valid) if key_valid is not None else None
+        cols = [C.DeviceColumn(C.INT64, n, key.view(torch.uint8), kv), C.DeviceColumn(C.INT32, n, i32.view(torch.uint8)),
+                C.DeviceColumn(C.DOUBLE, n, f64.view(torch.uint8)), C.DeviceColumn(C.TIMESTAMP, n, ts.view(torch.uint8)),
+                C.DeviceColumn(C.LARGE_STRING, n, chars, None, offs)]
+        return cols, dict(key=key, key_valid=key_valid, i32=i32, f64=f64, ts=ts, offs=offs, chars=chars)
+
+    n_vals = max(1, n_build * 4 // 5)
+    bkey = zipf_keys(torch, n_build, n_vals, 0.8, g, torch.int64)
+    R, hr = side(n_build, bkey, None)
+    pkey = torch.randint(0, 2 * n_vals, (n_probe,), device="cuda", generator=g, dtype=torch.int64)
+    pvalid = torch.rand(n_probe, device="cuda", generator=g) >= 0.10
+    L, hl = side(n_probe, pkey, pvalid)
+    return L, R, hl, hr, n_vals
+
+
+def column_bytes(C, type_id, n, char_bytes, nullable):
+    """Bytes of a column in the minimal layout: values (8 B offset per row + chars for strings)
+    plus one validity bit per row when nullable."""
+    b = 8 * n + char_bytes if type_id == C.LARGE_STRING else n * C.WIDTH[type_id]
+    return b + ((n + 7) // 8 if nullable else 0)
+
+
+def input_bytes(C, cols):
+    return sum(column_bytes(C, c.type_id, c.length, int(c.data.numel()) if c.type_id == C.LARGE_STRING else 0,
+                            c.validity is not None) for c in cols)
+
+
+def output_bytes(C, rt):
+    return sum(column_bytes(C, c.type_id, c.length, c.data_bytes if c.type_id == C.LARGE_STRING else 0,
+                            bool(c.validity)) for c in rt.cols[:rt.out.ncols])
 
 
 GB_AGGS = [(1, "sum", None), (1, "mean", None), (2, "sum", None), (2, "min", None), (2, "max", None),
@@ -211,10 +277,47 @@ def run_join(args, torch, C):
     C.profile_enable(False)
     ms_step = (t1 - t0) * 1e3 / args.steps
     alg = (n + m) * 16 + nout * 32
-    kern = {k: C.profile_get(k) for k in ("join_build", "join_count", "join_emit", "gather")}
+    kern = {k: C.profile_get(k) for k in JOIN_KERNELS}
     return {"workload": "inner join build 1M (permutation keys) x probe 10M (uniform [0,2M)), i64 keys, "
                         "i64/f64 payloads", "rows_per_s": (n + m) / (ms_step * 1e-3), "ms_per_step": ms_step,
             "out_rows": nout, "alg_bytes": alg, "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+
+
+JOIN_KERNELS = ("join_build", "join_count", "join_emit", "gather", "gather_str")
+
+
+def run_join_c3(args, torch, C, join_type):
+    n_probe = args.rows or 500_000_000
+    n_build = max(1, n_probe // 10)
+    L, R, _, _, n_vals = make_join_c3_inputs(torch, C, n_probe, n_build)
+    torch.cuda.synchronize()
+    inb = input_bytes(C, L) + input_bytes(C, R)
+    for _ in range(args.warmup):
+        C.join_raw(L, R, [0], [0], join_type).free()
+    torch.cuda.synchronize()
+    C.profile_reset()
+    C.profile_enable(True)
+    t0 = time.perf_counter()
+    nout, outb = 0, 0
+    for _ in range(args.steps):
+        rt = C.join_raw(L, R, [0], [0], join_type)
+        nout, outb = rt.cols[0].length, output_bytes(C, rt)
+        rt.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    C.profile_enable(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern = {k: C.profile_get(k) for k in JOIN_KERNELS}
+    name = {1: "left", 2: "full"}[join_type]
+    del L, R
+    torch.cuda.empty_cache()
+    C.pool_release()
+    return {"workload": f"{name} join build {n_build} (int64 key Zipf(0.8) over {n_vals}) x probe {n_probe} "
+                        f"(int64 key U[0,{2 * n_vals}), 10% null), payloads int32/f64/timestamp[ms]/large_utf8 "
+                        "(8-32 chars) on both sides, all 10 columns out",
+            "rows_per_s": (n_probe + n_build) / (ms_step * 1e-3), "ms_per_step": ms_step, "out_rows": nout,
+            "alg_bytes": inb + outb, "whole_step_frac": (inb + outb) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
 
 
@@ -260,7 +363,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--workload", choices=["groupby", "join", "partition"], default="groupby")
+    ap.add_argument("--workload", choices=["groupby", "join", "join_left", "join_full", "partition"],
+                    default="groupby")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -287,8 +391,12 @@ def main():
             res["secondary"] = {"join_1Mx10M": run_join(sa, torch, C)}
             sa.rows = 200_000_000
             res["secondary"]["partition_200M"] = run_partition(sa, torch, C)
+            sa.rows, sa.steps, sa.warmup = 0, 3, 1
+            res["secondary"]["left_join_50Mx500M"] = run_join_c3(sa, torch, C, 1)
     elif args.workload == "join":
         res = run_join(args, torch, C)
+    elif args.workload in ("join_left", "join_full"):
+        res = run_join_c3(args, torch, C, 1 if args.workload == "join_left" else 2)
     else:
         res = run_partition(args, torch, C)
     print(json.dumps(res))

@@ -120,8 +120,9 @@ typedef struct ldf_table_out {
 
 /* Allocation callbacks (the role TaskMemoryResource plays in the reference,
  * cpp/src/core/allocator.cu:27-81).  `stream` is the call's HIP stream; allocations are
- * stream ordered.  A NULL ldf_allocator* selects the library default (hipMallocAsync on the
- * stream from the device's default memory pool). */
+ * stream ordered.  A NULL ldf_allocator* selects the library default: a caching device
+ * allocator that keeps freed blocks (in size classes) per device and stream and reuses them in
+ * stream order; ldf_pool_release returns its cached blocks to the device. */
 typedef void* (*ldf_alloc_fn)(void* ctx, size_t bytes, void* stream);
 typedef void (*ldf_free_fn)(void* ctx, void* ptr, size_t bytes, void* stream);
 typedef struct ldf_allocator {
@@ -219,6 +220,11 @@ int ldf_profile_get(const char* kernel, int64_t* launches, double* total_ms);
 
 /* Frees every buffer of an output table through `alloc` (NULL = library default). */
 void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream);
+
+/* Synchronises each device that holds cached blocks of the default allocator and frees those
+ * blocks (blocks in use stay allocated).  No counterpart in the reference, whose task memory
+ * comes from Legate's pool (cpp/src/core/allocator.cu:27-81). */
+void ldf_pool_release(void);
 
 const char* ldf_last_error(void);
 const char* ldf_version(void);

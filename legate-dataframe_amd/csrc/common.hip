@@ -20,21 +20,105 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 // ------------------------------------------------------------------------------------------
 // default pool + kernel timing
 // ------------------------------------------------------------------------------------------
-void configure_default_pool()
+namespace {
+// size classes: multiples of 4 KiB up to 1 MiB, then 8 classes per power of two
+size_t pool_class(size_t b)
 {
-  static std::mutex mu;
-  static bool done[64] = {};
-  int dev             = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
-  std::lock_guard<std::mutex> lk(mu);
-  if (done[dev]) return;
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-    uint64_t thr = ~0ull;
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  if (b <= (size_t(1) << 20)) return (b + 4095) & ~size_t(4095);
+  const int msb     = 63 - __builtin_clzll((unsigned long long)b);
+  const size_t step = size_t(1) << (msb - 3);
+  return (b + step - 1) & ~(step - 1);
+}
+
+struct DevicePool {
+  std::mutex mu;
+  struct Key {
+    int dev;
+    hipStream_t s;
+    bool operator<(const Key& o) const { return dev != o.dev ? dev < o.dev : s < o.s; }
+  };
+  std::map<Key, std::map<size_t, std::vector<void*>>> free_blocks;
+  std::map<void*, std::pair<int, size_t>> live;  // block -> (device, class)
+
+  void release_device(int dev)  // caller holds mu; `dev` is the current device
+  {
+    (void)hipDeviceSynchronize();  // queued work may still use a cached block
+    for (auto it = free_blocks.begin(); it != free_blocks.end();) {
+      if (it->first.dev == dev) {
+        for (auto& kv : it->second)
+          for (void* p : kv.second) (void)hipFree(p);
+        it = free_blocks.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    (void)hipGetLastError();
   }
+};
+DevicePool& device_pool()
+{
+  static DevicePool* p = new DevicePool();  // never destroyed: blocks may outlive static teardown
+  return *p;
+}
+}  // namespace
+
+void* pool_allocate(size_t bytes, hipStream_t s)
+{
+  int dev = 0;
+  LDF_HIP_TRY(hipGetDevice(&dev));
+  const size_t cls = pool_class(bytes);
+  DevicePool& P    = device_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto fit = P.free_blocks.find({dev, s});
+  if (fit != P.free_blocks.end()) {
+    auto cit = fit->second.find(cls);
+    if (cit != fit->second.end() && !cit->second.empty()) {
+      void* p = cit->second.back();
+      cit->second.pop_back();
+      P.live[p] = {dev, cls};
+      return p;
+    }
+  }
+  void* p      = nullptr;
+  hipError_t e = hipMalloc(&p, cls);
+  if (e != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    P.release_device(dev);  // cached blocks of every stream on this device, then once more
+    p = nullptr;
+    e = hipMalloc(&p, cls);
+    if (e != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      throw Error(LDF_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  P.live[p] = {dev, cls};
+  return p;
+}
+
+void pool_deallocate(void* p, hipStream_t s)
+{
+  DevicePool& P = device_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.live.find(p);
+  if (it == P.live.end()) return;  // not a pool block
+  P.free_blocks[{it->second.first, s}][it->second.second].push_back(p);
+  P.live.erase(it);
+}
+
+void pool_release_all()
+{
+  DevicePool& P = device_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  std::vector<int> devs;
+  for (auto& kv : P.free_blocks) devs.push_back(kv.first.dev);
+  for (int d : devs) {
+    (void)hipSetDevice(d);
+    P.release_device(d);
+  }
+  (void)hipSetDevice(cur);
   (void)hipGetLastError();
-  done[dev] = true;
 }
 
 namespace {
@@ -240,133 +324,206 @@ void free_column(ldf_column* c, Alloc& a)
   c->offsets = nullptr;
 }
 
-// One wave covers 64 consecutive output rows so the validity of those rows is one ballot.
+// Gather of several columns by one map, so the map (4 B/row) is read once per output row for all
+// of them.  One wave covers 64 consecutive output rows so each column's validity for those rows
+// is one ballot.  String columns get their lengths here (into dst[c], uint64 per row); the
+// characters follow in k_gather_str_chars once the lengths are scanned into offsets.
+struct GatherSet {
+  int ncols;
+  int width[kMaxGather];  // 1, 2, 4, 8, or 0 for large_utf8
+  const void* src[kMaxGather];
+  const uint8_t* sval[kMaxGather];
+  const int64_t* soff[kMaxGather];
+  void* dst[kMaxGather];
+  uint64_t* dval[kMaxGather];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather_fixed(const T* __restrict__ src, const uint8_t* __restrict__ src_valid,
-                                                      const int32_t* __restrict__ map, int64_t n, T* __restrict__ dst,
-                                                      uint64_t* __restrict__ dst_valid, unsigned long long* null_count)
+__device__ __forceinline__ void gather_store(const void* src, void* dst, int32_t m, int64_t i)
+{
+  static_cast<T*>(dst)[i] = m >= 0 ? static_cast<const T*>(src)[m] : T(0);
+}
+
+__global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t* __restrict__ map, int64_t n,
+                                                      unsigned long long* __restrict__ null_count)
 {
   const int lane = threadIdx.x & 63;
-  uint64_t nulls = 0;
+  uint64_t nulls[kMaxGather];
+#pragma unroll
+  for (int c = 0; c < kMaxGather; ++c) nulls[c] = 0;
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = base + lane;
-    bool ok   = false;
-    T v       = T(0);
-    if (i < n) {
-      int32_t m = map[i];
-      if (m >= 0) {
-        ok = valid_at(src_valid, m);
-        v  = src[m];
+    const int64_t i = base + lane;
+    const int32_t m = i < n ? map[i] : -1;
+#pragma unroll
+    for (int c = 0; c < kMaxGather; ++c) {
+      if (c >= g.ncols) break;
+      const bool ok = m >= 0 && valid_at(g.sval[c], m);
+      if (i < n) {
+        switch (g.width[c]) {
+          case 0:
+            static_cast<uint64_t*>(g.dst[c])[i] = ok ? (uint64_t)(g.soff[c][m + 1] - g.soff[c][m]) : 0;
+            break;
+          case 1: gather_store<uint8_t>(g.src[c], g.dst[c], m, i); break;
+          case 2: gather_store<uint16_t>(g.src[c], g.dst[c], m, i); break;
+          case 4: gather_store<uint32_t>(g.src[c], g.dst[c], m, i); break;
+          default: gather_store<uint64_t>(g.src[c], g.dst[c], m, i); break;
+        }
       }
-      dst[i] = v;
-    }
-    if (dst_valid != nullptr) {
-      uint64_t b = __ballot(ok);
-      if (lane == 0) {
-        dst_valid[base >> 6] = b;
-        int64_t in_range = n - base < 64 ? n - base : 64;
-        nulls += (uint64_t)in_range - (uint64_t)__popcll(b);
+      if (g.dval[c] != nullptr) {
+        const uint64_t b = __ballot(ok);
+        if (lane == 0) {
+          g.dval[c][base >> 6]   = b;
+          const int64_t in_range = n - base < 64 ? n - base : 64;
+          nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(b);
+        }
       }
     }
   }
-  if (dst_valid != nullptr && lane == 0 && nulls) atomicAdd(null_count, (unsigned long long)nulls);
-}
-
-__global__ __launch_bounds__(256) void k_gather_str_len(const int64_t* __restrict__ src_off, const uint8_t* __restrict__ src_valid,
-                                                        const int32_t* __restrict__ map, int64_t n, uint64_t* __restrict__ len,
-                                                        uint64_t* __restrict__ dst_valid, unsigned long long* null_count)
-{
-  const int lane = threadIdx.x & 63;
-  uint64_t nulls = 0;
-  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = base + lane;
-    bool ok   = false;
-    if (i < n) {
-      int32_t m  = map[i];
-      uint64_t l = 0;
-      if (m >= 0) {
-        ok = valid_at(src_valid, m);
-        if (ok) l = (uint64_t)(src_off[m + 1] - src_off[m]);
-      }
-      len[i] = l;
-    }
-    if (dst_valid != nullptr) {
-      uint64_t b = __ballot(ok);
-      if (lane == 0) {
-        dst_valid[base >> 6] = b;
-        int64_t in_range = n - base < 64 ? n - base : 64;
-        nulls += (uint64_t)in_range - (uint64_t)__popcll(b);
-      }
-    }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < kMaxGather; ++c)
+      if (c < g.ncols && nulls[c] != 0) atomicAdd(&null_count[c], (unsigned long long)nulls[c]);
   }
-  if (dst_valid != nullptr && lane == 0 && nulls) atomicAdd(null_count, (unsigned long long)nulls);
 }
 
-__global__ __launch_bounds__(256) void k_gather_str_chars(const int64_t* __restrict__ src_off, const uint8_t* __restrict__ src_chars,
-                                                          const int32_t* __restrict__ map, int64_t n,
-                                                          const int64_t* __restrict__ dst_off, uint8_t* __restrict__ dst_chars)
+// Characters of gathered strings.  Each wave owns 64 consecutive output rows, whose characters
+// form one contiguous destination range [D0, D1); lanes write it as aligned 4-byte words
+// (256 contiguous bytes per wave store), finding each byte's row by binary search over the 65
+// row offsets staged in LDS.  Words shared with a neighbouring wave's range are written with
+// byte stores, so no two waves write the same byte.
+constexpr int kStrBlock = 256;
+__global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* __restrict__ src_off,
+                                                                const uint8_t* __restrict__ src_chars,
+                                                                const int32_t* __restrict__ map, int64_t n,
+                                                                const int64_t* __restrict__ dst_off,
+                                                                uint8_t* __restrict__ dst_chars)
 {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t d = dst_off[i], e = dst_off[i + 1];
-    if (e == d) continue;
-    int64_t s = src_off[map[i]];
-    for (int64_t k = 0; k < e - d; ++k) dst_chars[d + k] = src_chars[s + k];
+  __shared__ int64_t s_doff[kStrBlock / 64][65];
+  __shared__ int64_t s_soff[kStrBlock / 64][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t align = (int64_t)((uintptr_t)dst_chars & 3);
+  for (int64_t bbase = (int64_t)blockIdx.x * kStrBlock; bbase < n; bbase += (int64_t)gridDim.x * kStrBlock) {
+    const int64_t base = bbase + wv * 64;
+    const int nrows    = (int)(n - base < 0 ? 0 : (n - base < 64 ? n - base : 64));
+    if (lane < nrows) {
+      s_doff[wv][lane] = dst_off[base + lane];
+      const int32_t m  = map[base + lane];
+      s_soff[wv][lane] = m >= 0 ? src_off[m] : 0;
+    }
+    if (lane == 0 && nrows > 0) s_doff[wv][nrows] = dst_off[base + nrows];
+    __syncthreads();
+    if (nrows > 0) {
+      const int64_t D0 = s_doff[wv][0], D1 = s_doff[wv][nrows];
+      const int64_t W0 = ((D0 + align) & ~(int64_t)3) - align;
+      for (int64_t wd = W0 + 4 * lane; wd < D1; wd += 256) {
+        const int64_t d0 = wd > D0 ? wd : D0;
+        int lo = 0, hi = nrows - 1;  // last row whose first byte is <= d0
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_doff[wv][mid] <= d0) lo = mid;
+          else hi = mid - 1;
+        }
+        uint32_t word = 0, bytes = 0;
+        int r         = lo;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int64_t d = wd + b;
+          if (d >= D0 && d < D1) {
+            while (d >= s_doff[wv][r + 1]) ++r;
+            word |= (uint32_t)src_chars[s_soff[wv][r] + (d - s_doff[wv][r])] << (8 * b);
+            bytes |= 1u << b;
+          }
+        }
+        if (bytes == 0xFu) {
+          *reinterpret_cast<uint32_t*>(dst_chars + wd) = word;
+        } else {
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            if ((bytes >> b) & 1u) dst_chars[wd + b] = (uint8_t)(word >> (8 * b));
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
+                    bool force_nullable, Alloc& a, ldf_column* out)
+{
+  hipStream_t s = a.stream();
+  int done      = 0;
+  std::vector<Buf<uint64_t>> lens;
+  try {
+    for (int c0 = 0; c0 < ncols; c0 += kMaxGather) {
+      const int nc = ncols - c0 < kMaxGather ? ncols - c0 : kMaxGather;
+      GatherSet g{};
+      g.ncols = nc;
+      Buf<unsigned long long> cnt(a, kMaxGather);
+      fill_u64(reinterpret_cast<uint64_t*>(cnt.get()), 0, kMaxGather, s);
+      for (int j = 0; j < nc; ++j) {
+        const ldf_column_view& v = *src[c0 + j];
+        ldf_column* o            = &out[c0 + j];
+        const int w              = type_width(v.type_id);
+        alloc_column(v.type_id, n_out, force_nullable || v.validity != nullptr, 0, a, o);
+        ++done;
+        g.width[j] = w;
+        g.src[j]   = v.data;
+        g.sval[j]  = v.validity;
+        g.soff[j]  = v.offsets;
+        g.dval[j]  = reinterpret_cast<uint64_t*>(o->validity);
+        if (w == 0) {
+          lens.emplace_back(a, n_out > 0 ? n_out : 1);
+          g.dst[j] = lens.back().get();
+        } else {
+          g.dst[j] = o->data;
+        }
+      }
+      if (n_out > 0) {
+        ProfScope prof("gather", s);
+        hipLaunchKernelGGL(k_gather_multi, dim3(grid_for(n_out, 256, 256 * 16)), dim3(256), 0, s, g, map, n_out,
+                           cnt.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      // strings: offsets from the lengths, then the characters
+      size_t li = lens.size();
+      for (int j = nc - 1; j >= 0; --j) {
+        if (g.width[j] != 0) continue;
+        ldf_column* o           = &out[c0 + j];
+        const ldf_column_view& v = *src[c0 + j];
+        Buf<uint64_t>& len      = lens[--li];
+        exclusive_scan_u64(len.get(), reinterpret_cast<uint64_t*>(o->offsets), n_out, a);
+        len.reset();
+        const int64_t total = (int64_t)read_u64(reinterpret_cast<uint64_t*>(o->offsets) + n_out, s);
+        a.deallocate(o->data, 1);
+        o->data       = nullptr;
+        o->data_bytes = total;
+        o->data       = a.allocate((size_t)(total > 0 ? total : 1));
+        if (total > 0) {
+          ProfScope prof("gather_str", s);
+          hipLaunchKernelGGL(k_gather_str_chars, dim3(grid_for(n_out, kStrBlock, 256 * 16)), dim3(kStrBlock), 0, s,
+                             v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, o->offsets,
+                             static_cast<uint8_t*>(o->data));
+          LDF_HIP_TRY(hipGetLastError());
+        }
+      }
+      lens.clear();
+      uint64_t hc[kMaxGather];
+      LDF_HIP_TRY(hipMemcpyAsync(hc, cnt.get(), sizeof(hc), hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      for (int j = 0; j < nc; ++j) out[c0 + j].null_count = out[c0 + j].validity ? (int64_t)hc[j] : 0;
+    }
+  } catch (...) {
+    for (int i = 0; i < done; ++i) free_column(&out[i], a);
+    throw;
   }
 }
 
 void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out, bool force_nullable, Alloc& a,
                    ldf_column* out)
 {
-  hipStream_t s  = a.stream();
-  bool nullable  = force_nullable || src.validity != nullptr;
-  int w          = type_width(src.type_id);
-  Buf<unsigned long long> nc(a, 1);
-  fill_u64(reinterpret_cast<uint64_t*>(nc.get()), 0, 1, s);
-  unsigned grid = grid_for(n_out, 256, 256 * 16);
-  if (w == 0) {
-    alloc_column(src.type_id, n_out, nullable, 0, a, out);
-    a.deallocate(out->data, 1);
-    out->data = nullptr;
-    Buf<uint64_t> len(a, n_out > 0 ? n_out : 1);
-    hipLaunchKernelGGL(k_gather_str_len, dim3(grid), dim3(256), 0, s, src.offsets, src.validity, map, n_out, len.get(),
-                       (uint64_t*)out->validity, nc.get());
-    LDF_HIP_TRY(hipGetLastError());
-    exclusive_scan_u64(len.get(), reinterpret_cast<uint64_t*>(out->offsets), n_out, a);
-    int64_t total = (int64_t)read_u64(reinterpret_cast<uint64_t*>(out->offsets) + n_out, s);
-    out->data_bytes = total;
-    out->data       = a.allocate((size_t)(total > 0 ? total : 1));
-    if (total > 0) {
-      hipLaunchKernelGGL(k_gather_str_chars, dim3(grid), dim3(256), 0, s, src.offsets,
-                         static_cast<const uint8_t*>(src.data), map, n_out, out->offsets, static_cast<uint8_t*>(out->data));
-      LDF_HIP_TRY(hipGetLastError());
-    }
-  } else {
-    alloc_column(src.type_id, n_out, nullable, 0, a, out);
-    if (n_out > 0) {
-      ProfScope prof("gather", s);
-      switch (w) {
-        case 1:
-          hipLaunchKernelGGL(k_gather_fixed<uint8_t>, dim3(grid), dim3(256), 0, s, (const uint8_t*)src.data, src.validity, map,
-                             n_out, (uint8_t*)out->data, (uint64_t*)out->validity, nc.get());
-          break;
-        case 2:
-          hipLaunchKernelGGL(k_gather_fixed<uint16_t>, dim3(grid), dim3(256), 0, s, (const uint16_t*)src.data, src.validity,
-                             map, n_out, (uint16_t*)out->data, (uint64_t*)out->validity, nc.get());
-          break;
-        case 4:
-          hipLaunchKernelGGL(k_gather_fixed<uint32_t>, dim3(grid), dim3(256), 0, s, (const uint32_t*)src.data, src.validity,
-                             map, n_out, (uint32_t*)out->data, (uint64_t*)out->validity, nc.get());
-          break;
-        default:
-          hipLaunchKernelGGL(k_gather_fixed<uint64_t>, dim3(grid), dim3(256), 0, s, (const uint64_t*)src.data, src.validity,
-                             map, n_out, (uint64_t*)out->data, (uint64_t*)out->validity, nc.get());
-      }
-      LDF_HIP_TRY(hipGetLastError());
-    }
-  }
-  out->null_count = nullable ? (int64_t)read_u64(reinterpret_cast<const uint64_t*>(nc.get()), s) : 0;
+  const ldf_column_view* p = &src;
+  gather_columns(&p, 1, map, n_out, force_nullable, a, out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -456,6 +613,8 @@ void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream
   Alloc a(alloc, (hipStream_t)stream);
   for (int i = 0; i < out->ncols; ++i) free_column(&out->cols[i], a);
 }
+
+void ldf_pool_release(void) { pool_release_all(); }
 
 int ldf_bools_to_bitmask(const uint8_t* bools, int64_t n, uint8_t* bitmask, int64_t* null_count_dev, void* stream)
 {

@@ -2193,7 +2193,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         op.o[op.n++] = o;
       };
       if (generic) {
-        for (int i = 0; i < nkeys; ++i, ++done) gather_column(tv->cols[key_idx[i]], rep.get(), ng, false, a, &out->cols[done]);
+        std::vector<const ldf_column_view*> src(nkeys);
+        for (int i = 0; i < nkeys; ++i) src[i] = &tv->cols[key_idx[i]];
+        gather_columns(src.data(), nkeys, rep.get(), ng, false, a, out->cols);
+        done = nkeys;
       } else {
         const ldf_column_view& kc = tv->cols[key_idx[0]];
         const bool nullable       = kc.validity != nullptr;

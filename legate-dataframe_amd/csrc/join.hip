@@ -21,6 +21,8 @@
 // offsets in LDS), so one hot key with 10^5 matches does not serialise on one thread.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
 
@@ -419,15 +421,15 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
 
   // gather output columns
   out->ncols = n_lhs_out + n_rhs_out;
-  int done   = 0;
+  std::vector<const ldf_column_view*> lsrc(n_lhs_out), rsrc(n_rhs_out);
+  for (int i = 0; i < n_lhs_out; ++i) lsrc[i] = &lhs->cols[lhs_out[i]];
+  for (int i = 0; i < n_rhs_out; ++i) rsrc[i] = &rhs->cols[rhs_out[i]];
+  gather_columns(lsrc.data(), n_lhs_out, lmap.get(), n_out, jt == LDF_JOIN_FULL, a, out->cols);
   try {
-    for (int i = 0; i < n_lhs_out; ++i, ++done)
-      gather_column(lhs->cols[lhs_out[i]], lmap.get(), n_out, jt == LDF_JOIN_FULL, a, &out->cols[done]);
-    for (int i = 0; i < n_rhs_out; ++i, ++done)
-      gather_column(rhs->cols[rhs_out[i]], rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
-                    &out->cols[done]);
+    gather_columns(rsrc.data(), n_rhs_out, rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
+                   out->cols + n_lhs_out);
   } catch (...) {
-    for (int i = 0; i < done; ++i) free_column(&out->cols[i], a);
+    for (int i = 0; i < n_lhs_out; ++i) free_column(&out->cols[i], a);
     out->ncols = 0;
     throw;
   }

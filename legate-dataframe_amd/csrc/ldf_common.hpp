@@ -66,7 +66,6 @@ int guarded(F&& f)
 // ------------------------------------------------------------------------------------------
 // Keeps freed memory in the current device's default pool (release threshold = max) so that
 // stream-ordered allocations after a synchronisation do not go back to the driver.
-void configure_default_pool();
 
 // Records HIP events on `s` around a kernel launch when ldf_profile_enable(1) is set.
 class ProfScope {
@@ -81,6 +80,15 @@ class ProfScope {
   hipStream_t s_;
   hipEvent_t a_ = nullptr;
 };
+
+// Caching device allocator behind the default allocation path (no ldf_allocator given): blocks
+// are rounded to size classes (<= 12.5% slack) and kept per (device, stream) after they are
+// freed, so a later allocation on the same stream reuses them in stream order without a
+// hipMalloc / hipFree round trip.  On allocation failure the cache of the device is released
+// and the allocation retried once.  Throws Error(LDF_E_OOM).
+void* pool_allocate(size_t bytes, hipStream_t s);
+void pool_deallocate(void* p, hipStream_t s);
+void pool_release_all();
 
 class Alloc {
  public:
@@ -97,12 +105,7 @@ class Alloc {
       p = a_->alloc(a_->ctx, bytes, (void*)s_);
       if (p == nullptr) throw Error(LDF_E_OOM, "allocator returned NULL for " + std::to_string(bytes) + " bytes");
     } else {
-      configure_default_pool();
-      hipError_t e = hipMallocAsync(&p, bytes, s_);
-      if (e != hipSuccess || p == nullptr) {
-        (void)hipGetLastError();
-        throw Error(LDF_E_OOM, "hipMallocAsync failed for " + std::to_string(bytes) + " bytes");
-      }
+      p = pool_allocate(bytes, s_);
     }
     return p;
   }
@@ -114,7 +117,7 @@ class Alloc {
     if (a_ != nullptr && a_->free != nullptr) {
       a_->free(a_->ctx, p, bytes, (void*)s_);
     } else if (a_ == nullptr || a_->alloc == nullptr) {
-      (void)hipFreeAsync(p, s_);
+      pool_deallocate(p, s_);
     }
   }
 
@@ -262,6 +265,11 @@ uint64_t read_u64(const uint64_t* dev, hipStream_t s);
 // `force_nullable` allocates a validity buffer even when the source has none.
 void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out, bool force_nullable,
                    Alloc& a, ldf_column* out);
+// The same for `ncols` columns sharing one map (read once per row for up to kMaxGather columns).
+// On error every column this call allocated is freed.
+constexpr int kMaxGather = 8;
+void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
+                    bool force_nullable, Alloc& a, ldf_column* out);
 void alloc_column(int32_t type, int64_t n, bool nullable, int64_t string_bytes, Alloc& a,
                   ldf_column* out);
 void free_column(ldf_column* c, Alloc& a);

@@ -335,7 +335,9 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       Buf<int32_t> map(a, n);
       hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, perm.get(), n, map.get());
       LDF_HIP_TRY(hipGetLastError());
-      for (int i = 0; i < table->ncols; ++i) gather_column(table->cols[i], map.get(), n, false, a, &out->cols[i]);
+      std::vector<const ldf_column_view*> src(table->ncols);
+      for (int i = 0; i < table->ncols; ++i) src[i] = &table->cols[i];
+      gather_columns(src.data(), table->ncols, map.get(), n, false, a, out->cols);
     }
 
     Buf<uint64_t> offs(a, (size_t)nparts + 1);

@@ -9,5 +9,10 @@ libldf_hip.so (include/ldf_hip.h) and raise if that library is missing.
 
 __version__ = "0.1.0"
 
+# torch (device memory and streams) is imported before the native libraries: its HIP runtime
+# then serves libldf_hip.so's libamdhip64 dependency, so one HIP runtime owns the GPU in this
+# process (a second, separately loaded runtime would find no device).
+import torch  # noqa: F401,E402
+
 from legate_dataframe.lib.core.column import LogicalColumn  # noqa: F401,E402
 from legate_dataframe.lib.core.table import LogicalTable  # noqa: F401,E402

@@ -30,7 +30,7 @@ EXPORTED = [
     "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash",
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
-    "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get",
+    "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release",
 ]
 
 
@@ -88,6 +88,8 @@ def load(require: bool = True):
             if not require:
                 return None
             raise ImportError(f"{_LIB_PATH} is missing: run `python legate-dataframe_amd/build.py` first")
+        import torch  # noqa: F401  (its HIP runtime must be the one the library binds to)
+
         L = ctypes.CDLL(str(_LIB_PATH))
         P = ctypes.POINTER
         L.ldf_join.argtypes = [P(TableView), P(TableView), P(ctypes.c_int32), P(ctypes.c_int32), ctypes.c_int32,
@@ -393,6 +395,11 @@ def hash_partition_raw(table, cols, nparts: int, seed: int = 0):
     check(L.ldf_hash_partition(ctypes.byref(tv), int32_array(cols), len(cols), nparts, seed, None, stream_ptr(),
                                ctypes.byref(rt.out), offs))
     return rt, list(offs)
+
+
+def pool_release() -> None:
+    """Returns the default allocator's cached device blocks to the device."""
+    load().ldf_pool_release()
 
 
 def profile_enable(on: bool = True) -> None:
