@@ -321,6 +321,98 @@ def run_join_c3(args, torch, C, join_type):
             "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
 
 
+BATCH_GB_AGGS = [(1, "sum", None), (2, "sum", None), (1, "count", None)]
+
+
+def make_batched_inputs(torch, C, ninst=4096, seed=4):
+    """4096 independent join + groupby instances: probe rows log-uniform in [1e3, 4e6], build =
+    probe / 10, int64 keys U[0, 2 build) on both sides, build payload int64, probe payload f64;
+    the groupby table of an instance has the probe's row count, an int32 key U[0, g) with g
+    log-uniform in [1e3, 1e5], an int64 and a f64 value (SUM, SUM, COUNT)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    p = np.exp(rng.uniform(np.log(1e3), np.log(4e6), ninst)).astype(np.int64)
+    b = np.maximum(1, p // 10)
+    ng = np.exp(rng.uniform(np.log(1e3), np.log(1e5), ninst)).astype(np.int64)
+    P, B = int(p.sum()), int(b.sum())
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    pk = torch.empty(P, dtype=torch.int64, device="cuda")
+    bk = torch.empty(B, dtype=torch.int64, device="cuda")
+    gk = torch.empty(P, dtype=torch.int32, device="cuda")
+    pp = torch.rand(P, device="cuda", generator=g, dtype=torch.float64) * 2 - 1
+    bp = torch.randint(-(2**62), 2**62, (B,), device="cuda", generator=g, dtype=torch.int64)
+    gi = torch.randint(-(2**40), 2**40, (P,), device="cuda", generator=g, dtype=torch.int64)
+    po = np.concatenate([[0], np.cumsum(p)])
+    bo = np.concatenate([[0], np.cumsum(b)])
+    for i in range(ninst):
+        hi = int(2 * b[i])
+        pk[po[i]:po[i + 1]].random_(0, hi, generator=g)
+        bk[bo[i]:bo[i + 1]].random_(0, hi, generator=g)
+        gk[po[i]:po[i + 1]].random_(0, int(ng[i]), generator=g)
+    dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
+    L, R, G = [], [], []
+    for i in range(ninst):
+        ps, bs = slice(int(po[i]), int(po[i + 1])), slice(int(bo[i]), int(bo[i + 1]))
+        L.append([dc(pk[ps], C.INT64), dc(pp[ps], C.DOUBLE)])
+        R.append([dc(bk[bs], C.INT64), dc(bp[bs], C.INT64)])
+        G.append([dc(gk[ps], C.INT32), dc(gi[ps], C.INT64), dc(pp[ps], C.DOUBLE)])
+    keep = (pk, bk, gk, pp, bp, gi)
+    return L, R, G, keep, dict(probe_rows=P, build_rows=B, instances=ninst)
+
+
+def run_batched(args, torch, C):
+    ninst = args.instances
+    L, R, G, keep, info = make_batched_inputs(torch, C, ninst)
+    torch.cuda.synchronize()
+
+    def plain_step():
+        for i in range(ninst):
+            C.join_raw(L[i], R[i], [0], [0], 0).free()
+            C.groupby_raw(G[i], [0], BATCH_GB_AGGS).free()
+
+    plain_step()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_step()
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    C.pool_release()
+
+    t0 = time.perf_counter()
+    bj = C.batch_join(L, R, [0], [0], 0, config={"use_graph": 1})
+    bg = C.batch_groupby(G, [0], BATCH_GB_AGGS, config={"use_graph": 1})
+    torch.cuda.synchronize()
+    create_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(2):  # warm-up; the first run captures the graphs
+        bj.run()
+        bg.run()
+    torch.cuda.synchronize()
+    steps = max(3, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bj.run()
+        bg.run()
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+    jc, joff = bj.result()
+    gc, goff = bg.result()
+    res = {"workload": f"{ninst} join + groupby instances: probe rows log-uniform [1e3, 4e6], build = probe/10, "
+                       "int64 keys U[0, 2 build), int64/f64 payloads; groupby int32 key with 1e3..1e5 distinct, "
+                       "SUM(int64), SUM(f64), COUNT",
+           **info, "join_out_rows": int(joff[-1]), "groups": int(goff[-1]),
+           "plain_ms": plain_ms, "batched_ms": batched_ms, "speedup": plain_ms / batched_ms,
+           "batched_create_ms": create_ms, "speedup_incl_create": plain_ms / (batched_ms + create_ms),
+           "rows_per_s_batched": (2 * info["probe_rows"] + info["build_rows"]) / (batched_ms * 1e-3),
+           "config_used": {"join": bj.config_used(), "groupby": bg.config_used()}}
+    bj.close()
+    bg.close()
+    del L, R, G, keep
+    torch.cuda.empty_cache()
+    C.pool_release()
+    return res
+
+
 def run_partition(args, torch, C):
     n, P = args.rows or 1_000_000_000, 256
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -363,8 +455,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--workload", choices=["groupby", "join", "join_left", "join_full", "partition"],
+    ap.add_argument("--workload", choices=["groupby", "join", "join_left", "join_full", "partition", "batched"],
                     default="groupby")
+    ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -395,6 +488,8 @@ def main():
             res["secondary"]["left_join_50Mx500M"] = run_join_c3(sa, torch, C, 1)
     elif args.workload == "join":
         res = run_join(args, torch, C)
+    elif args.workload == "batched":
+        res = run_batched(args, torch, C)
     elif args.workload in ("join_left", "join_full"):
         res = run_join_c3(args, torch, C, 1 if args.workload == "join_left" else 2)
     else:

@@ -226,6 +226,48 @@ void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream
  * comes from Legate's pool (cpp/src/core/allocator.cu:27-81). */
 void ldf_pool_release(void);
 
+/* ---- batched form -------------------------------------------------------------------------
+ * Many independent joins or groupbys of one schema (the per-partition point tasks a Legate
+ * launch runs one by one: JoinTask / GroupByAggregationTask over each partition,
+ * cpp/src/join.cpp:410-420 and cpp/src/groupby_aggregation.cpp:211-226) executed together.
+ * ldf_batch_*_create inspects the instances (one host synchronisation for the sizes), allocates
+ * every table, work list and output buffer of the batch, and may autotune; ldf_batch_run then
+ * enqueues the whole computation -- one persistent work-queue kernel per phase draining
+ * (instance, tile) items -- with no host synchronisation and no allocation, captured in a
+ * hipGraph on its first call and replayed afterwards when cfg->use_graph is set.
+ * ldf_batch_result waits for the stream and describes the outputs: one table whose rows are the
+ * instances' results concatenated, instance i owning rows [row_offsets[i], row_offsets[i+1]);
+ * the buffers belong to the batch (valid until the next run or destroy).  Results are those
+ * of ldf_join / ldf_groupby on each instance (rows in another order).
+ * Limits: one fixed-width key column; join types INNER, LEFT, SEMI, ANTI; fixed-width output
+ * columns; groupby kinds SUM, MEAN, COUNT, COUNT_ALL, MIN, MAX over at most 4 value columns
+ * (an FP64 SUM / MEAN input outside the exact range |x| < 2^62 makes ldf_batch_result return
+ * LDF_E_UNSUPPORTED).  A run over inputs whose sizes changed after create may make
+ * ldf_batch_result return LDF_E_INVALID (create the batch again).  No counterpart in the
+ * reference. */
+typedef struct ldf_batch_config {
+  double load_factor; /* hash-table load factor in (0, 0.9]; 0 = autotuned */
+  int32_t tile_rows;  /* rows per work item: 1024, 2048 or 4096; 0 = autotuned */
+  int32_t lds_slots;  /* groupby: slots of the LDS hash table that aggregates one partition
+                         (256, 512, 1024 or 2048; the partition fan-out follows from it, so it
+                         sets how much of each instance's table lives in LDS); 0 = autotuned */
+  int32_t use_graph;  /* 1 = capture the run's launch sequence in a hipGraph and replay it */
+} ldf_batch_config;
+
+typedef struct ldf_batch ldf_batch;
+
+int ldf_batch_join_create(int32_t ninst, const ldf_table_view* lhs, const ldf_table_view* rhs,
+                          const int32_t* lhs_keys, const int32_t* rhs_keys, int32_t nkeys, int32_t join_type,
+                          int32_t nulls_equal, const int32_t* lhs_out, int32_t n_lhs_out, const int32_t* rhs_out,
+                          int32_t n_rhs_out, const ldf_batch_config* cfg, void* stream, ldf_batch** batch);
+int ldf_batch_groupby_create(int32_t ninst, const ldf_table_view* tables, const int32_t* keys, int32_t nkeys,
+                             const ldf_agg* aggs, int32_t naggs, const ldf_batch_config* cfg, void* stream,
+                             ldf_batch** batch);
+int ldf_batch_run(ldf_batch* batch, void* stream);
+int ldf_batch_result(ldf_batch* batch, void* stream, ldf_table_out* out, int64_t* row_offsets);
+int ldf_batch_config_used(const ldf_batch* batch, ldf_batch_config* cfg);
+void ldf_batch_destroy(ldf_batch* batch);
+
 const char* ldf_last_error(void);
 const char* ldf_version(void);
 

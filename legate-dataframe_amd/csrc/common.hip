@@ -282,6 +282,43 @@ static void scan_impl(const In* in, uint64_t* out, int64_t n, Alloc& a)
   LDF_HIP_TRY(hipGetLastError());
 }
 
+// Same recursion with scratch taken from a caller-provided workspace (no allocation, so the
+// launch sequence can be captured in a graph and replayed).
+template <typename In>
+static void scan_ws_impl(const In* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s)
+{
+  int64_t m      = n + 1;
+  int64_t ntiles = (m + kScanTile - 1) / kScanTile;
+  if (ntiles == 1) {
+    hipLaunchKernelGGL(k_scan_tile_apply<In>, dim3(1), dim3(kScanBlock), 0, s, in, n, (const uint64_t*)nullptr, out);
+    LDF_HIP_TRY(hipGetLastError());
+    return;
+  }
+  uint64_t* sums = ws;
+  uint64_t* offs = ws + ntiles;
+  hipLaunchKernelGGL(k_scan_tile_sums<In>, dim3(ntiles), dim3(kScanBlock), 0, s, in, n, sums);
+  LDF_HIP_TRY(hipGetLastError());
+  scan_ws_impl<uint64_t>(sums, offs, ntiles, ws + 2 * ntiles + 1, s);
+  hipLaunchKernelGGL(k_scan_tile_apply<In>, dim3(ntiles), dim3(kScanBlock), 0, s, in, n, (const uint64_t*)offs, out);
+  LDF_HIP_TRY(hipGetLastError());
+}
+
+int64_t scan_workspace_elems(int64_t n)
+{
+  int64_t total = 0;
+  for (int64_t m = n + 1, nt = (m + kScanTile - 1) / kScanTile; nt > 1; m = nt + 1, nt = (m + kScanTile - 1) / kScanTile)
+    total += 2 * nt + 1;
+  return total + 1;
+}
+void exclusive_scan_u64_ws(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s)
+{
+  scan_ws_impl<uint64_t>(in, out, n, ws, s);
+}
+void exclusive_scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s)
+{
+  scan_ws_impl<uint32_t>(in, out, n, ws, s);
+}
+
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, Alloc& a) { scan_impl<uint64_t>(in, out, n, a); }
 void exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n, Alloc& a)
 {

@@ -256,6 +256,11 @@ inline uint64_t next_pow2(uint64_t x)
 // out[i] = sum_{j<i} in[i] for i in [0, n]; out has n+1 entries; returns nothing (device only)
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, Alloc& a);
 void exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n, Alloc& a);
+// Variants that take their scratch from `ws` (scan_workspace_elems(n) words) instead of
+// allocating: usable inside a captured launch sequence.
+int64_t scan_workspace_elems(int64_t n);
+void exclusive_scan_u64_ws(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s);
+void exclusive_scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s);
 void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);

@@ -31,6 +31,8 @@ EXPORTED = [
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
     "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release",
+    "ldf_batch_join_create", "ldf_batch_groupby_create", "ldf_batch_run", "ldf_batch_result",
+    "ldf_batch_config_used", "ldf_batch_destroy",
 ]
 
 
@@ -65,6 +67,11 @@ class Allocator(ctypes.Structure):
 class Agg(ctypes.Structure):
     _fields_ = [("in_col", ctypes.c_int32), ("kind", ctypes.c_int32), ("count_mode", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("load_factor", ctypes.c_double), ("tile_rows", ctypes.c_int32), ("lds_slots", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32)]
 
 
 class LdfError(RuntimeError):
@@ -112,6 +119,21 @@ def load(require: bool = True):
         L.ldf_profile_enable.argtypes = [ctypes.c_int]
         L.ldf_profile_get.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_double)]
         L.ldf_profile_get.restype = ctypes.c_int
+        L.ldf_batch_join_create.argtypes = [ctypes.c_int32, P(TableView), P(TableView), P(ctypes.c_int32),
+                                            P(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_int32), ctypes.c_int32,
+                                            P(BatchConfig), ctypes.c_void_p, P(ctypes.c_void_p)]
+        L.ldf_batch_groupby_create.argtypes = [ctypes.c_int32, P(TableView), P(ctypes.c_int32), ctypes.c_int32,
+                                               P(Agg), ctypes.c_int32, P(BatchConfig), ctypes.c_void_p,
+                                               P(ctypes.c_void_p)]
+        L.ldf_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ldf_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(TableOut), P(ctypes.c_int64)]
+        L.ldf_batch_config_used.argtypes = [ctypes.c_void_p, P(BatchConfig)]
+        L.ldf_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.ldf_batch_destroy.restype = None
+        for f in ("ldf_batch_join_create", "ldf_batch_groupby_create", "ldf_batch_run", "ldf_batch_result",
+                  "ldf_batch_config_used"):
+            getattr(L, f).restype = ctypes.c_int
         L.ldf_last_error.restype = ctypes.c_char_p
         L.ldf_version.restype = ctypes.c_char_p
         for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_bools_to_bitmask",
@@ -416,3 +438,117 @@ def profile_get(kernel: str):
     ms = ctypes.c_double()
     load().ldf_profile_get(kernel.encode(), ctypes.byref(n), ctypes.byref(ms))
     return n.value, ms.value
+
+
+# ---------------------------------------------------------------------------------------------
+# batched form
+# ---------------------------------------------------------------------------------------------
+class _DevBuf:
+    """A device buffer owned by the library, exposed to torch without a copy."""
+
+    def __init__(self, ptr: int, nbytes: int):
+        self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False), "version": 3}
+
+
+def _view_bytes(ptr: int, nbytes: int):
+    import torch
+
+    if not ptr or nbytes <= 0:
+        return torch.empty(0, dtype=torch.uint8, device="cuda")
+    return torch.as_tensor(_DevBuf(ptr, nbytes), device="cuda")
+
+
+def _config(config) -> Optional[BatchConfig]:
+    if config is None:
+        return None
+    c = config if isinstance(config, dict) else dict(config)
+    return BatchConfig(float(c.get("load_factor", 0.0)), int(c.get("tile_rows", 0)), int(c.get("lds_slots", 0)),
+                       int(c.get("use_graph", 1)))
+
+
+class Batch:
+    """A planned batch of joins or groupbys (ldf_batch_*): run() enqueues it, result() waits and
+    returns the concatenated output columns (views of the batch's buffers) and the per-instance
+    row offsets."""
+
+    def __init__(self, handle, ncols: int, ninst: int, keep):
+        self.h = handle
+        self.ncols = ncols
+        self.ninst = ninst
+        self._keep = keep
+
+    def run(self) -> None:
+        check(load().ldf_batch_run(self.h, stream_ptr()))
+
+    def result(self):
+        import numpy as np
+
+        cols = (Column * max(1, self.ncols))()
+        out = TableOut(self.ncols, cols)
+        offs = (ctypes.c_int64 * (self.ninst + 1))()
+        check(load().ldf_batch_result(self.h, stream_ptr(), ctypes.byref(out), offs))
+        res = []
+        for i in range(out.ncols):
+            c = out.cols[i]
+            data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
+            valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
+            res.append(DeviceColumn(c.type_id, c.length, data, valid, None, c.null_count))
+        return res, np.frombuffer(offs, dtype=np.int64).copy()
+
+    def config_used(self) -> dict:
+        c = BatchConfig()
+        check(load().ldf_batch_config_used(self.h, ctypes.byref(c)))
+        return {"load_factor": c.load_factor, "tile_rows": c.tile_rows, "lds_slots": c.lds_slots,
+                "use_graph": c.use_graph}
+
+    def close(self) -> None:
+        if self.h:
+            load().ldf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _table_views(tables):
+    keep = []
+    arr = (TableView * len(tables))()
+    for i, t in enumerate(tables):
+        tv, a = make_table_view([c.view() for c in t])
+        arr[i] = tv
+        keep.append(a)
+    return arr, keep
+
+
+def batch_join(lhs_tables, rhs_tables, lhs_keys, rhs_keys, join_type: int, nulls_equal: bool = True,
+               lhs_out=None, rhs_out=None, config=None) -> Batch:
+    L = load()
+    assert len(lhs_tables) == len(rhs_tables) and len(lhs_tables) > 0
+    lhs_out = list(range(len(lhs_tables[0]))) if lhs_out is None else list(lhs_out)
+    if rhs_out is None:
+        rhs_out = [] if join_type in (3, 4) else list(range(len(rhs_tables[0])))
+    la, lk = _table_views(lhs_tables)
+    ra, rk = _table_views(rhs_tables)
+    cfg = _config(config)
+    h = ctypes.c_void_p()
+    check(L.ldf_batch_join_create(len(lhs_tables), la, ra, int32_array(lhs_keys), int32_array(rhs_keys),
+                                  len(lhs_keys), join_type, int(bool(nulls_equal)), int32_array(lhs_out),
+                                  len(lhs_out), int32_array(rhs_out), len(rhs_out),
+                                  ctypes.byref(cfg) if cfg is not None else None, stream_ptr(), ctypes.byref(h)))
+    return Batch(h, len(lhs_out) + len(rhs_out), len(lhs_tables), (lhs_tables, rhs_tables, la, ra, lk, rk))
+
+
+def batch_groupby(tables, keys, aggs, config=None) -> Batch:
+    """aggs: list of (in_col, kind_name, count_mode_name) as for groupby()."""
+    L = load()
+    ta, tk = _table_views(tables)
+    arr = (Agg * max(1, len(aggs)))(*[Agg(-1 if c is None else c, AGG[k], COUNT_MODE[m or "only_valid"], 0)
+                                      for c, k, m in aggs])
+    cfg = _config(config)
+    h = ctypes.c_void_p()
+    check(L.ldf_batch_groupby_create(len(tables), ta, int32_array(keys), len(keys), arr, len(aggs),
+                                     ctypes.byref(cfg) if cfg is not None else None, stream_ptr(), ctypes.byref(h)))
+    return Batch(h, len(keys) + len(aggs), len(tables), (tables, ta, tk))
