@@ -413,6 +413,142 @@ def run_batched(args, torch, C):
     return res
 
 
+SERVED_AGGS = [(1, "sum", None), (2, "sum", None), (None, "count_all", None)]
+
+
+def make_served_inputs(torch, C, nq, seed=5):
+    """8 device-resident tables of 10M..200M rows (log-spaced; int64 key uniform over the row count
+    for even tables, Zipf(1.1) over it for odd ones; int64 and f64 payloads) and nq queries: half
+    inner joins of a probe (int64 key uniform over the table's key range, f64 payload) with a table,
+    half groupby SUM(int64), SUM(f64), COUNT(*) over a row range; sizes log-uniform [1e4, 5e7]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    sizes = [int(round(10e6 * 20 ** (i / 7))) for i in range(8)]
+    tables = []
+    for i, n in enumerate(sizes):
+        k = zipf_keys(torch, n, n, 1.1, g, torch.int64) if i % 2 else \
+            torch.randint(0, n, (n,), device="cuda", generator=g, dtype=torch.int64)
+        a = torch.randint(-(2**40), 2**40, (n,), device="cuda", generator=g, dtype=torch.int64)
+        f = torch.rand(n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1
+        tables.append([C.DeviceColumn(C.INT64, n, k.view(torch.uint8)), C.DeviceColumn(C.INT64, n, a.view(torch.uint8)),
+                       C.DeviceColumn(C.DOUBLE, n, f.view(torch.uint8))])
+    queries = []
+    for j in range(nq):
+        t = int(rng.integers(0, 8))
+        n = sizes[t]
+        s = int(np.exp(rng.uniform(np.log(1e4), np.log(5e7))))
+        if j % 2 == 0:
+            pk = torch.randint(0, n, (s,), device="cuda", generator=g, dtype=torch.int64)
+            pp = torch.rand(s, device="cuda", generator=g, dtype=torch.float64)
+            queries.append(("join", t, [C.DeviceColumn(C.INT64, s, pk.view(torch.uint8)),
+                                        C.DeviceColumn(C.DOUBLE, s, pp.view(torch.uint8))]))
+        else:
+            s = min(s, n)
+            b = int(rng.integers(0, (n - s) // 64 + 1)) * 64
+            queries.append(("groupby", t, (b, min(n, b + s))))
+    return tables, queries, sizes
+
+
+def _table_slice(C, cols, b, e):
+    return [C.DeviceColumn(c.type_id, e - b, c.data[b * C.WIDTH[c.type_id]:e * C.WIDTH[c.type_id]]) for c in cols]
+
+
+def run_served(args, torch, C):
+    import threading
+
+    import numpy as np
+
+    nq = args.queries
+    t_start = time.perf_counter()
+    tables, queries, sizes = make_served_inputs(torch, C, nq)
+    torch.cuda.synchronize()
+    log(f"[served] inputs ready: 8 tables, {nq} queries ({time.perf_counter() - t_start:.1f} s)")
+
+    # plain sequential: one plain call per query (the join builds the table's hash table each time)
+    svc = []
+    for j, (kind, t, arg) in enumerate(queries):
+        if j % 20 == 0:
+            log(f"[served] plain sequential: query {j}/{nq} ({time.perf_counter() - t_start:.1f} s)")
+        t0 = time.perf_counter()
+        if kind == "join":
+            C.join_raw(arg, tables[t], [0], [0], 0).free()
+        else:
+            C.groupby_raw(_table_slice(C, tables[t], *arg), [0], SERVED_AGGS).free()
+        torch.cuda.synchronize()
+        svc.append(time.perf_counter() - t0)
+    svc = np.array(svc)
+    C.pool_release()
+
+    srv = C.Server(hbm_budget_bytes=200e9, slots=4, max_batch=64)
+    t0 = time.perf_counter()
+    for tb in tables:
+        srv.add_table(tb)
+    index_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[served] server up, indexes built in {index_ms:.0f} ms")
+
+    def serve(arrivals):
+        ids = [None] * nq
+        start = time.perf_counter()
+
+        def submitter():
+            for j, (kind, t, arg) in enumerate(queries):
+                d = start + arrivals[j] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[j] = srv.submit_join(t, arg, 0) if kind == "join" else srv.submit_groupby(t, arg[0], arg[1],
+                                                                                              SERVED_AGGS)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, ends, firsts, batch = [], [], [], []
+        for j in range(nq):
+            while ids[j] is None:
+                time.sleep(0.0002)
+            if j % 50 == 0:
+                log(f"[served] waiting for query {j}/{nq}")
+            _, st = srv.wait(ids[j])
+            srv.release(ids[j])
+            lat.append((st["end_ns"] - st["submit_ns"]) * 1e-6)
+            ends.append(st["end_ns"])
+            firsts.append(st["submit_ns"])
+            batch.append(st["batch_queries"])
+        th.join()
+        lat = np.array(lat)
+        wall = (max(ends) - min(firsts)) * 1e-9
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "queries_per_s": nq / wall, "mean_batch": float(np.mean(batch))}
+
+    burst = serve(np.zeros(nq))  # every query at once: the server's capacity
+    log(f"[served] burst done: {burst}")
+    rate = 0.7 * burst["queries_per_s"]
+    rng = np.random.default_rng(55)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nq))
+    poisson = serve(arrivals)
+    log(f"[served] poisson done: {poisson}")
+    srv.close()
+    # the plain path under the same arrivals: one query at a time, first come first served
+    t_free, lat_plain = 0.0, []
+    for a, s_ in zip(arrivals, svc):
+        t_free = max(t_free, a) + s_
+        lat_plain.append((t_free - a) * 1e3)
+    lat_plain = np.array(lat_plain)
+    del tables, queries
+    torch.cuda.empty_cache()
+    C.pool_release()
+    return {"workload": f"{nq} queries (half inner joins of a probe with a table, half groupby SUM,SUM,COUNT over "
+                        "a row range; sizes log-uniform [1e4, 5e7]) over 8 resident tables of "
+                        f"{sizes[0] // 10**6}M..{sizes[-1] // 10**6}M rows (int64 keys uniform / Zipf(1.1)); "
+                        "HBM budget 200 GB, 4 slots",
+            "served_burst": burst, "served_poisson": {"arrival_rate_per_s": rate, **poisson},
+            "plain_sequential": {"queries_per_s": nq / float(svc.sum()), "mean_service_ms": float(svc.mean() * 1e3),
+                                 "p50_ms_at_same_arrivals": float(np.percentile(lat_plain, 50)),
+                                 "p99_ms_at_same_arrivals": float(np.percentile(lat_plain, 99))},
+            "throughput_vs_sequential": burst["queries_per_s"] / (nq / float(svc.sum())),
+            "index_build_ms": index_ms}
+
+
 def run_partition(args, torch, C):
     n, P = args.rows or 1_000_000_000, 256
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -455,9 +591,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--workload", choices=["groupby", "join", "join_left", "join_full", "partition", "batched"],
+    ap.add_argument("--workload",
+                    choices=["groupby", "join", "join_left", "join_full", "partition", "batched", "served"],
                     default="groupby")
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -490,6 +628,8 @@ def main():
         res = run_join(args, torch, C)
     elif args.workload == "batched":
         res = run_batched(args, torch, C)
+    elif args.workload == "served":
+        res = run_served(args, torch, C)
     elif args.workload in ("join_left", "join_full"):
         res = run_join_c3(args, torch, C, 1 if args.workload == "join_left" else 2)
     else:

@@ -268,6 +268,60 @@ int ldf_batch_result(ldf_batch* batch, void* stream, ldf_table_out* out, int64_t
 int ldf_batch_config_used(const ldf_batch* batch, ldf_batch_config* cfg);
 void ldf_batch_destroy(ldf_batch* batch);
 
+/* ---- served form --------------------------------------------------------------------------
+ * A query server over device-resident tables.  Tables are registered once (each gets a
+ * persistent join index on its key column, built by the join kernels).  Queries are submitted
+ * asynchronously:
+ *   LDF_QUERY_JOIN     inner join of a probe table (lhs, key column probe_key) with a registered
+ *                      table (rhs, its key column), nulls_equal = true; output = every probe
+ *                      column then every table column, as ldf_join would return them;
+ *   LDF_QUERY_GROUPBY  groupby of the table rows [row_begin, row_end) on the table's key column
+ *                      with `aggs` (kinds of the batched form), as ldf_groupby would return it.
+ * Continuous batching: `slots` worker threads, each with its own HIP stream, repeatedly take the
+ * waiting queries -- the oldest one and, behind it, those of the same kind (and aggregations)
+ * that still fit -- while the memory they are estimated to need fits in `hbm_budget_bytes`
+ * (resident indexes and unreleased results count against it), and run them as one batch; the
+ * batches of different slots overlap on the GPU.  A groupby over more than 2^21 rows runs
+ * through ldf_groupby on the slot's stream.  ldf_server_wait blocks until a query finished and
+ * describes its result (buffers owned by the server until ldf_server_release).  Results are
+ * those of the plain entry points on the same query (row order aside), alone or in any mix.
+ * No counterpart in the reference. */
+enum { LDF_QUERY_JOIN = 0, LDF_QUERY_GROUPBY = 1 };
+
+typedef struct ldf_server_config {
+  double hbm_budget_bytes; /* memory the server may hold: indexes, batches in flight, results */
+  int32_t slots;           /* concurrent batches (worker threads / HIP streams), 1..16 */
+  int32_t max_batch;       /* queries per batch */
+} ldf_server_config;
+
+typedef struct ldf_query {
+  int32_t kind;  /* LDF_QUERY_* */
+  int32_t table; /* registered table id */
+  ldf_table_view probe; /* JOIN: probe columns (device, valid until the query finished) */
+  int32_t probe_key;
+  int32_t naggs;        /* GROUPBY */
+  const ldf_agg* aggs;  /* GROUPBY (copied at submit) */
+  int64_t row_begin;    /* GROUPBY: a multiple of 64 */
+  int64_t row_end;
+} ldf_query;
+
+typedef struct ldf_query_stats {
+  int64_t submit_ns; /* steady-clock times */
+  int64_t start_ns;
+  int64_t end_ns;
+  int32_t batch_queries; /* queries in the batch it ran in */
+  int32_t plain;         /* 1 = ran through the plain entry point */
+} ldf_query_stats;
+
+typedef struct ldf_server ldf_server;
+
+int ldf_server_create(const ldf_server_config* cfg, ldf_server** server);
+int ldf_server_add_table(ldf_server* server, const ldf_table_view* table, int32_t key_col, int32_t* table_id);
+int ldf_server_submit(ldf_server* server, const ldf_query* query, int64_t* query_id);
+int ldf_server_wait(ldf_server* server, int64_t query_id, ldf_table_out* out, ldf_query_stats* stats);
+int ldf_server_release(ldf_server* server, int64_t query_id);
+void ldf_server_destroy(ldf_server* server);
+
 const char* ldf_last_error(void);
 const char* ldf_version(void);
 

@@ -32,6 +32,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -94,6 +97,8 @@ struct JInst {
   int64_t brow;   // first build row in the concatenated build arrays
   int64_t prow;   // first probe row in the concatenated probe arrays
   int64_t ptile;  // first probe tile in the concatenated tile arrays
+  BSlot* tabp;            // the instance's table (cap + 2 slots)
+  const uint32_t* rowsp;  // build rows grouped by slot; entries are brow + local row
 };
 
 __global__ void k_bj_fill(BSlot* __restrict__ t, int64_t n)
@@ -123,16 +128,17 @@ __device__ __forceinline__ int64_t bj_insert(const JInst& I, int w, int64_t r, B
   }
 }
 
-__device__ __forceinline__ int64_t bj_find(const JInst& I, int w, int64_t r, const BSlot* tab, bool ne)
+// slot of the probe row's key in the instance's table (local index), -1 = no build row has it
+__device__ __forceinline__ int64_t bj_find(const JInst& I, int w, int64_t r, bool ne)
 {
-  if (!valid_at(I.pvalid, r)) return ne ? I.tab + (int64_t)I.cap + 1 : -1;
+  if (!valid_at(I.pvalid, r)) return ne ? (int64_t)I.cap + 1 : -1;
   const unsigned long long k = load_bits(I.pkey, w, r);
-  if (k == kBE) return I.tab + (int64_t)I.cap;
-  const BSlot* t = tab + I.tab;
+  if (k == kBE) return (int64_t)I.cap;
+  const BSlot* t = I.tabp;
   uint64_t i     = mix64(k) & I.mask;
   while (true) {
     unsigned long long cur = t[i].key;
-    if (cur == k) return I.tab + (int64_t)i;
+    if (cur == k) return (int64_t)i;
     if (cur == kBE) return -1;
     i = (i + 1) & I.mask;
   }
@@ -194,8 +200,7 @@ __global__ void k_bj_group(const int64_t* __restrict__ slot_of_row, int64_t nbt,
 // info[prow + r] = start | cnt << 32; btot[ptile + tile] = output rows of the tile
 __global__ __launch_bounds__(kQT) void k_bj_count(const JInst* __restrict__ inst, const Item* __restrict__ items,
                                                   int nitems, unsigned int* cursor, int tile, int w, bool ne, int jt,
-                                                  const BSlot* __restrict__ tab, uint64_t* __restrict__ info,
-                                                  uint64_t* __restrict__ btot)
+                                                  uint64_t* __restrict__ info, uint64_t* __restrict__ btot)
 {
   __shared__ int sh;
   __shared__ uint64_t smem[17];
@@ -208,11 +213,11 @@ __global__ __launch_bounds__(kQT) void k_bj_count(const JInst* __restrict__ inst
     const int64_t r1 = r0 + tile < I.np ? r0 + tile : I.np;
     uint64_t sum     = 0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kQT) {
-      const int64_t s = bj_find(I, w, r, tab, ne);
+      const int64_t s = bj_find(I, w, r, ne);
       uint32_t cnt = 0, st = 0;
       if (s >= 0) {
-        cnt = tab[s].cnt;
-        st  = tab[s].start;
+        cnt = I.tabp[s].cnt;
+        st  = I.tabp[s].start;
       }
       info[I.prow + r] = (uint64_t)st | ((uint64_t)cnt << 32);
       sum += bj_out_rows(cnt, jt);
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kQT) void k_bj_count(const JInst* __restrict__ inst
 // row offsets in LDS), writing instance-local row numbers (-1 = no rhs row).
 __global__ __launch_bounds__(kQT) void k_bj_emit(const JInst* __restrict__ inst, const Item* __restrict__ items,
                                                  int nitems, unsigned int* cursor, int tile, int jt,
-                                                 const uint64_t* __restrict__ info, const uint32_t* __restrict__ rows,
+                                                 const uint64_t* __restrict__ info,
                                                  const uint64_t* __restrict__ boff, int64_t out_cap,
                                                  int32_t* __restrict__ lmap, int32_t* __restrict__ rmap,
                                                  int32_t* __restrict__ flags)
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(kQT) void k_bj_emit(const JInst* __restrict__ inst,
       const uint64_t gq = gbase + q;
       lmap[gq]          = (int32_t)(base + lo);
       if (rmap != nullptr)
-        rmap[gq] = mc[lo] == 0 ? -1 : (int32_t)((int64_t)rows[st[lo] + (q - off[lo])] - I.brow);
+        rmap[gq] = mc[lo] == 0 ? -1 : (int32_t)((int64_t)I.rowsp[st[lo] + (q - off[lo])] - I.brow);
     }
   }
 }
@@ -886,9 +891,26 @@ struct BatchImpl {
     if (!v.empty()) LDF_HIP_TRY(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, cs));
     return d;
   }
+  // Buffers go back to the pool under the stream they were allocated on; runs on other streams
+  // (the public API allows any) are waited for with a device synchronisation unless the owner
+  // only ever ran the batch on `cs`.
+  bool only_cs = false;
+  // frees every plan buffer except `keep` (after a build whose other buffers are no longer used)
+  void keep_only(const std::vector<const void*>& keep)
+  {
+    LDF_HIP_TRY(hipStreamSynchronize(cs));
+    Alloc a(nullptr, cs);
+    std::vector<std::pair<void*, size_t>> rest;
+    for (auto& o : owned) {
+      if (std::find(keep.begin(), keep.end(), (const void*)o.first) != keep.end()) rest.push_back(o);
+      else a.deallocate(o.first, o.second);
+    }
+    owned.swap(rest);
+  }
   void release()
   {
-    (void)hipDeviceSynchronize();
+    if (only_cs) (void)hipStreamSynchronize(cs);
+    else (void)hipDeviceSynchronize();
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture) (void)hipStreamDestroy(capture);
@@ -947,7 +969,8 @@ std::vector<int> by_size(const std::vector<int64_t>& sz)
 // ---- join batch ---------------------------------------------------------------------------
 struct JoinBatch : BatchImpl {
   int ninst = 0, w = 0, jt = 0, tile = 0, btile = 0, nout = 0, grid_q = 0;
-  bool ne = true;
+  bool ne       = true;
+  bool prebuilt = false;  // probes prebuilt indexes: no build phases
   int64_t nslot = 0, nbt = 0, npt = 0, ntiles = 0, out_cap = 0;
   int nbitems = 0, npitems = 0;
   JInst* d_inst     = nullptr;
@@ -978,6 +1001,21 @@ struct JoinBatch : BatchImpl {
   {
     fill_u32(cursors, 0, 4, s);
     fill_u32(reinterpret_cast<uint32_t*>(flags), 0, 2, s);
+    if (!prebuilt) build(s);
+    if (npitems > 0) {
+      hipLaunchKernelGGL(k_bj_count, dim3(std::min(grid_q, npitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
+                         (const Item*)d_pitems, npitems, cursors + 1, tile, w, ne, jt, info, btot);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+    exclusive_scan_u64_ws(btot, boff, ntiles, ws, s);
+    hipLaunchKernelGGL(k_bj_ooff, dim3(grid_for(ninst + 1, 256)), dim3(256), 0, s, (const JInst*)d_inst, ninst,
+                       (const uint64_t*)boff, ntiles, ooff);
+    LDF_HIP_TRY(hipGetLastError());
+  }
+
+  // hash tables of all instances' build sides (CSR multimaps)
+  void build(hipStream_t s)
+  {
     hipLaunchKernelGGL(k_bj_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab, nslot);
     LDF_HIP_TRY(hipGetLastError());
     if (nbitems > 0) {
@@ -996,26 +1034,23 @@ struct JoinBatch : BatchImpl {
                          (const BSlot*)tab, fillc, rows);
       LDF_HIP_TRY(hipGetLastError());
     }
-    if (npitems > 0) {
-      hipLaunchKernelGGL(k_bj_count, dim3(std::min(grid_q, npitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
-                         (const Item*)d_pitems, npitems, cursors + 1, tile, w, ne, jt, (const BSlot*)tab, info, btot);
-      LDF_HIP_TRY(hipGetLastError());
-    }
-    exclusive_scan_u64_ws(btot, boff, ntiles, ws, s);
-    hipLaunchKernelGGL(k_bj_ooff, dim3(grid_for(ninst + 1, 256)), dim3(256), 0, s, (const JInst*)d_inst, ninst,
-                       (const uint64_t*)boff, ntiles, ooff);
-    LDF_HIP_TRY(hipGetLastError());
   }
 
   void enqueue(hipStream_t s) override
   {
     enqueue_count(s);
+    enqueue_tail(s);
+  }
+
+  // emit + gather, after a count pass
+  void enqueue_tail(hipStream_t s)
+  {
     fill_u64(reinterpret_cast<uint64_t*>(nulls), 0, kMaxBOut, s);
     const size_t lds = (size_t)(tile + 1) * 8 + (size_t)tile * 8;
     if (npitems > 0 && out_cap > 0) {
       hipLaunchKernelGGL(k_bj_emit, dim3(std::min(grid_q, npitems)), dim3(kQT), lds, s, (const JInst*)d_inst,
                          (const Item*)d_pitems, npitems, cursors + 2, tile, jt, (const uint64_t*)info,
-                         (const uint32_t*)rows, (const uint64_t*)boff, out_cap, lmap, rmap, flags);
+                         (const uint64_t*)boff, out_cap, lmap, rmap, flags);
       LDF_HIP_TRY(hipGetLastError());
     }
     if (out_cap > 0 && nout > 0) {
@@ -1049,10 +1084,18 @@ struct JoinBatch : BatchImpl {
   }
 };
 
+// A built join index: the table and grouped rows of one build side (instance 0 of a JoinBatch
+// whose probe side is empty).
+struct JoinIndex {
+  BSlot* tab            = nullptr;
+  const uint32_t* rows  = nullptr;
+  uint64_t cap          = 0;
+};
+
 JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table_view* rhs, const int32_t* lhs_keys,
                        const int32_t* rhs_keys, int32_t nkeys, int32_t jt, bool ne, const int32_t* lhs_out,
                        int32_t n_lhs_out, const int32_t* rhs_out, int32_t n_rhs_out, const ldf_batch_config& cfg,
-                       hipStream_t s)
+                       hipStream_t s, const std::vector<JoinIndex>* prebuilt = nullptr)
 {
   LDF_EXPECTS(ninst >= 1 && lhs != nullptr && rhs != nullptr, "batch join: no instances");
   LDF_EXPECTS(nkeys == 1 && lhs_keys != nullptr && rhs_keys != nullptr, "batch join: exactly one key column");
@@ -1131,14 +1174,17 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
     I.bkey     = rhs[i].cols[rk].data;
     I.bvalid   = rhs[i].cols[rk].validity;
     I.nb       = nb[i];
-    I.cap      = next_pow2((uint64_t)std::max<double>(64.0, std::ceil((double)nb[i] / cfg.load_factor)));
+    I.cap      = prebuilt ? (*prebuilt)[i].cap
+                          : next_pow2((uint64_t)std::max<double>(64.0, std::ceil((double)nb[i] / cfg.load_factor)));
     I.mask     = I.cap - 1;
     I.tab      = tabo;
-    I.brow     = brow;
+    I.brow     = prebuilt ? 0 : brow;
     I.prow     = prow;
     I.ptile    = ptile;
-    tabo += (int64_t)I.cap + 2;
-    brow += nb[i];
+    if (!prebuilt) {
+      tabo += (int64_t)I.cap + 2;
+      brow += nb[i];
+    }
     prow += np[i];
     ptile += (np[i] + b->tile - 1) / b->tile;
   }
@@ -1149,7 +1195,8 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
   LDF_EXPECTS(brow < (int64_t)UINT32_MAX, "batch join: at most 2^32-1 build rows per batch");
   std::vector<Item> bi, pi;
   for (int i : by_size(np)) {
-    for (int64_t t = 0; t < (nb[i] + b->btile - 1) / b->btile; ++t) bi.push_back({i, (int32_t)t});
+    if (!prebuilt)
+      for (int64_t t = 0; t < (nb[i] + b->btile - 1) / b->btile; ++t) bi.push_back({i, (int32_t)t});
     for (int64_t t = 0; t < (np[i] + b->tile - 1) / b->tile; ++t) pi.push_back({i, (int32_t)t});
   }
   LDF_EXPECTS(bi.size() < (size_t)INT32_MAX / 2 && pi.size() < (size_t)INT32_MAX / 2, "batch join: too many tiles");
@@ -1157,9 +1204,7 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
   b->npitems = (int)pi.size();
   b->grid_q  = resident_blocks(kQT, (size_t)(b->tile + 1) * 8 + (size_t)b->tile * 8);
 
-  b->d_inst   = b->upload(hi);
-  b->d_bitems = b->upload(bi);
-  b->d_pitems = b->upload(pi);
+  b->prebuilt = prebuilt != nullptr;
   b->tab      = b->dnew<BSlot>(b->nslot);
   b->slot_row = b->dnew<int64_t>(b->nbt);
   b->rows     = b->dnew<uint32_t>(b->nbt);
@@ -1174,6 +1219,13 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
   b->cursors  = b->dnew<uint32_t>(4);
   b->flags    = b->dnew<int32_t>(2);
   b->nulls    = b->dnew<unsigned long long>(kMaxBOut);
+  for (int i = 0; i < ninst; ++i) {
+    hi[i].tabp  = prebuilt ? (*prebuilt)[i].tab : b->tab + hi[i].tab;
+    hi[i].rowsp = prebuilt ? (*prebuilt)[i].rows : b->rows;
+  }
+  b->d_inst   = b->upload(hi);
+  b->d_bitems = b->upload(bi);
+  b->d_pitems = b->upload(pi);
   const size_t lds = (size_t)(b->tile + 1) * 8 + (size_t)b->tile * 8;
   if (lds > 64 * 1024) LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_bj_emit, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
@@ -1267,6 +1319,17 @@ struct GroupbyBatch : BatchImpl {
     hipLaunchKernelGGL(k_bg_ooff, dim3(grid_for(ninst + 1, 256)), dim3(256), 0, s, (const GInst*)d_inst, ninst,
                        (const uint64_t*)gpos, nparts, ooff);
     LDF_HIP_TRY(hipGetLastError());
+    if (out_cap > 0) {
+      hipLaunchKernelGGL(k_bg_gather, dim3(grid_for(out_cap, 256, 256 * 16)), dim3(256), 0, s, p,
+                         (const uint64_t*)gpos, nparts, out_cap, nulls);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+  }
+
+  // the final gather only, after create's sizing run (whose partition results are still in place)
+  void enqueue_tail(hipStream_t s)
+  {
+    fill_u64(reinterpret_cast<uint64_t*>(nulls), 0, kMaxBAgg + 1, s);
     if (out_cap > 0) {
       hipLaunchKernelGGL(k_bg_gather, dim3(grid_for(out_cap, 256, 256 * 16)), dim3(256), 0, s, p,
                          (const uint64_t*)gpos, nparts, out_cap, nulls);
@@ -1555,6 +1618,7 @@ GroupbyBatch* create_groupby(int32_t ninst, const ldf_table_view* tables, const 
       }
     }
     if (!grew) throw Error(LDF_E_UNSUPPORTED, "batch groupby: an instance has too many groups for the batched form");
+    b->only_cs = true;  // it only ever ran on s
     b.reset();
     return create_groupby(ninst, tables, keys, nkeys, aggs, naggs, cfg, s, &nb2);
   }
@@ -1776,10 +1840,499 @@ ldf_batch_config resolve_groupby_config(const ldf_batch_config* in, int32_t nins
   return best;
 }
 
+// =========================================================================================
+// served form
+// =========================================================================================
+// dst bit j = src bit (off + j), for j < n (whole 64-bit destination words)
+__global__ void k_copy_bits(const uint64_t* __restrict__ src, int64_t src_words, int64_t off, int64_t n,
+                            uint64_t* __restrict__ dst)
+{
+  const int64_t nw = (n + 63) / 64;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nw; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b  = off + 64 * j;
+    const int64_t w  = b >> 6;
+    const int sh     = (int)(b & 63);
+    uint64_t lo      = w < src_words ? src[w] : 0;
+    uint64_t hi      = (sh != 0 && w + 1 < src_words) ? src[w + 1] : 0;
+    uint64_t v       = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    const int64_t rem = n - 64 * j;
+    if (rem < 64) v &= (1ull << rem) - 1;
+    dst[j] = v;
+  }
+}
+
+int64_t now_ns()
+{
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+    .count();
+}
+
+struct ServedTable {
+  std::vector<ldf_column_view> cols;
+  int32_t key_col = 0;
+  int64_t rows    = 0;
+  std::unique_ptr<JoinBatch> index;  // instance 0 builds the table's key index
+  JoinIndex idx;
+  size_t bytes = 0;
+};
+
+struct Query {
+  int64_t id = 0;
+  ldf_query q{};
+  std::vector<ldf_column_view> probe_cols;
+  std::vector<ldf_agg> aggs;
+  bool plain       = false;
+  size_t est_bytes = 0;
+  size_t res_bytes = 0;
+  int status       = 1;  // 1 waiting or running, 0 done, < 0 failed
+  std::string err;
+  std::vector<ldf_column> cols;
+  ldf_query_stats st{};
+};
+
+size_t column_bytes(const ldf_column& c)
+{
+  size_t b = (size_t)std::max<int64_t>(0, c.data_bytes) + (size_t)std::max<int64_t>(0, c.validity_bytes);
+  if (c.type_id == LDF_TYPE_LARGE_STRING) b += (size_t)(c.length + 1) * 8;
+  return b;
+}
+
+struct Server {
+  ldf_server_config cfg{};
+  int dev = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> waiting;
+  std::map<int64_t, std::unique_ptr<Query>> queries;
+  std::vector<std::unique_ptr<ServedTable>> tables;
+  double held    = 0;  // bytes: indexes, batches in flight (estimates), unreleased results
+  int running    = 0;
+  bool stop      = false;
+  int64_t next_id = 1;
+  std::vector<std::thread> workers;
+  std::vector<hipStream_t> streams;
+
+  ~Server()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    for (auto& t : workers) t.join();
+    (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();
+    Alloc a(nullptr, nullptr);
+    for (auto& kv : queries)
+      for (auto& c : kv.second->cols) free_column(&c, a);
+    queries.clear();
+    tables.clear();
+    for (auto s : streams) (void)hipStreamDestroy(s);
+    (void)hipGetLastError();
+  }
+
+  // estimated device bytes a query needs while it runs and for its result
+  size_t estimate(const Query& q) const
+  {
+    if (q.q.kind == LDF_QUERY_JOIN) {
+      const ServedTable& t = *tables[q.q.table];
+      int64_t row_bytes    = 0;
+      for (auto& c : q.probe_cols) row_bytes += std::max(1, type_width(c.type_id));
+      for (auto& c : t.cols) row_bytes += std::max(1, type_width(c.type_id));
+      const int64_t np = q.probe_cols.empty() ? 0 : q.probe_cols[0].length;
+      return (size_t)np * (size_t)(16 + 2 * (row_bytes + 8)) + (1u << 20);
+    }
+    const int64_t n = q.q.row_end - q.q.row_begin;
+    return (size_t)n * (q.plain ? 96 : 72) + (1u << 20);
+  }
+
+  bool same_batch(const Query& a, const Query& b) const
+  {
+    if (a.q.kind != b.q.kind || a.plain || b.plain) return false;
+    if (a.q.kind == LDF_QUERY_JOIN) return true;
+    if (a.aggs.size() != b.aggs.size()) return false;
+    for (size_t i = 0; i < a.aggs.size(); ++i)
+      if (a.aggs[i].in_col != b.aggs[i].in_col || a.aggs[i].kind != b.aggs[i].kind ||
+          a.aggs[i].count_mode != b.aggs[i].count_mode)
+        return false;
+    return true;
+  }
+
+  // Takes the next batch: the oldest waiting query (run even over budget when nothing else runs,
+  // so a query larger than the budget cannot wait forever) and the later ones that batch with it
+  // and still fit.  Called with mu held.
+  bool take(std::vector<Query*>& out)
+  {
+    if (waiting.empty()) return false;
+    Query* head = queries[waiting.front()].get();
+    if (held + head->est_bytes > cfg.hbm_budget_bytes && running > 0) return false;
+    double used = held + head->est_bytes;
+    out.push_back(head);
+    std::deque<int64_t> rest;
+    waiting.pop_front();
+    while (!waiting.empty()) {
+      const int64_t id = waiting.front();
+      waiting.pop_front();
+      Query* q = queries[id].get();
+      if ((int)out.size() < cfg.max_batch && same_batch(*head, *q) && used + q->est_bytes <= cfg.hbm_budget_bytes) {
+        used += q->est_bytes;
+        out.push_back(q);
+      } else {
+        rest.push_back(id);
+      }
+    }
+    waiting.swap(rest);
+    held = used;
+    return true;
+  }
+
+  void worker(int slot)
+  {
+    (void)hipSetDevice(dev);
+    hipStream_t s = streams[slot];
+    for (;;) {
+      std::vector<Query*> batch;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || take(batch); });
+        if (batch.empty()) return;  // stopping
+        ++running;
+        const int64_t t = now_ns();
+        for (Query* q : batch) {
+          q->st.start_ns      = t;
+          q->st.batch_queries = (int32_t)batch.size();
+          q->st.plain         = q->plain ? 1 : 0;
+        }
+      }
+      int rc = LDF_OK;
+      std::string msg;
+      try {
+        execute(batch, s);
+      } catch (const Error& e) {
+        rc  = e.code;
+        msg = e.what();
+      } catch (const std::exception& e) {
+        rc  = LDF_E_INTERNAL;
+        msg = e.what();
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        const int64_t t = now_ns();
+        for (Query* q : batch) {
+          held -= (double)q->est_bytes;
+          if (rc == LDF_OK) {
+            held += (double)q->res_bytes;
+            q->status = 0;
+          } else {
+            q->status = rc;
+            q->err    = msg;
+          }
+          q->st.end_ns = t;
+        }
+        --running;
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+
+  // copies rows [lo, lo + n) of batch column c into a result column of the query
+  void copy_rows(const ldf_column& c, int64_t lo, int64_t n, ldf_column* o, Alloc& a, hipStream_t s)
+  {
+    alloc_column(c.type_id, n, c.validity != nullptr, 0, a, o);
+    const int w = type_width(c.type_id);
+    if (n > 0)
+      LDF_HIP_TRY(hipMemcpyAsync(o->data, static_cast<const uint8_t*>(c.data) + lo * w, (size_t)n * w,
+                                 hipMemcpyDeviceToDevice, s));
+    if (c.validity != nullptr && n > 0) {
+      hipLaunchKernelGGL(k_copy_bits, dim3(grid_for((n + 63) / 64, 256)), dim3(256), 0, s,
+                         reinterpret_cast<const uint64_t*>(c.validity), (c.validity_bytes + 7) / 8, lo, n,
+                         reinterpret_cast<uint64_t*>(o->validity));
+      LDF_HIP_TRY(hipGetLastError());
+    }
+  }
+
+  void execute(std::vector<Query*>& batch, hipStream_t s)
+  {
+    Alloc a(nullptr, s);
+    Query& h = *batch.front();
+    if (h.plain) {  // large groupby: the plain entry point on this stream
+      const ServedTable& t = *tables[h.q.table];
+      std::vector<ldf_column_view> v = slice(t, h.q.row_begin, h.q.row_end);
+      ldf_table_view tv{(int32_t)v.size(), v.data()};
+      h.cols.resize(1 + h.aggs.size());
+      ldf_table_out out{(int32_t)h.cols.size(), h.cols.data()};
+      const int rc = ldf_groupby(&tv, &t.key_col, 1, h.aggs.data(), (int32_t)h.aggs.size(), nullptr, s, &out);
+      if (rc != LDF_OK) throw Error(rc, ldf_last_error());
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      h.res_bytes = 0;
+      for (auto& c : h.cols) h.res_bytes += column_bytes(c);
+      return;
+    }
+    const int n = (int)batch.size();
+    ldf_batch_config cfg{0.5, 2048, 1024, 0};
+    std::unique_ptr<BatchImpl> b;
+    std::vector<std::vector<ldf_column_view>> views(n);
+    std::vector<ldf_table_view> tv(n);
+    if (h.q.kind == LDF_QUERY_JOIN) {
+      std::vector<ldf_table_view> rv(n);
+      std::vector<JoinIndex> idx(n);
+      for (int k = 0; k < n; ++k) {
+        const Query& q       = *batch[k];
+        const ServedTable& t = *tables[q.q.table];
+        tv[k]                = {(int32_t)q.probe_cols.size(), q.probe_cols.data()};
+        rv[k]                = {(int32_t)t.cols.size(), t.cols.data()};
+        idx[k]               = t.idx;
+      }
+      const int32_t lk = h.q.probe_key, rk = tables[h.q.table]->key_col;
+      std::vector<int32_t> lo(h.probe_cols.size()), ro(tables[h.q.table]->cols.size());
+      for (size_t i = 0; i < lo.size(); ++i) lo[i] = (int32_t)i;
+      for (size_t i = 0; i < ro.size(); ++i) ro[i] = (int32_t)i;
+      JoinBatch* jb = create_join(n, tv.data(), rv.data(), &lk, &rk, 1, LDF_JOIN_INNER, true, lo.data(),
+                                  (int32_t)lo.size(), ro.data(), (int32_t)ro.size(), cfg, s, &idx);
+      b.reset(jb);
+      b->only_cs = true;
+      jb->enqueue_tail(s);
+    } else {
+      for (int k = 0; k < n; ++k) {
+        const Query& q = *batch[k];
+        views[k]       = slice(*tables[q.q.table], q.q.row_begin, q.q.row_end);
+        tv[k]          = {(int32_t)views[k].size(), views[k].data()};
+      }
+      const int32_t key = tables[h.q.table]->key_col;
+      GroupbyBatch* gb  = create_groupby(n, tv.data(), &key, 1, h.aggs.data(), (int32_t)h.aggs.size(), cfg, s);
+      b.reset(gb);
+      b->only_cs = true;
+      gb->enqueue_tail(s);
+    }
+    std::vector<ldf_column> cols(kMaxBOut + kMaxBAgg + 1);
+    ldf_table_out out{(int32_t)cols.size(), cols.data()};
+    std::vector<int64_t> offs(n + 1);
+    b->result(s, &out, offs.data());
+    for (int k = 0; k < n; ++k) {
+      Query& q = *batch[k];
+      q.cols.resize(out.ncols);
+      for (int c = 0; c < out.ncols; ++c) copy_rows(cols[c], offs[k], offs[k + 1] - offs[k], &q.cols[c], a, s);
+    }
+    // null counts of each result column (one small count per column)
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < n; ++k) {
+      Query& q    = *batch[k];
+      q.res_bytes = 0;
+      for (auto& c : q.cols) {
+        c.null_count = 0;
+        if (c.validity != nullptr && c.length > 0) {
+          std::vector<uint64_t> bits((size_t)(c.length + 63) / 64);
+          LDF_HIP_TRY(hipMemcpy(bits.data(), c.validity, bits.size() * 8, hipMemcpyDeviceToHost));
+          int64_t valid = 0;
+          for (uint64_t w : bits) valid += __builtin_popcountll(w);
+          c.null_count = c.length - valid;
+        }
+        q.res_bytes += column_bytes(c);
+      }
+    }
+  }
+
+  // table rows [lo, hi) as views (validity sliced at a byte boundary: lo is a multiple of 64)
+  std::vector<ldf_column_view> slice(const ServedTable& t, int64_t lo, int64_t hi) const
+  {
+    std::vector<ldf_column_view> v = t.cols;
+    for (auto& c : v) {
+      const int w = type_width(c.type_id);
+      c.data      = static_cast<const uint8_t*>(c.data) + lo * w;
+      if (c.validity) c.validity += lo / 8;
+      c.length     = hi - lo;
+      c.null_count = -1;
+    }
+    return v;
+  }
+};
+
 }  // namespace
 }  // namespace ldf
 
 using namespace ldf;
+
+struct ldf_server {
+  std::unique_ptr<Server> impl;
+};
+
+extern "C" int ldf_server_create(const ldf_server_config* cfg, ldf_server** server)
+{
+  return guarded([&] {
+    LDF_EXPECTS(cfg != nullptr && server != nullptr, "server: NULL argument");
+    LDF_EXPECTS(cfg->hbm_budget_bytes > 0, "server: hbm_budget_bytes must be positive");
+    LDF_EXPECTS(cfg->slots >= 1 && cfg->slots <= 16, "server: slots must be in 1..16");
+    LDF_EXPECTS(cfg->max_batch >= 1, "server: max_batch must be positive");
+    *server = nullptr;
+    std::unique_ptr<ldf_server> h(new ldf_server());
+    h->impl.reset(new Server());
+    Server& S = *h->impl;
+    S.cfg     = *cfg;
+    LDF_HIP_TRY(hipGetDevice(&S.dev));
+    for (int i = 0; i < cfg->slots; ++i) {
+      hipStream_t st;
+      LDF_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      S.streams.push_back(st);
+    }
+    for (int i = 0; i < cfg->slots; ++i) S.workers.emplace_back([&S, i] { S.worker(i); });
+    *server = h.release();
+  });
+}
+
+extern "C" int ldf_server_add_table(ldf_server* server, const ldf_table_view* table, int32_t key_col,
+                                    int32_t* table_id)
+{
+  return guarded([&] {
+    LDF_EXPECTS(server != nullptr && table != nullptr && table_id != nullptr, "server: NULL argument");
+    Server& S = *server->impl;
+    LDF_EXPECTS(key_col >= 0 && key_col < table->ncols, "server: key column out of range");
+    std::unique_ptr<ServedTable> t(new ServedTable());
+    t->cols.assign(table->cols, table->cols + table->ncols);
+    t->key_col = key_col;
+    t->rows    = table->ncols > 0 ? table->cols[0].length : 0;
+    for (auto& c : t->cols) {
+      validate_view(c, t->rows, "server table");
+      if (type_width(c.type_id) <= 0) throw Error(LDF_E_UNSUPPORTED, "server: table columns must be fixed-width");
+    }
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      if (!S.tables.empty()) {
+        const ServedTable& t0 = *S.tables.front();
+        LDF_EXPECTS(t0.cols.size() == t->cols.size() && t0.key_col == key_col,
+                    "server: every table has the schema of the first");
+        for (size_t c = 0; c < t->cols.size(); ++c)
+          LDF_EXPECTS(t0.cols[c].type_id == t->cols[c].type_id, "server: every table has the schema of the first");
+      }
+    }
+    // the key index: a join batch of one instance with an empty probe side
+    hipStream_t s = S.streams[0];
+    ldf_column_view empty = t->cols[key_col];
+    empty.length          = 0;
+    empty.validity        = nullptr;
+    ldf_table_view lt{1, &empty};
+    ldf_table_view rt{(int32_t)t->cols.size(), t->cols.data()};
+    const int32_t zero = 0;
+    ldf_batch_config cfg{0.5, 2048, 1024, 0};
+    t->index.reset(create_join(1, &lt, &rt, &zero, &key_col, 1, LDF_JOIN_INNER, true, nullptr, 0, nullptr, 0, cfg, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    t->idx.tab  = t->index->tab;
+    t->idx.rows = t->index->rows;
+    t->index->keep_only({t->index->tab, t->index->rows});
+    uint64_t cap = next_pow2((uint64_t)std::max<double>(64.0, std::ceil((double)t->rows / cfg.load_factor)));
+    t->idx.cap  = cap;
+    for (auto& o : t->index->owned) t->bytes += o.second;
+    std::lock_guard<std::mutex> lk(S.mu);
+    LDF_EXPECTS(S.held + (double)t->bytes <= S.cfg.hbm_budget_bytes, "server: the table index exceeds the HBM budget");
+    S.held += (double)t->bytes;
+    *table_id = (int32_t)S.tables.size();
+    S.tables.push_back(std::move(t));
+  });
+}
+
+extern "C" int ldf_server_submit(ldf_server* server, const ldf_query* query, int64_t* query_id)
+{
+  return guarded([&] {
+    LDF_EXPECTS(server != nullptr && query != nullptr && query_id != nullptr, "server: NULL argument");
+    Server& S = *server->impl;
+    std::unique_ptr<Query> q(new Query());
+    q->q            = *query;
+    q->st.submit_ns = now_ns();
+    std::lock_guard<std::mutex> lk(S.mu);
+    LDF_EXPECTS(query->table >= 0 && query->table < (int32_t)S.tables.size(), "server: unknown table");
+    const ServedTable& t = *S.tables[query->table];
+    if (query->kind == LDF_QUERY_JOIN) {
+      LDF_EXPECTS(query->probe.ncols >= 1 && query->probe.cols != nullptr, "server: a join query needs probe columns");
+      LDF_EXPECTS(query->probe_key >= 0 && query->probe_key < query->probe.ncols, "server: probe key out of range");
+      q->probe_cols.assign(query->probe.cols, query->probe.cols + query->probe.ncols);
+      const int64_t n = q->probe_cols[0].length;
+      for (auto& c : q->probe_cols) {
+        validate_view(c, n, "server probe");
+        if (type_width(c.type_id) <= 0) throw Error(LDF_E_UNSUPPORTED, "server: probe columns must be fixed-width");
+      }
+      LDF_EXPECTS(q->probe_cols[query->probe_key].type_id == t.cols[t.key_col].type_id,
+                  "join: lhs and rhs key column types differ");
+      for (auto& kv : S.queries) {  // one probe schema per server, so that joins batch together
+        const Query& o = *kv.second;
+        if (o.q.kind != LDF_QUERY_JOIN) continue;
+        LDF_EXPECTS(o.probe_cols.size() == q->probe_cols.size() && o.q.probe_key == query->probe_key,
+                    "server: every join query has the probe schema of the first");
+        for (size_t c = 0; c < q->probe_cols.size(); ++c)
+          LDF_EXPECTS(o.probe_cols[c].type_id == q->probe_cols[c].type_id,
+                      "server: every join query has the probe schema of the first");
+        break;
+      }
+    } else if (query->kind == LDF_QUERY_GROUPBY) {
+      LDF_EXPECTS(query->row_begin >= 0 && query->row_begin <= query->row_end && query->row_end <= t.rows,
+                  "server: groupby rows out of range");
+      LDF_EXPECTS(query->row_begin % 64 == 0, "server: row_begin must be a multiple of 64");
+      LDF_EXPECTS(query->naggs >= 0 && query->naggs <= kMaxBAgg && (query->naggs == 0 || query->aggs != nullptr),
+                  "server: 0..16 aggregations");
+      q->aggs.assign(query->aggs, query->aggs + query->naggs);
+      for (auto& g : q->aggs) {
+        if (g.kind != LDF_AGG_SUM && g.kind != LDF_AGG_MEAN && g.kind != LDF_AGG_COUNT && g.kind != LDF_AGG_COUNT_ALL &&
+            g.kind != LDF_AGG_MIN && g.kind != LDF_AGG_MAX)
+          throw Error(LDF_E_UNSUPPORTED, "server: groupby queries take SUM, MEAN, COUNT, COUNT_ALL, MIN, MAX");
+        LDF_EXPECTS(g.kind == LDF_AGG_COUNT_ALL || (g.in_col >= 0 && g.in_col < (int32_t)t.cols.size()),
+                    "groupby: aggregation column index out of range");
+      }
+      q->plain = (query->row_end - query->row_begin) > (int64_t)1 << 21;
+    } else {
+      throw Error(LDF_E_INVALID, "server: unknown query kind");
+    }
+    q->q.probe.cols = nullptr;
+    q->q.aggs       = nullptr;
+    q->est_bytes    = S.estimate(*q);
+    q->id           = S.next_id++;
+    *query_id       = q->id;
+    S.waiting.push_back(q->id);
+    S.queries[q->id] = std::move(q);
+    S.cv_work.notify_one();
+  });
+}
+
+extern "C" int ldf_server_wait(ldf_server* server, int64_t query_id, ldf_table_out* out, ldf_query_stats* stats)
+{
+  return guarded([&] {
+    LDF_EXPECTS(server != nullptr, "server: NULL argument");
+    Server& S = *server->impl;
+    std::unique_lock<std::mutex> lk(S.mu);
+    auto it = S.queries.find(query_id);
+    LDF_EXPECTS(it != S.queries.end(), "server: unknown query id");
+    Query* q = it->second.get();
+    S.cv_done.wait(lk, [&] { return q->status != 1; });
+    if (stats) *stats = q->st;
+    if (q->status < 0) throw Error(q->status, q->err);
+    if (out) {
+      LDF_EXPECTS(out->cols != nullptr && out->ncols >= (int32_t)q->cols.size(), "server: output table too small");
+      out->ncols = (int32_t)q->cols.size();
+      for (size_t c = 0; c < q->cols.size(); ++c) out->cols[c] = q->cols[c];
+    }
+  });
+}
+
+extern "C" int ldf_server_release(ldf_server* server, int64_t query_id)
+{
+  return guarded([&] {
+    LDF_EXPECTS(server != nullptr, "server: NULL argument");
+    Server& S = *server->impl;
+    std::unique_ptr<Query> q;
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      auto it = S.queries.find(query_id);
+      LDF_EXPECTS(it != S.queries.end(), "server: unknown query id");
+      LDF_EXPECTS(it->second->status != 1, "server: the query has not finished");
+      q = std::move(it->second);
+      S.queries.erase(it);
+      S.held -= (double)q->res_bytes;
+    }
+    Alloc a(nullptr, nullptr);
+    for (auto& c : q->cols) free_column(&c, a);
+    S.cv_work.notify_all();
+  });
+}
+
+extern "C" void ldf_server_destroy(ldf_server* server) { delete server; }
 
 struct ldf_batch {
   std::unique_ptr<BatchImpl> impl;

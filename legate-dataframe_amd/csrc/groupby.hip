@@ -126,6 +126,12 @@ struct KeyTab {
 };
 
 // ---- key table --------------------------------------------------------------------------
+// Probes an insertion may take before it gives up and flags an overflow.  A table filled to its
+// 3/4 limit with well-mixed hashes essentially never has a run this long, while a table that
+// has filled up (the distinct-count estimate was low; the call is then run again with four times
+// the capacity) would otherwise cost every further new key a scan of the whole table.
+__device__ __forceinline__ uint64_t probe_limit(const KeyTab& t) { return t.mask < 4095 ? t.mask + 1 : 4096; }
+
 // slot of a single fixed-width key given by its bits (kv = the key is valid)
 __device__ __forceinline__ int64_t gb_insert_bits(unsigned long long key, bool kv, const KeyTab& t)
 {
@@ -138,7 +144,7 @@ __device__ __forceinline__ int64_t gb_insert_bits(unsigned long long key, bool k
     return (int64_t)t.cap;
   }
   uint64_t i = mix64(key) & t.mask;
-  for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+  for (uint64_t probes = 0, lim = probe_limit(t); probes < lim; ++probes) {
     unsigned long long cur = atomic_load_relaxed(&t.slots[i]);
     if (cur == key) return (int64_t)i;
     if (cur == kGEmpty) {
@@ -171,7 +177,7 @@ __device__ __forceinline__ int64_t gb_find_or_insert(const DKeys& k, int64_t r, 
       return (int64_t)t.cap;
     }
     uint64_t i = mix64(key) & t.mask;
-    for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+    for (uint64_t probes = 0, lim = probe_limit(t); probes < lim; ++probes) {
       unsigned long long cur = atomic_load_relaxed(&t.slots[i]);
       if (cur == key) return (int64_t)i;
       if (cur == kGEmpty) {
@@ -192,7 +198,7 @@ __device__ __forceinline__ int64_t gb_find_or_insert(const DKeys& k, int64_t r, 
     uint32_t fp               = (uint32_t)(h >> 32);
     unsigned long long packed = ((unsigned long long)fp << 32) | (uint32_t)r;
     uint64_t i                = h & t.mask;
-    for (uint64_t probes = 0; probes <= t.mask; ++probes) {
+    for (uint64_t probes = 0, lim = probe_limit(t); probes < lim; ++probes) {
       unsigned long long cur = atomic_load_relaxed(&t.slots[i]);
       if (cur == kGEmpty) {
         unsigned long long old = atomicCAS(&t.slots[i], kGEmpty, packed);

@@ -32,7 +32,8 @@ EXPORTED = [
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
     "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release",
     "ldf_batch_join_create", "ldf_batch_groupby_create", "ldf_batch_run", "ldf_batch_result",
-    "ldf_batch_config_used", "ldf_batch_destroy",
+    "ldf_batch_config_used", "ldf_batch_destroy", "ldf_server_create", "ldf_server_add_table",
+    "ldf_server_submit", "ldf_server_wait", "ldf_server_release", "ldf_server_destroy",
 ]
 
 
@@ -552,3 +553,109 @@ def batch_groupby(tables, keys, aggs, config=None) -> Batch:
     check(L.ldf_batch_groupby_create(len(tables), ta, int32_array(keys), len(keys), arr, len(aggs),
                                      ctypes.byref(cfg) if cfg is not None else None, stream_ptr(), ctypes.byref(h)))
     return Batch(h, len(keys) + len(aggs), len(tables), (tables, ta, tk))
+
+
+# ---------------------------------------------------------------------------------------------
+# served form
+# ---------------------------------------------------------------------------------------------
+QUERY_JOIN, QUERY_GROUPBY = 0, 1
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("table", ctypes.c_int32), ("probe", TableView),
+                ("probe_key", ctypes.c_int32), ("naggs", ctypes.c_int32), ("aggs", ctypes.POINTER(Agg)),
+                ("row_begin", ctypes.c_int64), ("row_end", ctypes.c_int64)]
+
+
+class QueryStats(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_int64), ("start_ns", ctypes.c_int64), ("end_ns", ctypes.c_int64),
+                ("batch_queries", ctypes.c_int32), ("plain", ctypes.c_int32)]
+
+
+def _server_lib():
+    L = load()
+    if not getattr(L, "_server_types", False):
+        P = ctypes.POINTER
+        L.ldf_server_create.argtypes = [P(ServerConfig), P(ctypes.c_void_p)]
+        L.ldf_server_add_table.argtypes = [ctypes.c_void_p, P(TableView), ctypes.c_int32, P(ctypes.c_int32)]
+        L.ldf_server_submit.argtypes = [ctypes.c_void_p, P(Query), P(ctypes.c_int64)]
+        L.ldf_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, P(TableOut), P(QueryStats)]
+        L.ldf_server_release.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.ldf_server_destroy.argtypes = [ctypes.c_void_p]
+        L.ldf_server_destroy.restype = None
+        for f in ("ldf_server_create", "ldf_server_add_table", "ldf_server_submit", "ldf_server_wait",
+                  "ldf_server_release"):
+            getattr(L, f).restype = ctypes.c_int
+        L._server_types = True
+    return L
+
+
+class Server:
+    """Query server over device-resident tables (ldf_server_*): submit_* return a query id at once;
+    wait() blocks for the result (views of server-owned buffers, valid until release())."""
+
+    def __init__(self, hbm_budget_bytes: float = 200e9, slots: int = 4, max_batch: int = 64):
+        L = _server_lib()
+        self.h = ctypes.c_void_p()
+        check(L.ldf_server_create(ctypes.byref(ServerConfig(float(hbm_budget_bytes), slots, max_batch)),
+                                  ctypes.byref(self.h)))
+        self._keep = {}
+        self._tables = []
+
+    def add_table(self, cols, key_col: int = 0) -> int:
+        tv, arr = make_table_view([c.view() for c in cols])
+        tid = ctypes.c_int32()
+        check(_server_lib().ldf_server_add_table(self.h, ctypes.byref(tv), key_col, ctypes.byref(tid)))
+        self._tables.append((cols, arr))
+        return tid.value
+
+    def _submit(self, q: Query, keep) -> int:
+        qid = ctypes.c_int64()
+        check(_server_lib().ldf_server_submit(self.h, ctypes.byref(q), ctypes.byref(qid)))
+        self._keep[qid.value] = keep
+        return qid.value
+
+    def submit_join(self, table: int, probe_cols, probe_key: int = 0) -> int:
+        tv, arr = make_table_view([c.view() for c in probe_cols])
+        q = Query(QUERY_JOIN, table, tv, probe_key, 0, None, 0, 0)
+        return self._submit(q, (probe_cols, arr))
+
+    def submit_groupby(self, table: int, row_begin: int, row_end: int, aggs) -> int:
+        arr = (Agg * max(1, len(aggs)))(*[Agg(-1 if c is None else c, AGG[k], COUNT_MODE[m or "only_valid"], 0)
+                                          for c, k, m in aggs])
+        q = Query(QUERY_GROUPBY, table, TableView(0, None), 0, len(aggs), arr, row_begin, row_end)
+        return self._submit(q, arr)
+
+    def wait(self, qid: int):
+        cols = (Column * 40)()
+        out = TableOut(40, cols)
+        st = QueryStats()
+        check(_server_lib().ldf_server_wait(self.h, qid, ctypes.byref(out), ctypes.byref(st)))
+        res = []
+        for i in range(out.ncols):
+            c = out.cols[i]
+            data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
+            valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
+            res.append(DeviceColumn(c.type_id, c.length, data, valid, None, c.null_count))
+        stats = {"submit_ns": st.submit_ns, "start_ns": st.start_ns, "end_ns": st.end_ns,
+                 "batch_queries": st.batch_queries, "plain": st.plain}
+        return res, stats
+
+    def release(self, qid: int) -> None:
+        check(_server_lib().ldf_server_release(self.h, qid))
+        self._keep.pop(qid, None)
+
+    def close(self) -> None:
+        if self.h:
+            _server_lib().ldf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
