@@ -365,6 +365,7 @@ def run_batched(args, torch, C):
     ninst = args.instances
     L, R, G, keep, info = make_batched_inputs(torch, C, ninst)
     torch.cuda.synchronize()
+    log(f"[batched] {ninst} instances ready, {info['probe_rows']} probe rows")
 
     def plain_step():
         for i in range(ninst):
@@ -378,6 +379,7 @@ def run_batched(args, torch, C):
     torch.cuda.synchronize()
     plain_ms = (time.perf_counter() - t0) * 1e3
     C.pool_release()
+    log(f"[batched] plain launches: {plain_ms:.0f} ms")
 
     t0 = time.perf_counter()
     bj = C.batch_join(L, R, [0], [0], 0, config={"use_graph": 1})
@@ -619,11 +621,20 @@ def main():
         if not args.no_secondary:
             sa = argparse.Namespace(**vars(args))
             sa.rows, sa.steps, sa.warmup = 0, max(3, args.steps // 2), 2
+            log("[bench] secondary: inner join 1M x 10M")
             res["secondary"] = {"join_1Mx10M": run_join(sa, torch, C)}
-            sa.rows = 200_000_000
-            res["secondary"]["partition_200M"] = run_partition(sa, torch, C)
+            sa.rows = 0
+            log("[bench] secondary: hash partition 1B -> 256")
+            res["secondary"]["partition_1B_256"] = run_partition(sa, torch, C)
+            C.pool_release()
             sa.rows, sa.steps, sa.warmup = 0, 3, 1
+            log("[bench] secondary: left join 50M x 500M")
             res["secondary"]["left_join_50Mx500M"] = run_join_c3(sa, torch, C, 1)
+            sa.steps = 3
+            log("[bench] secondary: batched 4096 instances")
+            res["secondary"]["batched_4096"] = run_batched(sa, torch, C)
+            log("[bench] secondary: served 200 queries")
+            res["secondary"]["served_200"] = run_served(sa, torch, C)
     elif args.workload == "join":
         res = run_join(args, torch, C)
     elif args.workload == "batched":
