@@ -203,6 +203,145 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t 
   }
 }
 
+// One fixed-width, non-null column: the same stable scatter with 512-thread blocks over
+// 4096-row steps, but each thread keeps the values it loaded for hashing in registers (and loads
+// the next step's values while this step is ranked and staged), and the values themselves are
+// staged in LDS, so the column is read once here and written once as contiguous runs.  The
+// smaller LDS plan lets three blocks share a CU, so one block's ranking overlaps another's
+// memory traffic.
+// LDS layout (dynamic): cursor u64[P] | wtab u16[W][P] | total u32[P] | start u32[P] |
+//                       stage_val T[kVSub] | stage_part u16[kVSub]
+constexpr int kVThreads = 512;
+constexpr int kVWaves   = kVThreads / kWave;
+constexpr int kVSub     = 4096;
+constexpr int kVPerWave = kVSub / kVWaves;  // 512 contiguous rows per wave
+constexpr int kVGroups  = kVPerWave / kWave;
+
+template <typename T>
+__global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restrict__ src, int64_t n, uint32_t seed,
+                                                            int64_t rows_per_block, uint32_t nparts, int pbits,
+                                                            const uint64_t* __restrict__ offsets, T* __restrict__ dst)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  uint64_t* cursor  = reinterpret_cast<uint64_t*>(smem_raw);
+  uint16_t* wtab    = reinterpret_cast<uint16_t*>(cursor + nparts);
+  uint32_t* total   = reinterpret_cast<uint32_t*>(wtab + kVWaves * nparts + ((kVWaves * nparts) & 1));
+  uint32_t* start   = total + nparts;
+  T* stage_v        = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(start + nparts) +
+                                    ((16 - (((uintptr_t)(start + nparts)) & 15)) & 15));
+  uint16_t* stage_p = reinterpret_cast<uint16_t*>(stage_v + kVSub);
+  __shared__ uint64_t scan_smem[17];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) cursor[p] = offsets[(int64_t)p * gridDim.x + blockIdx.x];
+
+  const int64_t begin = blockIdx.x * rows_per_block;
+  const int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
+
+  T next_val[kVGroups];
+#pragma unroll
+  for (int g = 0; g < kVGroups; ++g) {
+    const int64_t r = begin + wave * kVPerWave + g * kWave + lane;
+    next_val[g]     = r < end ? src[r] : T(0);
+  }
+  for (int64_t sub = begin; sub < end; sub += kVSub) {
+    const int sub_n = (int)((end - sub) < kVSub ? (end - sub) : kVSub);
+    T my_val[kVGroups];
+#pragma unroll
+    for (int g = 0; g < kVGroups; ++g) {
+      my_val[g]       = next_val[g];
+      const int64_t r = sub + kVSub + wave * kVPerWave + g * kWave + lane;
+      next_val[g]     = r < end ? src[r] : T(0);
+    }
+    for (uint32_t i = threadIdx.x; i < kVWaves * nparts; i += blockDim.x) wtab[i] = 0;
+    __syncthreads();
+
+    // 1. partition of each row; stable ranks within the wave's 512 rows
+    uint16_t my_part[kVGroups];
+    uint16_t my_rank[kVGroups];
+#pragma unroll
+    for (int g = 0; g < kVGroups; ++g) {
+      const int local   = wave * kVPerWave + g * kWave + lane;
+      const bool active = local < sub_n;
+      uint32_t p        = 0;
+      if (active)
+        p = hash_combine32(seed, murmur3_fixed((uint64_t)my_val[g], (int)sizeof(T), seed)) % nparts;
+      uint64_t same = __ballot(active);
+      for (int b = 0; b < pbits; ++b) {
+        uint64_t bb = __ballot(active && ((p >> b) & 1));
+        same &= ((p >> b) & 1) ? bb : ~bb;
+      }
+      uint32_t rank_in_group = (uint32_t)__popcll(same & lanes_below);
+      uint32_t cnt           = (uint32_t)__popcll(same);
+      int leader             = __ffsll((unsigned long long)same) - 1;
+      uint32_t old           = 0;
+      if (active && leader == lane) {
+        old                     = wtab[wave * nparts + p];
+        wtab[wave * nparts + p] = (uint16_t)(old + cnt);
+      }
+      old        = __shfl(old, leader < 0 ? 0 : leader, kWave);
+      my_part[g] = (uint16_t)p;
+      my_rank[g] = (uint16_t)(old + rank_in_group);
+    }
+    __syncthreads();
+
+    // 2. prefix over waves for every partition; per-partition totals of this step
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int w = 0; w < kVWaves; ++w) {
+        uint32_t c           = wtab[w * nparts + p];
+        wtab[w * nparts + p] = (uint16_t)run;
+        run += c;
+      }
+      total[p] = run;
+    }
+    __syncthreads();
+
+    // 3. exclusive scan of the totals -> start of each partition in the staging area
+    {
+      uint32_t per = (nparts + blockDim.x - 1) / blockDim.x;
+      uint32_t p0  = threadIdx.x * per;
+      uint64_t s   = 0;
+      for (uint32_t k = 0; k < per; ++k)
+        if (p0 + k < nparts) s += total[p0 + k];
+      uint64_t tot;
+      uint64_t run = block_exclusive_scan(s, scan_smem, &tot);
+      for (uint32_t k = 0; k < per; ++k)
+        if (p0 + k < nparts) {
+          start[p0 + k] = (uint32_t)run;
+          run += total[p0 + k];
+        }
+    }
+    __syncthreads();
+
+    // 4. stage values in partition order
+#pragma unroll
+    for (int g = 0; g < kVGroups; ++g) {
+      const int local = wave * kVPerWave + g * kWave + lane;
+      if (local < sub_n) {
+        const uint32_t p   = my_part[g];
+        const uint32_t pos = start[p] + wtab[wave * nparts + p] + my_rank[g];
+        stage_v[pos]       = my_val[g];
+        stage_p[pos]       = (uint16_t)p;
+      }
+    }
+    __syncthreads();
+
+    // 5. write contiguous runs
+    for (int s = threadIdx.x; s < sub_n; s += blockDim.x) {
+      const uint32_t p = stage_p[s];
+      dst[(int64_t)cursor[p] + (s - (int64_t)start[p])] = stage_v[s];
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) cursor[p] += total[p];
+    __syncthreads();
+  }
+}
+
 __global__ void k_hp_pick_offsets(const uint64_t* __restrict__ scan, uint32_t nparts, uint32_t nblk,
                                   uint64_t* __restrict__ out)
 {
@@ -227,6 +366,29 @@ static HashCols make_hash_cols(const ldf_table_view* t, const int32_t* cols, int
     hc.c[i] = to_dcol(t->cols[cols[i]]);
   }
   return hc;
+}
+
+static size_t scatter_v_lds_bytes(uint32_t nparts, int width)
+{
+  size_t b = nparts * 8 + (size_t)kVWaves * nparts * 2 + (((size_t)kVWaves * nparts) & 1) * 2 + nparts * 8;
+  b        = (b + 15) & ~size_t(15);
+  return ((b + (size_t)kVSub * (width + 2)) + 15) & ~size_t(15);
+}
+constexpr size_t kHpVLdsMax = 150 * 1024;
+
+template <typename T>
+static void launch_scatter_v(const ldf_column_view& c, int64_t n, uint32_t seed, int64_t rpb, int64_t nblk,
+                             uint32_t nparts, int pbits, const uint64_t* scan, void* dst, hipStream_t s)
+{
+  static bool attr = false;
+  if (!attr) {
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kHpVLdsMax));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_hp_scatter_v<T>, dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)), s,
+                     static_cast<const T*>(c.data), n, seed, rpb, nparts, pbits, scan, static_cast<T*>(dst));
+  LDF_HIP_TRY(hipGetLastError());
 }
 
 static size_t scatter_lds_bytes(uint32_t nparts)
@@ -327,9 +489,22 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       lds_attr = true;
     }
     ProfScope prof("hp_scatter", s);
-    hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, pbits,
-                       (const uint64_t*)scan.get(), sc, perm.get());
-    LDF_HIP_TRY(hipGetLastError());
+    // one fixed-width non-null column hashed on itself: the value-staged kernel
+    const bool single = direct && table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
+                        scatter_v_lds_bytes((uint32_t)nparts, sc.width[0]) <= kHpVLdsMax;
+    if (single) {
+      const uint64_t* sp = (const uint64_t*)scan.get();
+      switch (sc.width[0]) {
+        case 1: launch_scatter_v<uint8_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
+        case 2: launch_scatter_v<uint16_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
+        case 4: launch_scatter_v<uint32_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
+        default: launch_scatter_v<uint64_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, pbits,
+                         (const uint64_t*)scan.get(), sc, perm.get());
+      LDF_HIP_TRY(hipGetLastError());
+    }
 
     if (!direct) {
       Buf<int32_t> map(a, n);
