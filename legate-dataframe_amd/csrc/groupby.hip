@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 #include "ldf_agg.hpp"
@@ -108,6 +109,8 @@ struct GbPlan {
   int32_t rows_off;    // LDS: u32 rows [hot_cap]
   int32_t lds_bytes;
   int64_t rows_per_block;
+  int32_t dpart;  // COUNT_DISTINCT through pair partitions (partitioned path, one such accumulator)
+  int32_t dcol;   // its value column (index into col)
 };
 
 struct HotList {
@@ -381,6 +384,7 @@ __device__ __forceinline__ void acc_global(const GbPlan& p, const DKeys& keys, i
 }
 
 // LDS accumulators of hot slot h (global slot s for the global-only accumulators)
+template <bool DPART = false>
 __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, const DKeys& keys, int64_t r, int h,
                                         int64_t s, const RowVals v)
 {
@@ -424,6 +428,7 @@ __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, con
         else atomic_mul_u64(&a.g0[s], bits_as_int(vb, c.type, c.width));
         break;
       case ACC_DISTINCT:
+        if (DPART) break;  // the pair goes to a pair partition instead
         if (distinct_insert(a, c, keys, r, s, vb)) atomicAdd(&reinterpret_cast<uint32_t*>(lds + a.lds_off)[h], 1u);
         break;
       default: break;
@@ -615,6 +620,8 @@ struct PartBufs {
   const uint64_t* offsets;  // exclusive scan of counts[part][blk]
   int32_t key_width;
   int32_t nblk;
+  unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
+  const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
 };
 
 struct HotParts {  // per-block partials of the hot slots, index blk * H + h (FSUM: 3 words)
@@ -641,6 +648,16 @@ __device__ __forceinline__ uint32_t cold_sub(uint64_t key, int S)
 {
   return (uint32_t)(mix64(key + 0x2545f4914f6cdd1dull) % (uint64_t)S);
 }
+
+// COUNT_DISTINCT through pair partitions: the (hot index, value) pairs of hot rows are spread by
+// a hash of the pair over kHPairParts coarse partitions (then kHPairFine fine ones), so each
+// fine partition's pairs fit an LDS hash set.  Bits of one 64-bit hash: top 10 = coarse
+// partition, next 4 = fine partition, low bits = LDS set slot.
+constexpr int kHPairBits = 10;
+constexpr int kHPairParts = 1 << kHPairBits;
+constexpr int kHPairFine  = 16;
+constexpr int kPairSet    = 8192;  // LDS pair-set slots of the dedupe kernels (64 KB)
+__device__ __forceinline__ uint64_t pair_hash(unsigned long long pr) { return mix64(pr ^ 0x94d049bb133111ebull); }
 
 // Builds the LDS table of hot keys (and clears the hot accumulators when `accs`).
 __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
@@ -719,25 +736,38 @@ __device__ __forceinline__ void store_w(void* base, int width, int64_t i, uint64
   }
 }
 
-// counts[part * nblk + blk] of the rows whose key is not hot
-__global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts)
+// counts[part * nblk + blk] of the rows whose key is not hot; with dpart also
+// hcounts[hpart * nblk + blk] of the hot rows' distinct pairs
+__global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
+                                                         uint32_t* hcounts)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
   __shared__ uint32_t hist[kNumParts];
+  __shared__ uint32_t hhist[kHPairParts];
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) hist[i] = 0;
+  for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
   hot_setup(p, lds, hot, nhot, false, &s_null_hot, &s_sent_hot);
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
   const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
   const DCol& kc      = keys.c[0];
+  const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
   for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
     const bool kv = valid_at(kc.valid, r);
     uint64_t k    = kv ? load_bits(kc.data, kc.width, r) : 0;
-    if (hot_lookup(p, lds, k, kv, s_null_hot, s_sent_hot) < 0) atomicAdd(&hist[cold_part(k)], 1u);
+    const int h   = hot_lookup(p, lds, k, kv, s_null_hot, s_sent_hot);
+    if (h < 0) {
+      atomicAdd(&hist[cold_part(k)], 1u);
+    } else if (p.dpart && valid_at(dc.valid, r)) {
+      const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)load_bits(dc.data, dc.width, r);
+      atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) counts[(int64_t)i * gridDim.x + blockIdx.x] = hist[i];
+  if (p.dpart)
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * gridDim.x + blockIdx.x] = hhist[i];
 }
 
 // Main pass: hot rows into LDS accumulators, other rows written straight into their partition:
@@ -745,15 +775,20 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
 // accumulator is exact and order-independent).  LDS: [0, p.lds_bytes) hot table +
 // accumulators, then cursor u64[kNumParts].
+template <bool DPART>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + p.lds_bytes);
+  unsigned long long* cursor  = reinterpret_cast<unsigned long long*>(lds + p.lds_bytes);
+  unsigned long long* hcursor = cursor + kNumParts;  // dpart only
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
+  if (DPART)
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
+      hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + blockIdx.x];
   hot_setup(p, lds, hot, nhot, true, &s_null_hot, &s_sent_hot);
 
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
@@ -791,12 +826,16 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
       if (r >= end) continue;
       const int h = nhot > 0 ? hot_lookup(p, lds, kb[j], kv[j], s_null_hot, s_sent_hot) : -1;
       if (h >= 0) {
-        acc_lds(p, lds, keys, r, h, hot.gslot[h], v[j]);
+        acc_lds<DPART>(p, lds, keys, r, h, hot.gslot[h], v[j]);
+        if (DPART && ((v[j].valid >> p.dcol) & 1)) {
+          const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)pick(v[j], p.dcol);
+          pb.hpairs[atomicAdd(&hcursor[pair_hash(pr) >> (64 - kHPairBits)], 1ull)] = pr;
+        }
         continue;
       }
       uint32_t fl = (v[j].valid & 0xF) | (kv[j] ? kFlagKeyValid : 0);
       int di      = 0;
-      for (int i = 0; i < p.nacc; ++i) {
+      for (int i = 0; i < p.nacc && !DPART; ++i) {
         const AccDev& a = p.a[i];
         if (a.kind != ACC_DISTINCT) continue;
         if ((v[j].valid >> a.col) & 1) {
@@ -1147,6 +1186,199 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
 }
 
 // FP64 fallback (sums that left the fixed-point range) and the VARIANCE second pass.
+// ---- COUNT_DISTINCT through pair partitions -----------------------------------------------
+// slot of a single fixed-width key given by its bits, -1 when absent
+__device__ __forceinline__ int64_t gb_find_bits(unsigned long long key, const KeyTab& t)
+{
+  if (key == kGEmpty) return (int64_t)t.cap;
+  uint64_t i = mix64(key) & t.mask;
+  for (uint64_t probes = 0, lim = probe_limit(t); probes < lim; ++probes) {
+    const unsigned long long cur = t.slots[i];
+    if (cur == key) return (int64_t)i;
+    if (cur == kGEmpty) return -1;
+    i = (i + 1) & t.mask;
+  }
+  return -1;
+}
+
+// Inserts pr into an LDS set of kPairSet slots; 1 = new, 0 = present, -1 = no room.
+__device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned long long pr, uint64_t hsh)
+{
+  uint32_t i = (uint32_t)(hsh & (kPairSet - 1));
+  for (int t = 0; t < kPairSet; ++t) {
+    const unsigned long long old = atomicCAS(&set[i], kGEmpty, pr);
+    if (old == kGEmpty) return 1;
+    if (old == pr) return 0;
+    i = (i + 1) & (kPairSet - 1);
+  }
+  return -1;
+}
+
+// One block per coarse hot-pair partition: split its pairs into kHPairFine fine partitions
+// (within the coarse range of `fine`), then deduplicate each fine partition in an LDS hash set
+// (in several hash passes when it holds more pairs than the set takes at ~70% load), counting
+// new pairs per hot index in LDS; each hot key's count is added to its slot once per block.
+__global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long long* __restrict__ pairs,
+                                                          const uint64_t* __restrict__ hoffsets, int nblk,
+                                                          unsigned long long* __restrict__ fine, int hot_cap,
+                                                          HotList hot, unsigned long long* __restrict__ dist,
+                                                          int32_t* __restrict__ err)
+{
+  __shared__ unsigned long long set[kPairSet];
+  __shared__ uint32_t cnt[2048];
+  __shared__ uint32_t fh[kHPairFine];
+  __shared__ unsigned long long fcur[kHPairFine];
+  __shared__ int64_t fstart[kHPairFine + 1];
+  __shared__ int bad;
+  const int q      = blockIdx.x;
+  const int64_t b0 = (int64_t)hoffsets[(int64_t)q * nblk];
+  const int64_t b1 = (int64_t)hoffsets[(int64_t)(q + 1) * nblk];
+  const int nhot   = (int)min(*hot.count, (unsigned int)hot_cap);
+  for (int i = threadIdx.x; i < nhot; i += blockDim.x) cnt[i] = 0;
+  if (threadIdx.x < kHPairFine) fh[threadIdx.x] = 0;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x)
+    atomicAdd(&fh[(pair_hash(pairs[r]) >> (64 - kHPairBits - 4)) & (kHPairFine - 1)], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int f = 0; f < kHPairFine; ++f) {
+      fstart[f] = run;
+      fcur[f]   = (unsigned long long)(b0 + run);
+      run += fh[f];
+    }
+    fstart[kHPairFine] = run;
+  }
+  __syncthreads();
+  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x) {
+    const unsigned long long pr = pairs[r];
+    fine[atomicAdd(&fcur[(pair_hash(pr) >> (64 - kHPairBits - 4)) & (kHPairFine - 1)], 1ull)] = pr;
+  }
+  __syncthreads();
+  for (int f = 0; f < kHPairFine; ++f) {
+    const int64_t r0 = b0 + fstart[f], r1 = b0 + fstart[f + 1];
+    const int passes = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+    for (int pass = 0; pass < passes; ++pass) {
+      for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
+      __syncthreads();
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const unsigned long long pr = fine[r];
+        const uint64_t hsh          = pair_hash(pr);
+        if (passes > 1 && (int)((hsh >> 24) % (uint64_t)passes) != pass) continue;
+        const int k = pair_set_insert(set, pr, hsh);
+        if (k > 0) atomicAdd(&cnt[(uint32_t)(pr >> 32)], 1u);
+        else if (k < 0) bad = 1;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nhot; i += blockDim.x)
+    if (cnt[i]) atomicAdd(&dist[hot.gslot[i]], (unsigned long long)cnt[i]);
+  if (threadIdx.x == 0 && bad) atomicOr(err, 1);
+}
+
+// One block per cold partition (whose rows hold every row of their keys): the (key, value)
+// pairs of its rows with a valid value are split by pair hash into groups of at most ~70% of
+// the LDS set (through `fine`, within the partition's range), each group is deduplicated in the
+// LDS set, and each new pair counts for its key in an LDS key table that holds all of the
+// partition's keys; every key's count is added to its slot at the end.
+constexpr int kColdKeySlots = 2048;
+constexpr int kColdGroups   = 64;
+__global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol, int dwidth, KeyTab t,
+                                                           unsigned long long* __restrict__ fine,
+                                                           unsigned long long* __restrict__ dist,
+                                                           int32_t* __restrict__ err)
+{
+  __shared__ unsigned long long set[kPairSet];
+  __shared__ unsigned long long kkeys[kColdKeySlots];
+  __shared__ uint32_t kcnt[kColdKeySlots];
+  __shared__ uint32_t gh[kColdGroups];
+  __shared__ unsigned long long gcur[kColdGroups];
+  __shared__ int64_t gstart[kColdGroups + 1];
+  __shared__ unsigned long long npairs;
+  __shared__ int bad;
+  const int part   = blockIdx.x;
+  const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
+  const int64_t r1 = (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
+  for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
+    kkeys[i] = kGEmpty;
+    kcnt[i]  = 0;
+  }
+  if (threadIdx.x < kColdGroups) gh[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    npairs = 0;
+    bad    = 0;
+  }
+  __syncthreads();
+  {
+    uint32_t c = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) c += (pb.flags[r] >> dcol) & 1;
+    if (c) atomicAdd(&npairs, (unsigned long long)c);
+  }
+  __syncthreads();
+  const int ng = (int)(npairs / (unsigned long long)(0.7 * kPairSet)) + 1;
+  if (ng > kColdGroups) {  // more pairs than this kernel plans for: the caller falls back
+    if (threadIdx.x == 0) atomicOr(err, 16);
+    return;
+  }
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    if (!((pb.flags[r] >> dcol) & 1)) continue;
+    const unsigned long long pr = (load_bits(pb.key, pb.key_width, r) << 32) | (uint32_t)load_bits(pb.val[dcol], dwidth, r);
+    atomicAdd(&gh[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int g = 0; g < ng; ++g) {
+      gstart[g] = run;
+      gcur[g]   = (unsigned long long)(r0 + run);
+      run += gh[g];
+    }
+    gstart[ng] = run;
+  }
+  __syncthreads();
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    if (!((pb.flags[r] >> dcol) & 1)) continue;
+    const unsigned long long pr = (load_bits(pb.key, pb.key_width, r) << 32) | (uint32_t)load_bits(pb.val[dcol], dwidth, r);
+    fine[atomicAdd(&gcur[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1ull)] = pr;
+  }
+  __syncthreads();
+  for (int g = 0; g < ng; ++g) {
+    for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
+    __syncthreads();
+    for (int64_t r = r0 + gstart[g] + threadIdx.x; r < r0 + gstart[g + 1]; r += blockDim.x) {
+      const unsigned long long pr = fine[r];
+      const int k                 = pair_set_insert(set, pr, pair_hash(pr));
+      if (k < 0) bad |= 2;
+      if (k <= 0) continue;
+      const unsigned long long kb = pr >> 32;
+      uint32_t i                  = (uint32_t)(mix64(kb) & (kColdKeySlots - 1));
+      int found                   = -1;
+      for (int tries = 0; tries < kColdKeySlots; ++tries) {
+        unsigned long long cur = kkeys[i];
+        if (cur == kGEmpty) cur = atomicCAS(&kkeys[i], kGEmpty, kb);
+        if (cur == kGEmpty || cur == kb) {
+          found = (int)i;
+          break;
+        }
+        i = (i + 1) & (kColdKeySlots - 1);
+      }
+      if (found < 0) bad |= 4;
+      else atomicAdd(&kcnt[found], 1u);
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
+    if (kcnt[i] == 0) continue;
+    const int64_t sl = gb_find_bits(kkeys[i], t);
+    if (sl >= 0) atomicAdd(&dist[sl], (unsigned long long)kcnt[i]);
+    else bad |= 8;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && bad) atomicOr(err, bad);
+}
+
 template <bool GENERIC>
 __global__ __launch_bounds__(256) void k_gb_second_pass(DKeys keys, int64_t n, KeyTab t, ColDev c, double* acc,
                                                         const double* mean)
@@ -1718,6 +1950,17 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
   if (ndistinct > 0 && (keys.c[0].width > 4 || keys.c[0].valid != nullptr || ndistinct > 3)) use_v3 = false;
   for (int i = 0; i < naggs; ++i)
     if (aggs[i].kind == LDF_AGG_VARIANCE || aggs[i].kind == LDF_AGG_STDDEV) use_v3 = false;
+  // COUNT_DISTINCT through pair partitions deduplicated in LDS (one such accumulator on the
+  // partitioned path); a dedupe table that ever overflows sends the call back to the pair-set path
+  bool dpart = use_v3 && ndistinct == 1;
+  {
+    const char* env = std::getenv("LDF_GROUPBY_DPART");
+    if (env != nullptr && env[0] == '0') dpart = false;
+  }
+  int dacc = -1;
+  for (int i = 0; i < nacc; ++i)
+    if (low.acc[i].kind == ACC_DISTINCT) dacc = i;
+  const int32_t dist_lds_off = dacc >= 0 ? plan.a[dacc].lds_off : -1;
 
   for (;;) {
     LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "groupby: too many groups");
@@ -1794,8 +2037,18 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     Buf<uint64_t> poffs;
     Buf<unsigned char> precs;
     Buf<unsigned long long> hparts;
+    Buf<uint32_t> hcounts;
+    Buf<uint64_t> hoffs;
+    Buf<unsigned long long> hpairs, hfine;
+    plan.dpart = 0;
+    plan.dcol  = 0;
+    if (dacc >= 0) plan.a[dacc].lds_off = (use_v3 && dpart) ? -1 : dist_lds_off;
     if (n > 0 && use_v3) {
-      const int stage_bytes = kNumParts * 8;
+      if (dpart) {
+        plan.dpart = 1;
+        plan.dcol  = plan.a[dacc].col;
+      }
+      const int stage_bytes = kNumParts * 8 + (dpart ? kHPairParts * 8 : 0);
       const int main_lds    = plan.lds_bytes + stage_bytes;
       int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
       int nblk              = 256 * per_cu;
@@ -1810,20 +2063,33 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         static bool attr = false;
         if (!attr) {
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           attr = true;
         }
+        if (dpart) {
+          hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
+          hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
+        }
         ProfScope prof("gb_hist", s);
-        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk), dim3(kGbThreads), plan.rows_off, s, keys, n, plan, hot, pcounts.get());
+        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk), dim3(kGbThreads), plan.rows_off, s, keys, n, plan, hot, pcounts.get(),
+                           hcounts.get());
         LDF_HIP_TRY(hipGetLastError());
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)kNumParts * nblk, a);
-      const int64_t ncold = (int64_t)read_u64(poffs.get() + (size_t)kNumParts * nblk, s);
+      if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nblk, a);
+      uint64_t totals[2] = {0, 0};
+      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)kNumParts * nblk, 8, hipMemcpyDeviceToHost, s));
+      if (dpart)
+        LDF_HIP_TRY(hipMemcpyAsync(&totals[1], hoffs.get() + (size_t)kHPairParts * nblk, 8, hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      const int64_t ncold = (int64_t)totals[0];
+      const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart)
       // 2. sets of (key, value) pairs for COUNT_DISTINCT: hot keys by global slot, others by key
       uint64_t hcap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)(n - ncold))));
       uint64_t ccap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)ncold)));
-      if (ndistinct > 0) {
+      if (ndistinct > 0 && !dpart) {
         psets = Buf<unsigned long long>(a, (size_t)ndistinct * (hcap + ccap));
         fill_u64((uint64_t*)psets.get(), kGEmpty, (int64_t)ndistinct * (hcap + ccap), s);
         int di = 0;
@@ -1854,6 +2120,12 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       pb.key = carve((size_t)ncold * pb.key_width);
       for (int c = 0; c < plan.ncol; ++c) pb.val[c] = carve((size_t)ncold * plan.col[c].width);
       pb.flags = carve((size_t)ncold);
+      if (dpart) {
+        hpairs      = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, nhp));
+        hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
+        pb.hpairs   = hpairs.get();
+        pb.hoffsets = hoffs.get();
+      }
       // 4. per-block partials of the hot slots
       HotParts hp;
       std::memset(&hp, 0, sizeof(hp));
@@ -1872,7 +2144,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       // 5. the pass over the rows
       {
         ProfScope prof("gb_aggregate", s);
-        hipLaunchKernelGGL(k_gb_main3, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
+        if (dpart)
+          hipLaunchKernelGGL(k_gb_main3<true>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
+        else
+          hipLaunchKernelGGL(k_gb_main3<false>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
         LDF_HIP_TRY(hipGetLastError());
       }
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
@@ -1925,6 +2200,22 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         hipLaunchKernelGGL(k_gb_part3, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t);
         LDF_HIP_TRY(hipGetLastError());
       }
+      // 7. COUNT_DISTINCT: pairs of the other keys by (partition, sub-pass), pairs of the hot keys
+      //    by pair partition; both after the partition pass has given every key its slot
+      if (dpart) {
+        ProfScope prof("gb_distinct", s);
+        const AccDev& d = plan.a[dacc];
+        if (ncold > 0) {
+          hipLaunchKernelGGL(k_gb_cold_distinct, dim3(kNumParts), dim3(1024), 0, s, pb, plan.dcol,
+                             plan.col[plan.dcol].width, t, hfine.get(), d.g0, d.err);
+          LDF_HIP_TRY(hipGetLastError());
+        }
+        if (nhp > 0) {
+          hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, s, (const unsigned long long*)hpairs.get(),
+                             (const uint64_t*)hoffs.get(), nblk, hfine.get(), hot_cap, hot, d.g0, d.err);
+          LDF_HIP_TRY(hipGetLastError());
+        }
+      }
     } else if (n > 0) {
       // one wave of resident blocks (256 CUs; two 1024-thread blocks per CU when LDS allows)
       int per_cu = plan.lds_bytes > 0 ? std::max(1, std::min(2, (160 * 1024) / plan.lds_bytes)) : 2;
@@ -1947,6 +2238,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     std::vector<unsigned long long> hs(nstatus);
     LDF_HIP_TRY(hipMemcpyAsync(hs.data(), status.get(), nstatus * 8, hipMemcpyDeviceToHost, s));
     LDF_HIP_TRY(hipStreamSynchronize(s));
+    if (use_v3 && dpart && dacc >= 0 && (int32_t)hs[2 + dacc]) {  // a dedupe table overflowed
+      dpart = false;
+      continue;
+    }
     if ((int32_t)hs[1]) {
       LDF_EXPECTS(cap < maxcap, "groupby: key table overflow");
       cap = std::min(maxcap, cap * 4);
