@@ -120,7 +120,8 @@ typedef struct ldf_table_out {
 
 /* Allocation callbacks (the role TaskMemoryResource plays in the reference,
  * cpp/src/core/allocator.cu:27-81).  `stream` is the call's HIP stream; allocations are
- * stream ordered.  A NULL ldf_allocator* selects the library default: a caching device
+ * stream ordered and must be aligned to 256 bytes (as hipMalloc's are; a misaligned pointer is
+ * freed again and the call fails with LDF_E_INVALID).  A NULL ldf_allocator* selects the library default: a caching device
  * allocator that keeps freed blocks (in size classes) per device and stream and reuses them in
  * stream order; ldf_pool_release returns its cached blocks to the device. */
 typedef void* (*ldf_alloc_fn)(void* ctx, size_t bytes, void* stream);

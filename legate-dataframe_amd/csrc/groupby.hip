@@ -613,16 +613,74 @@ constexpr int kFlagKeyValid = 1 << 4;
 constexpr int kFlagDistinct = 5;  // bits 5..7: the row's (key, value) pair was new, per DISTINCT
 constexpr int kPartProbes   = 64;
 
+// A partition row is one record of `stride` bytes (8, 16, 32 or 64): the key and value bits,
+// each field at a multiple of its width (widest first), then the flags byte (bits 0-3 value
+// valid, bit 4 key valid, bits 5-7 new distinct pair).  One aligned record per row keeps the
+// scattered stores of the main pass to whole 16-byte (or 8-byte) pieces of one 32-byte segment.
 struct PartBufs {
-  void* key;
-  void* val[kMaxCols];
-  uint8_t* flags;           // bits 0-3 value valid, bit 4 key valid, bits 5-7 new distinct pair
+  unsigned char* rec;
   const uint64_t* offsets;  // exclusive scan of counts[part][blk]
+  int32_t stride;
+  int32_t key_off;
+  int32_t val_off[kMaxCols];
+  int32_t flags_off;
   int32_t key_width;
   int32_t nblk;
   unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
   const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
 };
+
+// Record words in registers.  `off` and `width` are the same for every lane; word k is built and
+// read with constant k and masks (no dynamically indexed private array, which would live in
+// scratch memory).
+__device__ __forceinline__ uint64_t rec_word(int k, int off, int width, uint64_t bits)
+{
+  const uint64_t v = width >= 8 ? bits : bits & ((1ull << (8 * width)) - 1);
+  return (off >> 3) == k ? v << ((off & 7) * 8) : 0ull;
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t rec_get(const uint64_t (&w)[W], int off, int width)
+{
+  const int wi = off >> 3;
+  uint64_t x   = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) x |= w[k] & (0ull - (uint64_t)(k == wi));
+  x >>= (off & 7) * 8;
+  return width >= 8 ? x : x & ((1ull << (8 * width)) - 1);
+}
+
+__device__ __forceinline__ void rec_store(unsigned char* base, int stride, int64_t i, const uint64_t (&w)[8])
+{
+  unsigned char* d = base + i * (int64_t)stride;
+  if (stride == 8) {
+    *reinterpret_cast<uint64_t*>(d) = w[0];
+    return;
+  }
+  uint4* q = reinterpret_cast<uint4*>(d);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (16 * k < stride)
+      q[k] = make_uint4((uint32_t)w[2 * k], (uint32_t)(w[2 * k] >> 32), (uint32_t)w[2 * k + 1],
+                        (uint32_t)(w[2 * k + 1] >> 32));
+}
+
+template <int W>
+__device__ __forceinline__ void rec_load(const unsigned char* base, int64_t i, uint64_t (&w)[W])
+{
+  const unsigned char* d = base + i * (int64_t)(8 * W);
+  if (W == 1) {
+    w[0] = *reinterpret_cast<const uint64_t*>(d);
+    return;
+  }
+  const uint4* q = reinterpret_cast<const uint4*>(d);
+#pragma unroll
+  for (int k = 0; k < W / 2; ++k) {
+    const uint4 x = q[k];
+    w[2 * k]      = (uint64_t)x.x | ((uint64_t)x.y << 32);
+    w[2 * k + 1]  = (uint64_t)x.z | ((uint64_t)x.w << 32);
+  }
+}
 
 struct HotParts {  // per-block partials of the hot slots, index blk * H + h (FSUM: 3 words)
   uint32_t* rows;
@@ -726,16 +784,6 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
   }
 }
 
-__device__ __forceinline__ void store_w(void* base, int width, int64_t i, uint64_t bits)
-{
-  switch (width) {
-    case 1: static_cast<uint8_t*>(base)[i] = (uint8_t)bits; break;
-    case 2: static_cast<uint16_t*>(base)[i] = (uint16_t)bits; break;
-    case 4: static_cast<uint32_t*>(base)[i] = (uint32_t)bits; break;
-    default: static_cast<uint64_t*>(base)[i] = bits; break;
-  }
-}
-
 // counts[part * nblk + blk] of the rows whose key is not hot; with dpart also
 // hcounts[hpart * nblk + blk] of the hot rows' distinct pairs
 __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
@@ -753,15 +801,28 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
   const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
   const DCol& kc      = keys.c[0];
   const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
-  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
-    const bool kv = valid_at(kc.valid, r);
-    uint64_t k    = kv ? load_bits(kc.data, kc.width, r) : 0;
-    const int h   = hot_lookup(p, lds, k, kv, s_null_hot, s_sent_hot);
-    if (h < 0) {
-      atomicAdd(&hist[cold_part(k)], 1u);
-    } else if (p.dpart && valid_at(dc.valid, r)) {
-      const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)load_bits(dc.data, dc.width, r);
-      atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
+  constexpr int R = 8;  // rows per thread whose loads are in flight together
+  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
+    uint64_t k[R], d[R];
+    bool kv[R], dv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int64_t r = r0 + (int64_t)j * blockDim.x;
+      kv[j]           = r < end && valid_at(kc.valid, r);
+      k[j]            = kv[j] ? load_bits(kc.data, kc.width, r) : 0;
+      dv[j]           = p.dpart && r < end && valid_at(dc.valid, r);
+      d[j]            = dv[j] ? load_bits(dc.data, dc.width, r) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (r0 + (int64_t)j * blockDim.x >= end) break;
+      const int h = hot_lookup(p, lds, k[j], kv[j], s_null_hot, s_sent_hot);
+      if (h < 0) {
+        atomicAdd(&hist[cold_part(k[j])], 1u);
+      } else if (dv[j]) {
+        const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)d[j];
+        atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
+      }
     }
   }
   __syncthreads();
@@ -855,12 +916,17 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         ++di;
       }
       const int64_t dest = (int64_t)atomicAdd(&cursor[cold_part(kb[j])], 1ull);
-      store_w(pb.key, pb.key_width, dest, kb[j]);
-      if (p.ncol > 0) store_w(pb.val[0], p.col[0].width, dest, v[j].b0);
-      if (p.ncol > 1) store_w(pb.val[1], p.col[1].width, dest, v[j].b1);
-      if (p.ncol > 2) store_w(pb.val[2], p.col[2].width, dest, v[j].b2);
-      if (p.ncol > 3) store_w(pb.val[3], p.col[3].width, dest, v[j].b3);
-      pb.flags[dest] = (uint8_t)fl;
+      uint64_t w[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        uint64_t x = rec_word(k, pb.key_off, pb.key_width, kb[j]) | rec_word(k, pb.flags_off, 1, fl);
+        if (p.ncol > 0) x |= rec_word(k, pb.val_off[0], p.col[0].width, v[j].b0);
+        if (p.ncol > 1) x |= rec_word(k, pb.val_off[1], p.col[1].width, v[j].b1);
+        if (p.ncol > 2) x |= rec_word(k, pb.val_off[2], p.col[2].width, v[j].b2);
+        if (p.ncol > 3) x |= rec_word(k, pb.val_off[3], p.col[3].width, v[j].b3);
+        w[k] = x;
+      }
+      rec_store(pb.rec, pb.stride, dest, w);
     }
   }
   __syncthreads();
@@ -1061,6 +1127,7 @@ __device__ __forceinline__ void acc_spill(const GbPlan& p, int64_t s, const RowV
 
 // One block per (partition, sub-pass); the S blocks of a partition share an XCD (block % 8) so
 // the partition's rows are re-read from that XCD's L2.
+template <int W>
 __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, PartBufs pb, KeyTab t)
 {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -1103,12 +1170,14 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
       fr[j]           = 0;
       vr[j]           = RowVals{0, 0, 0, 0, 0};
       if (i < end) {
-        kr[j]                = load_bits(pb.key, pb.key_width, i);
-        fr[j]                = pb.flags[i];
+        uint64_t w[W];
+        rec_load<W>(pb.rec, i, w);
+        kr[j]                = rec_get<W>(w, pb.key_off, pb.key_width);
+        fr[j]                = (uint32_t)rec_get<W>(w, pb.flags_off, 1);
         uint64_t x[kMaxCols] = {0, 0, 0, 0};
 #pragma unroll
         for (int c = 0; c < kMaxCols; ++c)
-          if (c < p.ncol) x[c] = load_bits(pb.val[c], p.col[c].width, i);
+          if (c < p.ncol) x[c] = rec_get<W>(w, pb.val_off[c], p.col[c].width);
         vr[j] = RowVals{x[0], x[1], x[2], x[3], fr[j] & 0xF};
       }
     }
@@ -1285,6 +1354,16 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
 // partition's keys; every key's count is added to its slot at the end.
 constexpr int kColdKeySlots = 2048;
 constexpr int kColdGroups   = 64;
+
+// the (key, value) pair of partition row r when its value is valid
+__device__ __forceinline__ bool cold_pair(const PartBufs& pb, int dcol, int dwidth, int64_t r, unsigned long long& pr)
+{
+  const unsigned char* d = pb.rec + r * (int64_t)pb.stride;
+  if (!((d[pb.flags_off] >> dcol) & 1)) return false;
+  pr = (load_bits(d + pb.key_off, pb.key_width, 0) << 32) | (uint32_t)load_bits(d + pb.val_off[dcol], dwidth, 0);
+  return true;
+}
+
 __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol, int dwidth, KeyTab t,
                                                            unsigned long long* __restrict__ fine,
                                                            unsigned long long* __restrict__ dist,
@@ -1296,36 +1375,26 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   __shared__ uint32_t gh[kColdGroups];
   __shared__ unsigned long long gcur[kColdGroups];
   __shared__ int64_t gstart[kColdGroups + 1];
-  __shared__ unsigned long long npairs;
   __shared__ int bad;
   const int part   = blockIdx.x;
   const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
   const int64_t r1 = (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
+  // groups of at most 0.7 * kPairSet pairs when every row has a pair
+  const int ng = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+  if (ng > kColdGroups) {  // more rows than this kernel plans for: the caller falls back
+    if (threadIdx.x == 0) atomicOr(err, 16);
+    return;
+  }
   for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
     kkeys[i] = kGEmpty;
     kcnt[i]  = 0;
   }
   if (threadIdx.x < kColdGroups) gh[threadIdx.x] = 0;
-  if (threadIdx.x == 0) {
-    npairs = 0;
-    bad    = 0;
-  }
+  if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  {
-    uint32_t c = 0;
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) c += (pb.flags[r] >> dcol) & 1;
-    if (c) atomicAdd(&npairs, (unsigned long long)c);
-  }
-  __syncthreads();
-  const int ng = (int)(npairs / (unsigned long long)(0.7 * kPairSet)) + 1;
-  if (ng > kColdGroups) {  // more pairs than this kernel plans for: the caller falls back
-    if (threadIdx.x == 0) atomicOr(err, 16);
-    return;
-  }
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    if (!((pb.flags[r] >> dcol) & 1)) continue;
-    const unsigned long long pr = (load_bits(pb.key, pb.key_width, r) << 32) | (uint32_t)load_bits(pb.val[dcol], dwidth, r);
-    atomicAdd(&gh[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1u);
+    unsigned long long pr;
+    if (cold_pair(pb, dcol, dwidth, r, pr)) atomicAdd(&gh[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1339,9 +1408,9 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   }
   __syncthreads();
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    if (!((pb.flags[r] >> dcol) & 1)) continue;
-    const unsigned long long pr = (load_bits(pb.key, pb.key_width, r) << 32) | (uint32_t)load_bits(pb.val[dcol], dwidth, r);
-    fine[atomicAdd(&gcur[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1ull)] = pr;
+    unsigned long long pr;
+    if (cold_pair(pb, dcol, dwidth, r, pr))
+      fine[atomicAdd(&gcur[(uint32_t)((pair_hash(pr) >> 24) % (uint64_t)ng)], 1ull)] = pr;
   }
   __syncthreads();
   for (int g = 0; g < ng; ++g) {
@@ -2065,7 +2134,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           attr = true;
         }
         if (dpart) {
@@ -2108,18 +2180,26 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       pb.key_width = keys.c[0].width;
       pb.nblk      = nblk;
       pb.offsets   = poffs.get();
-      size_t rec   = (size_t)pb.key_width + 1;
-      for (int c = 0; c < plan.ncol; ++c) rec += (size_t)plan.col[c].width;
-      precs             = Buf<unsigned char>(a, (size_t)std::max<int64_t>(1, ncold) * rec + 64 * (kMaxCols + 2));
-      unsigned char* rp = precs.get();
-      auto carve        = [&](size_t bytes) {
-        unsigned char* r0 = rp;
-        rp += (bytes + 63) & ~size_t(63);
-        return r0;
-      };
-      pb.key = carve((size_t)ncold * pb.key_width);
-      for (int c = 0; c < plan.ncol; ++c) pb.val[c] = carve((size_t)ncold * plan.col[c].width);
-      pb.flags = carve((size_t)ncold);
+      {  // record layout: fields widest first (each at a multiple of its width), flags byte last
+        int32_t off = 0;
+        for (int wdt = 8; wdt >= 1; wdt >>= 1) {
+          if (pb.key_width == wdt) {
+            pb.key_off = off;
+            off += wdt;
+          }
+          for (int c = 0; c < plan.ncol; ++c)
+            if (plan.col[c].width == wdt) {
+              pb.val_off[c] = off;
+              off += wdt;
+            }
+        }
+        pb.flags_off = off++;
+        pb.stride    = 8;
+        while (pb.stride < off) pb.stride *= 2;
+        LDF_EXPECTS(pb.stride <= 64, "groupby: partition record over 64 bytes");
+      }
+      precs  = Buf<unsigned char>(a, (size_t)std::max<int64_t>(1, ncold) * (size_t)pb.stride);
+      pb.rec = precs.get();
       if (dpart) {
         hpairs      = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, nhp));
         hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
@@ -2197,7 +2277,12 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       if (ncold > 0) {
         ProfScope prof("gb_partitions", s);
         const int grid = kNumParts * q.S;
-        hipLaunchKernelGGL(k_gb_part3, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t);
+        switch (pb.stride) {
+          case 8: hipLaunchKernelGGL(k_gb_part3<1>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
+          case 16: hipLaunchKernelGGL(k_gb_part3<2>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
+          case 32: hipLaunchKernelGGL(k_gb_part3<4>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
+          default: hipLaunchKernelGGL(k_gb_part3<8>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
+        }
         LDF_HIP_TRY(hipGetLastError());
       }
       // 7. COUNT_DISTINCT: pairs of the other keys by (partition, sub-pass), pairs of the hot keys

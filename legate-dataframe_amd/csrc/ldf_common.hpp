@@ -104,6 +104,10 @@ class Alloc {
     if (a_ != nullptr && a_->alloc != nullptr) {
       p = a_->alloc(a_->ctx, bytes, (void*)s_);
       if (p == nullptr) throw Error(LDF_E_OOM, "allocator returned NULL for " + std::to_string(bytes) + " bytes");
+      if ((reinterpret_cast<uintptr_t>(p) & 255) != 0) {  // the kernels store 16-byte vectors
+        if (a_->free != nullptr) a_->free(a_->ctx, p, bytes, (void*)s_);
+        throw Error(LDF_E_INVALID, "allocator returned a pointer not aligned to 256 bytes");
+      }
     } else {
       p = pool_allocate(bytes, s_);
     }
