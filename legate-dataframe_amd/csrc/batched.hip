@@ -636,10 +636,7 @@ __global__ __launch_bounds__(kGT) void k_bg_aggregate(const GInst* __restrict__ 
                         (unsigned long long)bits_as_int(vb, ct, cw));
               break;
             case B_FSUM: {
-              uint64_t l0, l1, l2;
-              if (to_fixed192(bits_as_double(vb, ct, cw), l0, l1, l2))
-                lds_fsum_add(reinterpret_cast<unsigned long long*>(lds + a.off) + 6 * h, l0, l1, l2);
-              else
+              if (!lds_fsum_add(reinterpret_cast<unsigned long long*>(lds + a.off) + 6 * h, bits_as_double(vb, ct, cw)))
                 p.flags[0] = 1;
               break;
             }
@@ -1418,6 +1415,8 @@ GroupbyBatch* create_groupby(int32_t ninst, const ldf_table_view* tables, const 
     const ldf_table_view& T = tables[i];
     LDF_EXPECTS(T.ncols == T0.ncols, "batch groupby: instances differ in their columns");
     n[i] = T.ncols > 0 ? T.cols[0].length : 0;
+    // the exact sums' LDS words take fewer than 2^31 addends per table
+    LDF_EXPECTS(n[i] < INT32_MAX, "batch groupby: each instance is limited to 2^31-1 rows");
     for (int c = 0; c < T.ncols; ++c) {
       LDF_EXPECTS(T.cols[c].type_id == T0.cols[c].type_id, "batch groupby: instances differ in their column types");
       validate_view(T.cols[c], n[i], "batch groupby");

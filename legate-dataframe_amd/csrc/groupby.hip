@@ -384,7 +384,6 @@ __device__ __forceinline__ void acc_global(const GbPlan& p, const DKeys& keys, i
 }
 
 // LDS accumulators of hot slot h (global slot s for the global-only accumulators)
-template <bool DPART = false>
 __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, const DKeys& keys, int64_t r, int h,
                                         int64_t s, const RowVals v)
 {
@@ -405,10 +404,8 @@ __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, con
                   (unsigned long long)bits_as_int(vb, c.type, c.width));
         break;
       case ACC_FSUM: {
-        uint64_t l0, l1, l2;
         unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
-        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) lds_fsum_add(&base[6 * h], l0, l1, l2);
-        else *a.err = 1;
+        if (!lds_fsum_add(&base[6 * h], bits_as_double(vb, c.type, c.width))) *a.err = 1;
         break;
       }
       case ACC_MIN:
@@ -428,7 +425,6 @@ __device__ __forceinline__ void acc_lds(const GbPlan& p, unsigned char* lds, con
         else atomic_mul_u64(&a.g0[s], bits_as_int(vb, c.type, c.width));
         break;
       case ACC_DISTINCT:
-        if (DPART) break;  // the pair goes to a pair partition instead
         if (distinct_insert(a, c, keys, r, s, vb)) atomicAdd(&reinterpret_cast<uint32_t*>(lds + a.lds_off)[h], 1u);
         break;
       default: break;
@@ -831,6 +827,96 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
     for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * gridDim.x + blockIdx.x] = hhist[i];
 }
 
+// LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
+// hot): the accumulator loop is outside and the row loop inside, so each accumulator's kind,
+// column and type are resolved once per R rows.
+template <bool DPART, int R>
+__device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds, const DKeys& keys, int64_t r0,
+                                             const int (&h)[R], const HotList& hot, const RowVals (&v)[R])
+{
+  uint32_t* rows = reinterpret_cast<uint32_t*>(lds + p.rows_off);
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (h[j] >= 0) atomicAdd(&rows[h[j]], 1u);
+  for (int i = 0; i < p.nacc; ++i) {
+    const AccDev& a = p.a[i];
+    if (a.kind == ACC_ROWS) continue;
+    const int col = a.col;
+    const int ct  = p.col[col].type;
+    const int cw  = p.col[col].width;
+    bool ok[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ok[j] = h[j] >= 0 && ((v[j].valid >> col) & 1);
+    switch (a.kind) {
+      case ACC_VCOUNT: {
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + a.lds_off);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j]) atomicAdd(&cnt[h[j]], 1u);
+        break;
+      }
+      case ACC_ISUM: {
+        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j]) atomicAdd(&acc[h[j]], (unsigned long long)bits_as_int(pick(v[j], col), ct, cw));
+        break;
+      }
+      case ACC_FSUM: {
+        unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
+        bool bad                 = false;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j] && !lds_fsum_add(&base[6 * h[j]], bits_as_double(pick(v[j], col), ct, cw))) bad = true;
+        if (bad) *a.err = 1;
+        break;
+      }
+      case ACC_MIN:
+      case ACC_MAX: {
+        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
+        uint32_t* flags         = reinterpret_cast<uint32_t*>(lds + p.flags_off);
+        const bool fl           = is_float_type(ct);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!ok[j]) continue;
+          const uint64_t vb = pick(v[j], col);
+          if (fl && isnan(bits_as_double(vb, ct, cw))) {
+            atomicOr(&flags[h[j]], 1u << a.flag_bit);
+            continue;
+          }
+          const uint64_t o = ordered_of(vb, ct, cw);
+          if (a.kind == ACC_MIN) atomicMin(&acc[h[j]], (unsigned long long)o);
+          else atomicMax(&acc[h[j]], (unsigned long long)o);
+        }
+        break;
+      }
+      case ACC_PROD: {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!ok[j]) continue;
+          const uint64_t vb = pick(v[j], col);
+          const int64_t s   = hot.gslot[h[j]];
+          if (is_float_type(ct)) atomic_mul_f64(&a.g0[s], bits_as_double(vb, ct, cw));
+          else atomic_mul_u64(&a.g0[s], bits_as_int(vb, ct, cw));
+        }
+        break;
+      }
+      case ACC_DISTINCT: {
+        if (DPART) break;  // the pair goes to a pair partition instead
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + a.lds_off);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!ok[j]) continue;
+          const int64_t r = r0 + (int64_t)j * blockDim.x + threadIdx.x;
+          if (distinct_insert(a, p.col[col], keys, r, hot.gslot[h[j]], pick(v[j], col))) atomicAdd(&cnt[h[j]], 1u);
+        }
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
 // Main pass: hot rows into LDS accumulators, other rows written straight into their partition:
 // each takes the next position of its partition's region with an LDS atomic on the region
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
@@ -881,19 +967,23 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
       }
     }
+    int hx[kGbRowsPerThread];  // hot index; -1 not hot; -2 past the end
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
       const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
-      if (r >= end) continue;
-      const int h = nhot > 0 ? hot_lookup(p, lds, kb[j], kv[j], s_null_hot, s_sent_hot) : -1;
-      if (h >= 0) {
-        acc_lds<DPART>(p, lds, keys, r, h, hot.gslot[h], v[j]);
-        if (DPART && ((v[j].valid >> p.dcol) & 1)) {
-          const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)pick(v[j], p.dcol);
-          pb.hpairs[atomicAdd(&hcursor[pair_hash(pr) >> (64 - kHPairBits)], 1ull)] = pr;
-        }
-        continue;
+      hx[j]           = r >= end ? -2 : (nhot > 0 ? hot_lookup(p, lds, kb[j], kv[j], s_null_hot, s_sent_hot) : -1);
+    }
+    acc_hot_rows<DPART, kGbRowsPerThread>(p, lds, keys, sub, hx, hot, v);
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      if (DPART && hx[j] >= 0 && ((v[j].valid >> p.dcol) & 1)) {
+        const unsigned long long pr = ((unsigned long long)(uint32_t)hx[j] << 32) | (uint32_t)pick(v[j], p.dcol);
+        pb.hpairs[atomicAdd(&hcursor[pair_hash(pr) >> (64 - kHPairBits)], 1ull)] = pr;
       }
+    }
+#pragma unroll
+    for (int j = 0; j < kGbRowsPerThread; ++j) {
+      if (hx[j] != -1) continue;
       uint32_t fl = (v[j].valid & 0xF) | (kv[j] ? kFlagKeyValid : 0);
       int di      = 0;
       for (int i = 0; i < p.nacc && !DPART; ++i) {
@@ -919,6 +1009,10 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
       uint64_t w[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
+        if (8 * k >= pb.stride) {
+          w[k] = 0;
+          continue;
+        }
         uint64_t x = rec_word(k, pb.key_off, pb.key_width, kb[j]) | rec_word(k, pb.flags_off, 1, fl);
         if (p.ncol > 0) x |= rec_word(k, pb.val_off[0], p.col[0].width, v[j].b0);
         if (p.ncol > 1) x |= rec_word(k, pb.val_off[1], p.col[1].width, v[j].b1);
@@ -1059,10 +1153,8 @@ __device__ __forceinline__ void acc_part(const GbPlan& p, const PartPlan& q, uns
                   (unsigned long long)bits_as_int(vb, c.type, c.width));
         break;
       case ACC_FSUM: {
-        uint64_t l0, l1, l2;
         unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
-        if (to_fixed192(bits_as_double(vb, c.type, c.width), l0, l1, l2)) lds_fsum_add(&base[6 * slot], l0, l1, l2);
-        else *a.err = 1;
+        if (!lds_fsum_add(&base[6 * slot], bits_as_double(vb, c.type, c.width))) *a.err = 1;
         break;
       }
       case ACC_MIN:
@@ -1158,7 +1250,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   const int64_t begin = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
   const int64_t end   = (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
   const uint32_t mask = (uint32_t)q.tcap - 1;
-  constexpr int R      = 4;
+  // the LDS exact-sum words take fewer than 2^31 addends: a larger partition uses the global slots
+  const bool big  = end - begin >= (int64_t(1) << 31);
+  constexpr int R = 4;
   for (int64_t i0 = begin + threadIdx.x; i0 < end; i0 += (int64_t)R * blockDim.x) {
     uint64_t kr[R];
     uint32_t fr[R];
@@ -1193,7 +1287,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
       // Slots never become empty again, so every thread reaches the same verdict for a key and a
       // key is either in this table or spilled, never both.
       int slot = -1;
-      if (kv && k != kGEmpty) {
+      if (kv && k != kGEmpty && !big) {
         uint32_t h = (uint32_t)(mix64(k) & mask);
         for (int probe = 0; probe < kPartProbes; ++probe) {
           unsigned long long cur = keys[h];
@@ -2124,6 +2218,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       int64_t rpb           = (n + nblk - 1) / nblk;
       rpb                   = (rpb + kSub3 - 1) / kSub3 * kSub3;
       nblk                  = (int)((n + rpb - 1) / rpb);
+      LDF_EXPECTS(rpb < INT32_MAX, "groupby: more than 2^31 rows per block of the main pass");
       plan.rows_per_block   = rpb;
       // 1. histogram of the non-hot rows per (partition, block)
       pcounts = Buf<uint32_t>(a, (size_t)kNumParts * nblk);
@@ -2308,6 +2403,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       int64_t rpb = (n + nblk - 1) / nblk;
       rpb         = (rpb + 1023) / 1024 * 1024;
       nblk        = (int)((n + rpb - 1) / rpb);
+      LDF_EXPECTS(rpb < INT32_MAX, "groupby: more than 2^31 rows per block of the main pass");
       plan.rows_per_block = rpb;
       if (generic) launch_main<true>(keys, n, t, plan, hot, nblk, s);
       else launch_main<false>(keys, n, t, plan, hot, nblk, s);

@@ -61,17 +61,34 @@ __device__ __forceinline__ void add_fixed192(unsigned long long* a0, unsigned lo
   if (y2 != 0) atomicAdd(a2, (unsigned long long)y2);
 }
 
-// LDS form of an exact sum: six u64 words, word k accumulating 32-bit chunk k of the 192-bit
-// two's-complement addends.  The adds need no return value (2^32 addends cannot overflow a
-// word); lds_fsum_value folds the words back into three 64-bit limbs once, at flush time.
-__device__ __forceinline__ void lds_fsum_add(unsigned long long* w, uint64_t l0, uint64_t l1, uint64_t l2)
+// LDS form of an exact sum: six signed 64-bit words, word k accumulating +-(32-bit chunk k of
+// round-toward-zero(|x| * 2^96)).  A value's 53-bit mantissa touches at most three consecutive
+// chunks, so an add is at most three LDS atomics without return value, whatever the sign.  A word
+// stays exact while it receives fewer than 2^31 addends (callers keep the rows per table below
+// that); lds_fsum_value folds the words into the 192-bit two's-complement sum once, at flush time.
+// Returns false for values the fixed-point range cannot hold (non-finite or |x| >= 2^62).
+__device__ __forceinline__ bool lds_fsum_add(unsigned long long* w, double x)
 {
-  if ((uint32_t)l0) atomicAdd(&w[0], (unsigned long long)(uint32_t)l0);
-  if (l0 >> 32) atomicAdd(&w[1], (unsigned long long)(l0 >> 32));
-  if ((uint32_t)l1) atomicAdd(&w[2], (unsigned long long)(uint32_t)l1);
-  if (l1 >> 32) atomicAdd(&w[3], (unsigned long long)(l1 >> 32));
-  if ((uint32_t)l2) atomicAdd(&w[4], (unsigned long long)(uint32_t)l2);
-  if (l2 >> 32) atomicAdd(&w[5], (unsigned long long)(l2 >> 32));
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const int ex     = (int)((b >> 52) & 0x7ff);
+  if (ex == 0x7ff || ex >= 1023 + 62) return false;
+  if (ex == 0) return true;  // zero or subnormal: below 2^-96, contributes 0
+  uint64_t m = (b & ((1ull << 52) - 1)) | (1ull << 52);
+  int shift  = ex - 1075 + 96;  // position of m's lowest bit in the fixed-point integer
+  if (shift < 0) {
+    m     = (-shift < 64) ? (m >> (-shift)) : 0;
+    shift = 0;
+  }
+  const int k0      = shift >> 5;  // <= 3: shift <= 105
+  const int r       = shift & 31;
+  const uint64_t lo = m << r;
+  const uint32_t hi = r ? (uint32_t)(m >> (64 - r)) : 0u;  // m << r < 2^84
+  const bool neg    = (b >> 63) != 0;
+  const uint32_t c0 = (uint32_t)lo, c1 = (uint32_t)(lo >> 32);
+  if (c0) atomicAdd(&w[k0], neg ? 0ull - (unsigned long long)c0 : (unsigned long long)c0);
+  if (c1) atomicAdd(&w[k0 + 1], neg ? 0ull - (unsigned long long)c1 : (unsigned long long)c1);
+  if (hi) atomicAdd(&w[k0 + 2], neg ? 0ull - (unsigned long long)hi : (unsigned long long)hi);
+  return true;
 }
 
 __device__ __forceinline__ void add192(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t a0, uint64_t a1, uint64_t a2)
@@ -88,12 +105,19 @@ __device__ __forceinline__ void add192(uint64_t& r0, uint64_t& r1, uint64_t& r2,
 __device__ __forceinline__ void lds_fsum_value(const unsigned long long* w, uint64_t& r0, uint64_t& r1, uint64_t& r2)
 {
   r0 = r1 = r2 = 0;
-  add192(r0, r1, r2, w[0], 0, 0);
-  add192(r0, r1, r2, w[1] << 32, w[1] >> 32, 0);
-  add192(r0, r1, r2, 0, w[2], 0);
-  add192(r0, r1, r2, 0, w[3] << 32, w[3] >> 32);
-  add192(r0, r1, r2, 0, 0, w[4]);
-  add192(r0, r1, r2, 0, 0, w[5] << 32);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {  // r += sign_extend(w[k]) << 32k  (mod 2^192)
+    const uint64_t x = w[k];
+    const uint64_t e = (x >> 63) ? ~0ull : 0ull;
+    switch (k) {
+      case 0: add192(r0, r1, r2, x, e, e); break;
+      case 1: add192(r0, r1, r2, x << 32, (x >> 32) | (e << 32), e); break;
+      case 2: add192(r0, r1, r2, 0, x, e); break;
+      case 3: add192(r0, r1, r2, 0, x << 32, (x >> 32) | (e << 32)); break;
+      case 4: add192(r0, r1, r2, 0, 0, x); break;
+      default: add192(r0, r1, r2, 0, 0, x << 32); break;
+    }
+  }
 }
 
 // Correctly rounded double of a 192-bit two's-complement integer times 2^-96.

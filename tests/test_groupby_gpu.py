@@ -129,6 +129,39 @@ def test_groupby_float_sum_exact_under_cancellation(gpu):
         assert s == float(exact), (key, s, float(exact))
 
 
+def _fixed96_sums(k, x):
+    """Per key: sum of trunc(x * 2^96) over the key's values as a Python integer (the fixed-point
+    semantics of the GPU's exact sums), then correctly rounded to float64 by true division."""
+    m, e = np.frexp(x)
+    mant = (m * 2.0**53).astype(np.int64)  # exact: x = mant * 2^(e - 53)
+    sh = (e.astype(np.int64) - 53 + 96).tolist()
+    acc = {}
+    for key, mt, s in zip(k.tolist(), mant.tolist(), sh):
+        v = mt << s if s >= 0 else (abs(mt) >> -s) * (1 if mt >= 0 else -1)
+        acc[key] = acc.get(key, 0) + v
+    return {key: v / (1 << 96) for key, v in acc.items()}
+
+
+def test_groupby_float_sum_exact_partitioned_path(gpu):
+    """The exact sums of the partitioned path (over 2^20 rows: hot keys in LDS, other keys through
+    partitions): mixed signs, magnitudes from 1e-20 to 1e17 and exact cancellation; every group's
+    sum is bit for bit the correctly rounded sum of trunc(x * 2^96)."""
+    rng = np.random.default_rng(11)
+    n = 1_500_000
+    hot = rng.integers(0, 8, n // 3)
+    cold = rng.integers(8, 300_000, n - n // 3)
+    k = rng.permutation(np.concatenate([hot, cold])).astype(np.int32)
+    x = rng.choice([-1.0, 1.0], n) * rng.uniform(1, 2, n) * 10.0 ** rng.uniform(-20, 17, n)
+    x[: n // 4] = -x[n // 4 : n // 2]  # exact cancellation between rows of different keys
+    t = pa.table({"k": k, "x": x})
+    got = gpu_groupby(t, ["k"], [("x", "sum", "s"), ("x", "count", "c")])
+    ref = _fixed96_sums(k, x)
+    gk, gs = got["k"].to_pylist(), got["s"].to_pylist()
+    assert len(gk) == len(ref)
+    bad = [(key, s, ref[key]) for key, s in zip(gk, gs) if s != ref[key]]
+    assert not bad, bad[:5]
+
+
 def test_groupby_out_of_range_values_fall_back(gpu):
     """|x| >= 2^62, inf and NaN leave the fixed-point range: the FP64 fallback keeps Arrow's
     IEEE results (inf, nan)."""
