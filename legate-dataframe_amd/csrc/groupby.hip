@@ -111,6 +111,7 @@ struct GbPlan {
   int64_t rows_per_block;
   int32_t dpart;  // COUNT_DISTINCT through pair partitions (partitioned path, one such accumulator)
   int32_t dcol;   // its value column (index into col)
+  int32_t part_bits;  // partitioned path: 2^part_bits partitions
 };
 
 struct HotList {
@@ -597,13 +598,13 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
 
 // ---- partitioned path for keys outside the hot set ---------------------------------------
 // Rows whose key is not hot are not aggregated with global atomics: the main pass scatters them
-// (key, value bits, a flags byte) into kNumParts hash partitions, LDS-staged so each partition
+// (key, value bits, a flags byte) into 2^part_bits hash partitions, LDS-staged so each partition
 // receives contiguous runs, and k_gb_part3 aggregates every (partition, sub-pass) in one block's
 // LDS table.  Each key therefore lives in exactly one place -- a hot LDS slot, a partition
 // block's LDS slot, or (only when a partition block's table is full) the global table -- and
 // the final values are stored, not added, into the global slot arrays.
-constexpr int kPartBits = 10;
-constexpr int kNumParts = 1 << kPartBits;
+constexpr int kMinPartBits = 10;
+constexpr int kMaxPartBits = 12;
 constexpr int kSub3     = kGbThreads * kGbRowsPerThread;  // rows staged per step
 constexpr int kFlagKeyValid = 1 << 4;
 constexpr int kFlagDistinct = 5;  // bits 5..7: the row's (key, value) pair was new, per DISTINCT
@@ -626,14 +627,41 @@ struct PartBufs {
   const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
 };
 
-// Record words in registers.  `off` and `width` are the same for every lane; word k is built and
-// read with constant k and masks (no dynamically indexed private array, which would live in
-// scratch memory).
-__device__ __forceinline__ uint64_t rec_word(int k, int off, int width, uint64_t bits)
-{
-  const uint64_t v = width >= 8 ? bits : bits & ((1ull << (8 * width)) - 1);
-  return (off >> 3) == k ? v << ((off & 7) * 8) : 0ull;
-}
+// A record being built in registers: eight named words (no private array, which a dynamically
+// indexed access would place in scratch memory).  `off` is the same for every lane, so put()
+// branches on it with scalar branches; values come zero-extended from their column.
+struct RecWords {
+  uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0, w7 = 0;
+  __device__ __forceinline__ void put(int off, uint64_t bits)
+  {
+    const uint64_t sv = bits << ((off & 7) * 8);
+    switch (off >> 3) {
+      case 0: w0 |= sv; break;
+      case 1: w1 |= sv; break;
+      case 2: w2 |= sv; break;
+      case 3: w3 |= sv; break;
+      case 4: w4 |= sv; break;
+      case 5: w5 |= sv; break;
+      case 6: w6 |= sv; break;
+      default: w7 |= sv; break;
+    }
+  }
+  __device__ __forceinline__ void store(unsigned char* base, int stride, int64_t i) const
+  {
+    unsigned char* d = base + i * (int64_t)stride;
+    if (stride == 8) {
+      *reinterpret_cast<uint64_t*>(d) = w0;
+      return;
+    }
+    uint4* q = reinterpret_cast<uint4*>(d);
+    q[0]     = make_uint4((uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32));
+    if (stride >= 32) q[1] = make_uint4((uint32_t)w2, (uint32_t)(w2 >> 32), (uint32_t)w3, (uint32_t)(w3 >> 32));
+    if (stride == 64) {
+      q[2] = make_uint4((uint32_t)w4, (uint32_t)(w4 >> 32), (uint32_t)w5, (uint32_t)(w5 >> 32));
+      q[3] = make_uint4((uint32_t)w6, (uint32_t)(w6 >> 32), (uint32_t)w7, (uint32_t)(w7 >> 32));
+    }
+  }
+};
 
 template <int W>
 __device__ __forceinline__ uint64_t rec_get(const uint64_t (&w)[W], int off, int width)
@@ -644,21 +672,6 @@ __device__ __forceinline__ uint64_t rec_get(const uint64_t (&w)[W], int off, int
   for (int k = 0; k < W; ++k) x |= w[k] & (0ull - (uint64_t)(k == wi));
   x >>= (off & 7) * 8;
   return width >= 8 ? x : x & ((1ull << (8 * width)) - 1);
-}
-
-__device__ __forceinline__ void rec_store(unsigned char* base, int stride, int64_t i, const uint64_t (&w)[8])
-{
-  unsigned char* d = base + i * (int64_t)stride;
-  if (stride == 8) {
-    *reinterpret_cast<uint64_t*>(d) = w[0];
-    return;
-  }
-  uint4* q = reinterpret_cast<uint4*>(d);
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (16 * k < stride)
-      q[k] = make_uint4((uint32_t)w[2 * k], (uint32_t)(w[2 * k] >> 32), (uint32_t)w[2 * k + 1],
-                        (uint32_t)(w[2 * k + 1] >> 32));
 }
 
 template <int W>
@@ -694,13 +707,13 @@ struct PartPlan {
   int32_t lds_bytes;
 };
 
-__device__ __forceinline__ uint32_t cold_part(uint64_t key)
+// partition of a key from its hash hk = mix64(key): the top `bits` bits (the LDS tables index by
+// low bits of the same hash)
+__device__ __forceinline__ uint32_t cold_part(uint64_t hk, int bits) { return (uint32_t)(hk >> (64 - bits)); }
+// sub-pass of a key in [0, S) from bits 22..53 of the same hash (multiply-high range reduction)
+__device__ __forceinline__ uint32_t cold_sub(uint64_t hk, int S)
 {
-  return (uint32_t)(mix64(key ^ 0x5bd1e9955bd1e995ull) >> (64 - kPartBits));
-}
-__device__ __forceinline__ uint32_t cold_sub(uint64_t key, int S)
-{
-  return (uint32_t)(mix64(key + 0x2545f4914f6cdd1dull) % (uint64_t)S);
+  return (uint32_t)((((hk >> 22) & 0xffffffffull) * (uint64_t)S) >> 32);
 }
 
 // COUNT_DISTINCT through pair partitions: the (hot index, value) pairs of hot rows are spread by
@@ -764,14 +777,15 @@ __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotL
   __syncthreads();
 }
 
-__device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* lds, uint64_t k, bool kv, int null_hot,
-                                          int sent_hot)
+// hot index of key k (hash hk = mix64(k)), -1 when k is not hot
+__device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* lds, uint64_t k, uint64_t hk, bool kv,
+                                          int null_hot, int sent_hot)
 {
   if (!kv) return null_hot;
   if (k == kGEmpty) return sent_hot;
   const int T                     = 2 * p.hot_cap;
   const unsigned long long* tkeys = reinterpret_cast<const unsigned long long*>(lds + p.tab_off);
-  uint32_t i                      = (uint32_t)(mix64(k) & (uint64_t)(T - 1));
+  uint32_t i                      = (uint32_t)(hk & (uint64_t)(T - 1));
   while (true) {
     unsigned long long cur = tkeys[i];
     if (cur == k) return reinterpret_cast<const uint16_t*>(lds + p.tidx_off)[i];
@@ -781,20 +795,26 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
 }
 
 // counts[part * nblk + blk] of the rows whose key is not hot; with dpart also
-// hcounts[hpart * nblk + blk] of the hot rows' distinct pairs
+// hcounts[hpart * nblk + blk] of the hot rows' distinct pairs.  kHistSplit blocks share each
+// main-pass block's rows (two resident blocks per CU) and add into zeroed counts.
+constexpr int kHistSplit = 2;
 __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
                                                          uint32_t* hcounts)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  __shared__ uint32_t hist[kNumParts];
   __shared__ uint32_t hhist[kHPairParts];
-  const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
-  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) hist[i] = 0;
+  const int nparts = 1 << p.part_bits;
+  uint32_t* hist   = reinterpret_cast<uint32_t*>(lds + p.rows_off);  // after the hot key table
+  const int nhot   = (int)min(*hot.count, (unsigned int)p.hot_cap);
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
   for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
   hot_setup(p, lds, hot, nhot, false, &s_null_hot, &s_sent_hot);
-  const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
-  const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
+  const int mblk      = blockIdx.x / kHistSplit;  // main-pass block
+  const int64_t share = p.rows_per_block / kHistSplit;
+  const int64_t begin = (int64_t)mblk * p.rows_per_block + (int64_t)(blockIdx.x % kHistSplit) * share;
+  const int64_t end   = begin + share < n ? begin + share : n;
+  const int nmain     = (int)(gridDim.x / kHistSplit);
   const DCol& kc      = keys.c[0];
   const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
   constexpr int R = 8;  // rows per thread whose loads are in flight together
@@ -812,9 +832,10 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if (r0 + (int64_t)j * blockDim.x >= end) break;
-      const int h = hot_lookup(p, lds, k[j], kv[j], s_null_hot, s_sent_hot);
+      const uint64_t hk = mix64(k[j]);
+      const int h       = hot_lookup(p, lds, k[j], hk, kv[j], s_null_hot, s_sent_hot);
       if (h < 0) {
-        atomicAdd(&hist[cold_part(k[j])], 1u);
+        atomicAdd(&hist[cold_part(hk, p.part_bits)], 1u);
       } else if (dv[j]) {
         const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)d[j];
         atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
@@ -822,9 +843,11 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) counts[(int64_t)i * gridDim.x + blockIdx.x] = hist[i];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    if (hist[i]) atomicAdd(&counts[(int64_t)i * nmain + mblk], hist[i]);
   if (p.dpart)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * gridDim.x + blockIdx.x] = hhist[i];
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
+      if (hhist[i]) atomicAdd(&hcounts[(int64_t)i * nmain + mblk], hhist[i]);
 }
 
 // LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
@@ -921,7 +944,7 @@ __device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds
 // each takes the next position of its partition's region with an LDS atomic on the region
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
 // accumulator is exact and order-independent).  LDS: [0, p.lds_bytes) hot table +
-// accumulators, then cursor u64[kNumParts].
+// accumulators, then cursor u64[2^part_bits] (then hcursor u64[kHPairParts] with dpart).
 template <bool DPART>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp)
@@ -929,10 +952,11 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
   unsigned long long* cursor  = reinterpret_cast<unsigned long long*>(lds + p.lds_bytes);
-  unsigned long long* hcursor = cursor + kNumParts;  // dpart only
+  const int nparts            = 1 << p.part_bits;
+  unsigned long long* hcursor = cursor + nparts;  // dpart only
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
-  for (int i = threadIdx.x; i < kNumParts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
   if (DPART)
     for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
       hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + blockIdx.x];
@@ -968,10 +992,13 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
       }
     }
     int hx[kGbRowsPerThread];  // hot index; -1 not hot; -2 past the end
+    uint32_t part[kGbRowsPerThread];
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
-      const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
-      hx[j]           = r >= end ? -2 : (nhot > 0 ? hot_lookup(p, lds, kb[j], kv[j], s_null_hot, s_sent_hot) : -1);
+      const int64_t r   = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      const uint64_t hk = mix64(kb[j]);
+      part[j]           = cold_part(hk, p.part_bits);
+      hx[j] = r >= end ? -2 : (nhot > 0 ? hot_lookup(p, lds, kb[j], hk, kv[j], s_null_hot, s_sent_hot) : -1);
     }
     acc_hot_rows<DPART, kGbRowsPerThread>(p, lds, keys, sub, hx, hot, v);
 #pragma unroll
@@ -1005,22 +1032,15 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         }
         ++di;
       }
-      const int64_t dest = (int64_t)atomicAdd(&cursor[cold_part(kb[j])], 1ull);
-      uint64_t w[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        if (8 * k >= pb.stride) {
-          w[k] = 0;
-          continue;
-        }
-        uint64_t x = rec_word(k, pb.key_off, pb.key_width, kb[j]) | rec_word(k, pb.flags_off, 1, fl);
-        if (p.ncol > 0) x |= rec_word(k, pb.val_off[0], p.col[0].width, v[j].b0);
-        if (p.ncol > 1) x |= rec_word(k, pb.val_off[1], p.col[1].width, v[j].b1);
-        if (p.ncol > 2) x |= rec_word(k, pb.val_off[2], p.col[2].width, v[j].b2);
-        if (p.ncol > 3) x |= rec_word(k, pb.val_off[3], p.col[3].width, v[j].b3);
-        w[k] = x;
-      }
-      rec_store(pb.rec, pb.stride, dest, w);
+      const int64_t dest = (int64_t)atomicAdd(&cursor[part[j]], 1ull);
+      RecWords w;
+      w.put(pb.key_off, kb[j]);
+      w.put(pb.flags_off, fl);
+      if (p.ncol > 0) w.put(pb.val_off[0], v[j].b0);
+      if (p.ncol > 1) w.put(pb.val_off[1], v[j].b1);
+      if (p.ncol > 2) w.put(pb.val_off[2], v[j].b2);
+      if (p.ncol > 3) w.put(pb.val_off[3], v[j].b3);
+      w.store(pb.rec, pb.stride, dest);
     }
   }
   __syncthreads();
@@ -1129,44 +1149,75 @@ __global__ __launch_bounds__(256) void k_gb_hot_reduce(GbPlan p, HotList hot, Ho
   }
 }
 
-// accumulate a partition row into LDS slot `slot` (DISTINCT from the row's flag bits)
-__device__ __forceinline__ void acc_part(const GbPlan& p, const PartPlan& q, unsigned char* lds, int slot, const RowVals v,
-                                         uint32_t fl)
+// LDS accumulators of R partition rows of one thread (slot[j] < 0: the row is not accumulated
+// here); DISTINCT counts come from the rows' flag bits.  Accumulator loop outside, row loop inside.
+template <int R>
+__device__ __forceinline__ void acc_part_rows(const GbPlan& p, const PartPlan& q, unsigned char* lds,
+                                              const int (&slot)[R], const RowVals (&v)[R], const uint32_t (&fl)[R])
 {
-  atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.rows_off)[slot], 1u);
+  uint32_t* rows = reinterpret_cast<uint32_t*>(lds + q.rows_off);
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (slot[j] >= 0) atomicAdd(&rows[slot[j]], 1u);
   int di = 0;
   for (int i = 0; i < p.nacc; ++i) {
     const AccDev& a = p.a[i];
     if (a.kind == ACC_ROWS) continue;
     if (a.kind == ACC_DISTINCT) {
-      if ((fl >> (kFlagDistinct + di)) & 1) atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.acc_off[i])[slot], 1u);
+      uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + q.acc_off[i]);
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (slot[j] >= 0 && ((fl[j] >> (kFlagDistinct + di)) & 1)) atomicAdd(&cnt[slot[j]], 1u);
       ++di;
       continue;
     }
-    if (!((v.valid >> a.col) & 1)) continue;
-    const ColDev& c   = p.col[a.col];
-    const uint64_t vb = pick(v, a.col);
+    const int col = a.col;
+    const int ct  = p.col[col].type;
+    const int cw  = p.col[col].width;
+    bool ok[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ok[j] = slot[j] >= 0 && ((v[j].valid >> col) & 1);
     switch (a.kind) {
-      case ACC_VCOUNT: atomicAdd(&reinterpret_cast<uint32_t*>(lds + q.acc_off[i])[slot], 1u); break;
-      case ACC_ISUM:
-        atomicAdd(&reinterpret_cast<unsigned long long*>(lds + q.acc_off[i])[slot],
-                  (unsigned long long)bits_as_int(vb, c.type, c.width));
+      case ACC_VCOUNT: {
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + q.acc_off[i]);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j]) atomicAdd(&cnt[slot[j]], 1u);
         break;
+      }
+      case ACC_ISUM: {
+        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j]) atomicAdd(&acc[slot[j]], (unsigned long long)bits_as_int(pick(v[j], col), ct, cw));
+        break;
+      }
       case ACC_FSUM: {
         unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
-        if (!lds_fsum_add(&base[6 * slot], bits_as_double(vb, c.type, c.width))) *a.err = 1;
+        bool bad                 = false;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (ok[j] && !lds_fsum_add(&base[6 * slot[j]], bits_as_double(pick(v[j], col), ct, cw))) bad = true;
+        if (bad) *a.err = 1;
         break;
       }
       case ACC_MIN:
       case ACC_MAX: {
-        if (is_float_type(c.type) && isnan(bits_as_double(vb, c.type, c.width))) {
-          atomicOr(&reinterpret_cast<uint32_t*>(lds + q.flags_off)[slot], 1u << a.flag_bit);
-          break;
+        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
+        uint32_t* flags         = reinterpret_cast<uint32_t*>(lds + q.flags_off);
+        const bool isf          = is_float_type(ct);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!ok[j]) continue;
+          const uint64_t vb = pick(v[j], col);
+          if (isf && isnan(bits_as_double(vb, ct, cw))) {
+            atomicOr(&flags[slot[j]], 1u << a.flag_bit);
+            continue;
+          }
+          const uint64_t o = ordered_of(vb, ct, cw);
+          if (a.kind == ACC_MIN) atomicMin(&acc[slot[j]], (unsigned long long)o);
+          else atomicMax(&acc[slot[j]], (unsigned long long)o);
         }
-        uint64_t o               = ordered_of(vb, c.type, c.width);
-        unsigned long long* cell = &reinterpret_cast<unsigned long long*>(lds + q.acc_off[i])[slot];
-        if (a.kind == ACC_MIN) atomicMin(cell, (unsigned long long)o);
-        else atomicMax(cell, (unsigned long long)o);
         break;
       }
       default: break;
@@ -1227,7 +1278,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   const int rest  = blockIdx.x >> 3;
   const int sub   = rest % q.S;
   const int part  = (rest / q.S) * 8 + xcd;
-  if (part >= kNumParts) return;
+  if (part >= (1 << p.part_bits)) return;
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(lds + q.key_off);
   for (int i = threadIdx.x; i < q.tcap; i += blockDim.x) {
     keys[i]                                           = kGEmpty;
@@ -1252,65 +1303,65 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   const uint32_t mask = (uint32_t)q.tcap - 1;
   // the LDS exact-sum words take fewer than 2^31 addends: a larger partition uses the global slots
   const bool big  = end - begin >= (int64_t(1) << 31);
-  constexpr int R = 4;
+  constexpr int R = W >= 8 ? 2 : 4;  // rows per thread in flight (R * W record words in registers)
   for (int64_t i0 = begin + threadIdx.x; i0 < end; i0 += (int64_t)R * blockDim.x) {
-    uint64_t kr[R];
-    uint32_t fr[R];
-    RowVals vr[R];
+    uint64_t w[R][W];
 #pragma unroll
     for (int j = 0; j < R; ++j) {  // issue the loads of R rows before using any
       const int64_t i = i0 + (int64_t)j * blockDim.x;
-      kr[j]           = 0;
-      fr[j]           = 0;
-      vr[j]           = RowVals{0, 0, 0, 0, 0};
-      if (i < end) {
-        uint64_t w[W];
-        rec_load<W>(pb.rec, i, w);
-        kr[j]                = rec_get<W>(w, pb.key_off, pb.key_width);
-        fr[j]                = (uint32_t)rec_get<W>(w, pb.flags_off, 1);
-        uint64_t x[kMaxCols] = {0, 0, 0, 0};
-#pragma unroll
-        for (int c = 0; c < kMaxCols; ++c)
-          if (c < p.ncol) x[c] = rec_get<W>(w, pb.val_off[c], p.col[c].width);
-        vr[j] = RowVals{x[0], x[1], x[2], x[3], fr[j] & 0xF};
-      }
+      if (i < end) rec_load<W>(pb.rec, i, w[j]);
     }
+    int slot[R];  // LDS slot; -1 spilled to the global table; -2 not a row of this block
+    uint32_t fl[R];
+    uint64_t kr[R];
+    RowVals vr[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const int64_t i = i0 + (int64_t)j * blockDim.x;
-      if (i >= end) continue;
-      const uint64_t k = kr[j];
-      if (q.S > 1 && (int)cold_sub(k, q.S) != sub) continue;
-      const uint32_t fl = fr[j];
-      const bool kv     = fl & kFlagKeyValid;
+      slot[j] = -2;
+      fl[j]   = 0;
+      kr[j]   = 0;
+      vr[j]   = RowVals{0, 0, 0, 0, 0};
+      if (i0 + (int64_t)j * blockDim.x >= end) continue;
+      const uint64_t k  = rec_get<W>(w[j], pb.key_off, pb.key_width);
+      const uint64_t hk = mix64(k);
+      if (q.S > 1 && (int)cold_sub(hk, q.S) != sub) continue;
+      kr[j]                = k;
+      fl[j]                = (uint32_t)rec_get<W>(w[j], pb.flags_off, 1);
+      uint64_t x[kMaxCols] = {0, 0, 0, 0};
+#pragma unroll
+      for (int c = 0; c < kMaxCols; ++c)
+        if (c < p.ncol) x[c] = rec_get<W>(w[j], pb.val_off[c], p.col[c].width);
+      vr[j]         = RowVals{x[0], x[1], x[2], x[3], fl[j] & 0xF};
+      const bool kv = fl[j] & kFlagKeyValid;
       // A key spills when the first kPartProbes slots of its probe sequence hold other keys.
       // Slots never become empty again, so every thread reaches the same verdict for a key and a
       // key is either in this table or spilled, never both.
-      int slot = -1;
+      slot[j] = -1;
       if (kv && k != kGEmpty && !big) {
-        uint32_t h = (uint32_t)(mix64(k) & mask);
+        uint32_t h = (uint32_t)(hk & mask);
         for (int probe = 0; probe < kPartProbes; ++probe) {
           unsigned long long cur = keys[h];
           if (cur == k) {
-            slot = (int)h;
+            slot[j] = (int)h;
             break;
           }
           if (cur == kGEmpty) {
             unsigned long long old = atomicCAS(&keys[h], kGEmpty, (unsigned long long)k);
             if (old == kGEmpty || old == k) {
-              slot = (int)h;
+              slot[j] = (int)h;
               break;
             }
           }
           h = (h + 1) & mask;
         }
       }
-      if (slot >= 0) {
-        acc_part(p, q, lds, slot, vr[j], fl);
-      } else {
-        const int64_t s = gb_insert_bits(k, kv, t);
-        if (s >= 0) acc_spill(p, s, vr[j], fl);
-      }
+    }
+    acc_part_rows<R>(p, q, lds, slot, vr, fl);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (slot[j] != -1) continue;
+      const int64_t s = gb_insert_bits(kr[j], (fl[j] & kFlagKeyValid) != 0, t);
+      if (s >= 0) acc_spill(p, s, vr[j], fl[j]);
     }
   }
   __syncthreads();
@@ -2211,7 +2262,36 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         plan.dpart = 1;
         plan.dcol  = plan.a[dacc].col;
       }
-      const int stage_bytes = kNumParts * 8 + (dpart ? kHPairParts * 8 : 0);
+      // partition tables (k_gb_part3): LDS slots per block from the accumulators' bytes per slot
+      PartPlan q;
+      std::memset(&q, 0, sizeof(q));
+      int per_slot = 8 + 4 + 4;
+      for (int i = 0; i < nacc; ++i) {
+        switch (plan.a[i].kind) {
+          case ACC_VCOUNT:
+          case ACC_DISTINCT: per_slot += 4; break;
+          case ACC_ISUM:
+          case ACC_MIN:
+          case ACC_MAX: per_slot += 8; break;
+          case ACC_FSUM: per_slot += 48; break;
+          default: break;
+        }
+      }
+      q.tcap = 256;
+      while (q.tcap * 2 * per_slot + 64 * (nacc + 3) <= 150 * 1024 && q.tcap < 8192) q.tcap *= 2;
+      // Partitions: the fewest (2^10 .. 2^12) whose keys fit one table each -- the sample's
+      // estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load -- as far as the
+      // main pass's LDS cursors allow; keys beyond that take sub-passes over a partition.
+      const int hc_bytes = dpart ? kHPairParts * 8 : 0;
+      int pbits          = kMinPartBits;
+      while (pbits < kMaxPartBits && 1.25 * (double)est / (double)(1 << pbits) > 0.45 * q.tcap &&
+             plan.lds_bytes + (1 << (pbits + 1)) * 8 + hc_bytes <= 150 * 1024)
+        ++pbits;
+      plan.part_bits             = pbits;
+      const int nparts           = 1 << pbits;
+      const double keys_per_part = 1.25 * (double)est / nparts;
+      q.S                        = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
+      const int stage_bytes      = nparts * 8 + hc_bytes;
       const int main_lds    = plan.lds_bytes + stage_bytes;
       int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
       int nblk              = 256 * per_cu;
@@ -2221,8 +2301,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       LDF_EXPECTS(rpb < INT32_MAX, "groupby: more than 2^31 rows per block of the main pass");
       plan.rows_per_block   = rpb;
       // 1. histogram of the non-hot rows per (partition, block)
-      pcounts = Buf<uint32_t>(a, (size_t)kNumParts * nblk);
-      poffs   = Buf<uint64_t>(a, (size_t)kNumParts * nblk + 1);
+      pcounts = Buf<uint32_t>(a, (size_t)nparts * nblk);
+      poffs   = Buf<uint64_t>(a, (size_t)nparts * nblk + 1);
       {
         static bool attr = false;
         if (!attr) {
@@ -2235,24 +2315,38 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           attr = true;
         }
+        fill_u32(pcounts.get(), 0, (int64_t)nparts * nblk, s);
         if (dpart) {
           hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
           hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
+          fill_u32(hcounts.get(), 0, (int64_t)kHPairParts * nblk, s);
         }
         ProfScope prof("gb_hist", s);
-        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk), dim3(kGbThreads), plan.rows_off, s, keys, n, plan, hot, pcounts.get(),
-                           hcounts.get());
+        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk * kHistSplit), dim3(kGbThreads), plan.rows_off + nparts * 4, s, keys,
+                           n, plan, hot, pcounts.get(), hcounts.get());
         LDF_HIP_TRY(hipGetLastError());
       }
-      exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)kNumParts * nblk, a);
+      exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nblk, a);
       if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nblk, a);
       uint64_t totals[2] = {0, 0};
-      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)kNumParts * nblk, 8, hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)nparts * nblk, 8, hipMemcpyDeviceToHost, s));
       if (dpart)
         LDF_HIP_TRY(hipMemcpyAsync(&totals[1], hoffs.get() + (size_t)kHPairParts * nblk, 8, hipMemcpyDeviceToHost, s));
       LDF_HIP_TRY(hipStreamSynchronize(s));
       const int64_t ncold = (int64_t)totals[0];
       const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart)
+      {  // LDF_GROUPBY_TRACE=1: one line per call with the partitioned path's plan
+        static const bool trace = [] {
+          const char* e = std::getenv("LDF_GROUPBY_TRACE");
+          return e != nullptr && e[0] == '1';
+        }();
+        if (trace)
+          std::fprintf(stderr,
+                       "[ldf groupby] rows=%lld est=%lld hot_cap=%d blocks=%d part_bits=%d sub_passes=%d "
+                       "table_slots=%d cold_rows=%lld hot_pairs=%lld dpart=%d\n",
+                       (long long)n, (long long)est, plan.hot_cap, nblk, plan.part_bits, q.S, q.tcap,
+                       (long long)ncold, (long long)nhp, (int)dpart);
+      }
       // 2. sets of (key, value) pairs for COUNT_DISTINCT: hot keys by global slot, others by key
       uint64_t hcap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)(n - ncold))));
       uint64_t ccap = next_pow2(std::max<uint64_t>(1024, (uint64_t)(1.5 * (double)ncold)));
@@ -2328,25 +2422,6 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
       LDF_HIP_TRY(hipGetLastError());
       // 6. one block per (partition, sub-pass)
-      PartPlan q;
-      std::memset(&q, 0, sizeof(q));
-      int per_slot = 8 + 4 + 4;
-      for (int i = 0; i < nacc; ++i) {
-        switch (plan.a[i].kind) {
-          case ACC_VCOUNT:
-          case ACC_DISTINCT: per_slot += 4; break;
-          case ACC_ISUM:
-          case ACC_MIN:
-          case ACC_MAX: per_slot += 8; break;
-          case ACC_FSUM: per_slot += 48; break;
-          default: break;
-        }
-      }
-      q.tcap = 256;
-      while (q.tcap * 2 * per_slot + 64 * (nacc + 3) <= 150 * 1024 && q.tcap < 8192) q.tcap *= 2;
-      // the sample's estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load
-      const double keys_per_part = 1.25 * (double)est / kNumParts;
-      q.S          = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
       int32_t loff = 0;
       auto ltake   = [&](int bytes) {
         int32_t o = loff;
@@ -2371,7 +2446,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       q.lds_bytes = loff;
       if (ncold > 0) {
         ProfScope prof("gb_partitions", s);
-        const int grid = kNumParts * q.S;
+        const int grid = nparts * q.S;
         switch (pb.stride) {
           case 8: hipLaunchKernelGGL(k_gb_part3<1>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
           case 16: hipLaunchKernelGGL(k_gb_part3<2>, dim3(grid), dim3(kGbThreads), q.lds_bytes, s, plan, q, pb, t); break;
@@ -2386,7 +2461,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         ProfScope prof("gb_distinct", s);
         const AccDev& d = plan.a[dacc];
         if (ncold > 0) {
-          hipLaunchKernelGGL(k_gb_cold_distinct, dim3(kNumParts), dim3(1024), 0, s, pb, plan.dcol,
+          hipLaunchKernelGGL(k_gb_cold_distinct, dim3(nparts), dim3(1024), 0, s, pb, plan.dcol,
                              plan.col[plan.dcol].width, t, hfine.get(), d.g0, d.err);
           LDF_HIP_TRY(hipGetLastError());
         }

@@ -79,14 +79,20 @@ def test_groupby_c2_mix_vs_oracle(gpu, seed):
     assert_groupby_equal(got, exp, ["k"], rel=1e-12)
 
 
-@pytest.mark.parametrize("keys_kind", ["uniform_300k", "zipf_1M", "few_16", "nullkeys_int64", "int64_wide"])
+@pytest.mark.parametrize("keys_kind", ["uniform_300k", "zipf_1M", "few_16", "nullkeys_int64", "int64_wide",
+                                       "uniform_1200k", "uniform_wide_4M"])
 def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatch):
     """Inputs of >= 1M rows take the partitioned path (hot keys in LDS, the rest hash-partitioned
     and aggregated per partition in LDS); LDF_GROUPBY_PARTITIONED=0 forces the global-table path.
-    Both must equal the oracle (integers exactly, FP64 sums within 1e-12)."""
+    Both must equal the oracle (integers exactly, FP64 sums within 1e-12).  The key counts span
+    2^10, 2^11 and 2^12 partitions, and (uniform_wide_4M, ~2.9M keys) sub-passes per partition."""
     rng = np.random.default_rng(21)
-    n = 1_500_000
-    if keys_kind == "uniform_300k":
+    n = 4_500_000 if keys_kind == "uniform_wide_4M" else 1_500_000
+    if keys_kind == "uniform_1200k":
+        k = pa.array(rng.integers(0, 1_200_000, n).astype(np.int32))
+    elif keys_kind == "uniform_wide_4M":
+        k = pa.array(rng.integers(0, 4_000_000, n).astype(np.int32))
+    elif keys_kind == "uniform_300k":
         k = pa.array(rng.integers(0, 300_000, n).astype(np.int32))
     elif keys_kind == "zipf_1M":
         k = _c2_like(n, 3, n_keys=1_000_000)["k"]
@@ -102,7 +108,7 @@ def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatc
                   "u": pa.array(rng.integers(0, 10_000, n).astype(np.int32), mask=rng.random(n) < 0.01)})
     aggs = [("f", "sum", "fs"), ("f", "mean", "fm"), ("i", "sum", "is"), ("i", "min", "imin"), ("i", "max", "imax"),
             ("i", "count", "ic"), ("f", "max", "fmax"), (None, "count_all", "n")]
-    if keys_kind in ("uniform_300k", "zipf_1M", "few_16"):
+    if keys_kind in ("uniform_300k", "zipf_1M", "few_16", "uniform_1200k", "uniform_wide_4M"):
         aggs.append(("u", "count_distinct", "nu"))
     got = gpu_groupby(t, ["k"], aggs)
     monkeypatch.setenv("LDF_GROUPBY_PARTITIONED", "0")
