@@ -726,15 +726,16 @@ constexpr int kHPairFine  = 16;
 constexpr int kPairSet    = 8192;  // LDS pair-set slots of the dedupe kernels (64 KB)
 __device__ __forceinline__ uint64_t pair_hash(unsigned long long pr) { return mix64(pr ^ 0x94d049bb133111ebull); }
 
-// Builds the LDS table of hot keys (and clears the hot accumulators when `accs`).
+// Builds the LDS table of hot keys when `table` and clears the hot accumulators when `accs`.
 __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
-                                 int* s_null_hot, int* s_sent_hot)
+                                 bool table, int* s_null_hot, int* s_sent_hot)
 {
   const int H               = p.hot_cap;
   const int T               = 2 * H;
   unsigned long long* tkeys = reinterpret_cast<unsigned long long*>(lds + p.tab_off);
   uint16_t* tidx            = reinterpret_cast<uint16_t*>(lds + p.tidx_off);
-  for (int i = threadIdx.x; i < T; i += blockDim.x) tkeys[i] = kGEmpty;
+  if (table)
+    for (int i = threadIdx.x; i < T; i += blockDim.x) tkeys[i] = kGEmpty;
   if (accs) {
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
       reinterpret_cast<uint32_t*>(lds + p.rows_off)[i]  = 0;
@@ -757,7 +758,7 @@ __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotL
     *s_sent_hot = -1;
   }
   __syncthreads();
-  for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+  for (int h = threadIdx.x; h < nhot && table; h += blockDim.x) {
     int sp = hot.special[h];
     if (sp == 1) *s_sent_hot = h;
     else if (sp == 2) *s_null_hot = h;
@@ -794,12 +795,14 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
   }
 }
 
-// counts[part * nblk + blk] of the rows whose key is not hot; with dpart also
-// hcounts[hpart * nblk + blk] of the hot rows' distinct pairs.  kHistSplit blocks share each
-// main-pass block's rows (two resident blocks per CU) and add into zeroed counts.
+// Classifies every row: codes[r] = its hot index, or kCodeCold | its partition; counts
+// counts[part * nblk + blk] of the rows whose key is not hot and, with dpart, hcounts[hpart * nblk +
+// blk] of the hot rows' distinct pairs.  kHistSplit blocks share each main-pass block's rows and
+// add into zeroed counts.
 constexpr int kHistSplit = 2;
+constexpr uint32_t kCodeCold = 0x8000u;
 __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
-                                                         uint32_t* hcounts)
+                                                         uint32_t* hcounts, uint16_t* __restrict__ codes)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
@@ -809,7 +812,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
   const int nhot   = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
   for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
-  hot_setup(p, lds, hot, nhot, false, &s_null_hot, &s_sent_hot);
+  hot_setup(p, lds, hot, nhot, false, true, &s_null_hot, &s_sent_hot);
   const int mblk      = blockIdx.x / kHistSplit;  // main-pass block
   const int64_t share = p.rows_per_block / kHistSplit;
   const int64_t begin = (int64_t)mblk * p.rows_per_block + (int64_t)(blockIdx.x % kHistSplit) * share;
@@ -831,11 +834,14 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      if (r0 + (int64_t)j * blockDim.x >= end) break;
+      const int64_t r = r0 + (int64_t)j * blockDim.x;
+      if (r >= end) break;
       const uint64_t hk = mix64(k[j]);
       const int h       = hot_lookup(p, lds, k[j], hk, kv[j], s_null_hot, s_sent_hot);
+      const uint32_t cp = cold_part(hk, p.part_bits);
+      codes[r]          = (uint16_t)(h >= 0 ? (uint32_t)h : (kCodeCold | cp));
       if (h < 0) {
-        atomicAdd(&hist[cold_part(hk, p.part_bits)], 1u);
+        atomicAdd(&hist[cp], 1u);
       } else if (dv[j]) {
         const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)d[j];
         atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
@@ -947,7 +953,7 @@ __device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds
 // accumulators, then cursor u64[2^part_bits] (then hcursor u64[kHPairParts] with dpart).
 template <bool DPART>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
-                                                         HotParts hp)
+                                                         HotParts hp, const uint16_t* __restrict__ codes)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
@@ -960,7 +966,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
   if (DPART)
     for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
       hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + blockIdx.x];
-  hot_setup(p, lds, hot, nhot, true, &s_null_hot, &s_sent_hot);
+  hot_setup(p, lds, hot, nhot, true, false, &s_null_hot, &s_sent_hot);  // rows come classified
 
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
   const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
@@ -969,13 +975,16 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
     RowVals v[kGbRowsPerThread];
     uint64_t kb[kGbRowsPerThread];
     bool kv[kGbRowsPerThread];
+    uint32_t code[kGbRowsPerThread];
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
       const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
       v[j]            = RowVals{0, 0, 0, 0, 0};
       kb[j]           = 0;
       kv[j]           = false;
+      code[j]         = 0;
       if (r < end) {
+        code[j]              = codes[r];
         kv[j]                = valid_at(kc.valid, r);
         kb[j]                = kv[j] ? load_bits(kc.data, kc.width, r) : 0;
         uint32_t vm          = 0;
@@ -995,10 +1004,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
     uint32_t part[kGbRowsPerThread];
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
-      const int64_t r   = sub + (int64_t)j * blockDim.x + threadIdx.x;
-      const uint64_t hk = mix64(kb[j]);
-      part[j]           = cold_part(hk, p.part_bits);
-      hx[j] = r >= end ? -2 : (nhot > 0 ? hot_lookup(p, lds, kb[j], hk, kv[j], s_null_hot, s_sent_hot) : -1);
+      const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      part[j]         = code[j] & (kCodeCold - 1);
+      hx[j]           = r >= end ? -2 : ((code[j] & kCodeCold) ? -1 : (int)code[j]);
     }
     acc_hot_rows<DPART, kGbRowsPerThread>(p, lds, keys, sub, hx, hot, v);
 #pragma unroll
@@ -2252,6 +2260,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     Buf<unsigned char> precs;
     Buf<unsigned long long> hparts;
     Buf<uint32_t> hcounts;
+    Buf<uint16_t> codes;
     Buf<uint64_t> hoffs;
     Buf<unsigned long long> hpairs, hfine;
     plan.dpart = 0;
@@ -2321,9 +2330,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
           fill_u32(hcounts.get(), 0, (int64_t)kHPairParts * nblk, s);
         }
+        codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
         hipLaunchKernelGGL(k_gb_hist3, dim3(nblk * kHistSplit), dim3(kGbThreads), plan.rows_off + nparts * 4, s, keys,
-                           n, plan, hot, pcounts.get(), hcounts.get());
+                           n, plan, hot, pcounts.get(), hcounts.get(), codes.get());
         LDF_HIP_TRY(hipGetLastError());
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nblk, a);
@@ -2414,9 +2424,11 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       {
         ProfScope prof("gb_aggregate", s);
         if (dpart)
-          hipLaunchKernelGGL(k_gb_main3<true>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
+          hipLaunchKernelGGL(k_gb_main3<true>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp,
+                             (const uint16_t*)codes.get());
         else
-          hipLaunchKernelGGL(k_gb_main3<false>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp);
+          hipLaunchKernelGGL(k_gb_main3<false>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp,
+                             (const uint16_t*)codes.get());
         LDF_HIP_TRY(hipGetLastError());
       }
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
