@@ -220,44 +220,63 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int64_t kScanTile = (int64_t)kScanBlock * kScanItems;
 
+// Tile sums: the order of the adds is irrelevant, so lanes read consecutive elements (coalesced).
 template <typename In>
 __global__ __launch_bounds__(kScanBlock) void k_scan_tile_sums(const In* __restrict__ in, int64_t n,
                                                                uint64_t* __restrict__ sums)
 {
   __shared__ uint64_t smem[17];
-  int64_t base = blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  uint64_t s   = 0;
+  const int64_t base = blockIdx.x * kScanTile + threadIdx.x;
+  uint64_t s         = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    int64_t i = base + j;
+    const int64_t i = base + (int64_t)j * kScanBlock;
     if (i < n) s += (uint64_t)in[i];
   }
   uint64_t t = block_reduce_sum(s, smem);
   if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
+// Tile scan: coalesced loads into LDS, each thread scans kScanItems consecutive elements there
+// (one pad word per kScanItems keeps those reads off a single bank), coalesced stores back.
+__device__ __forceinline__ int scan_pad(int idx) { return idx + (idx >> 4); }
+
 template <typename In>
 __global__ __launch_bounds__(kScanBlock) void k_scan_tile_apply(const In* __restrict__ in, int64_t n,
                                                                 const uint64_t* __restrict__ tile_off,
                                                                 uint64_t* __restrict__ out)
 {
+  static_assert(kScanItems == 16, "scan_pad assumes 16 items per thread");
   __shared__ uint64_t smem[17];
-  int64_t base = blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  __shared__ uint64_t tile[kScanTile + kScanTile / kScanItems];
+  const int64_t base = blockIdx.x * kScanTile;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int idx   = j * kScanBlock + threadIdx.x;
+    const int64_t i = base + idx;
+    tile[scan_pad(idx)] = (i < n) ? (uint64_t)in[i] : 0;
+  }
+  __syncthreads();
   uint64_t v[kScanItems];
   uint64_t s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    int64_t i = base + j;
-    v[j]      = (i < n) ? (uint64_t)in[i] : 0;
+    v[j] = tile[scan_pad(threadIdx.x * kScanItems + j)];
     s += v[j];
   }
   uint64_t total;
   uint64_t run = block_exclusive_scan(s, smem, &total) + (tile_off ? tile_off[blockIdx.x] : 0);
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    int64_t i = base + j;
-    if (i <= n) out[i] = run;  // i == n writes the total
+    tile[scan_pad(threadIdx.x * kScanItems + j)] = run;
     run += v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int idx   = j * kScanBlock + threadIdx.x;
+    const int64_t i = base + idx;
+    if (i <= n) out[i] = tile[scan_pad(idx)];  // i == n writes the total
   }
 }
 
