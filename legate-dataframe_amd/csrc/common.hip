@@ -11,6 +11,7 @@
 
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
+#include "ldf_rec.hpp"
 
 namespace ldf {
 
@@ -392,7 +393,100 @@ struct GatherSet {
   const int64_t* soff[kMaxGather];
   void* dst[kMaxGather];
   uint64_t* dval[kMaxGather];
+  int64_t* start[kMaxGather];  // packed path: each output row's source start of a string column
 };
+
+// Packed source rows for a gather through a random map: each source row's fields -- fixed-width
+// values, a string's start (8 bytes) and length (4 bytes), and a byte of validity bits when any
+// column has validity -- in one record, so an output row costs one random record read instead of
+// one per column (and per string offset).
+struct PackSet {
+  int ncols;
+  int32_t off[kMaxGather];   // value, or string start
+  int32_t loff[kMaxGather];  // string length
+  int32_t voff;              // validity bits (bit c = column c), -1 when no column has validity
+  int32_t stride;
+};
+
+__global__ __launch_bounds__(256) void k_pack_rows(GatherSet g, PackSet ps, int64_t n, unsigned char* __restrict__ rec)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    RecWords w;
+    uint32_t vb = 0;
+#pragma unroll
+    for (int c = 0; c < kMaxGather; ++c) {
+      if (c >= g.ncols) break;
+      if (g.width[c] > 0) {
+        w.put(ps.off[c], load_bits(g.src[c], g.width[c], r));
+      } else {
+        const int64_t s0 = g.soff[c][r];
+        w.put(ps.off[c], (uint64_t)s0);
+        w.put(ps.loff[c], (uint64_t)(uint32_t)(g.soff[c][r + 1] - s0));
+      }
+      if (valid_at(g.sval[c], r)) vb |= 1u << c;
+    }
+    if (ps.voff >= 0) w.put(ps.voff, vb);
+    w.store(rec, ps.stride, r);
+  }
+}
+
+__device__ __forceinline__ void store_width(void* dst, int width, int64_t i, uint64_t v)
+{
+  switch (width) {
+    case 1: static_cast<uint8_t*>(dst)[i] = (uint8_t)v; break;
+    case 2: static_cast<uint16_t*>(dst)[i] = (uint16_t)v; break;
+    case 4: static_cast<uint32_t*>(dst)[i] = (uint32_t)v; break;
+    default: static_cast<uint64_t*>(dst)[i] = v; break;
+  }
+}
+
+// k_gather_multi through packed records (W = record stride / 8).
+template <int W>
+__global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, const unsigned char* __restrict__ rec,
+                                                       const int32_t* __restrict__ map, int64_t n,
+                                                       unsigned long long* __restrict__ null_count)
+{
+  const int lane = threadIdx.x & 63;
+  uint64_t nulls[kMaxGather];
+#pragma unroll
+  for (int c = 0; c < kMaxGather; ++c) nulls[c] = 0;
+  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + lane;
+    const int32_t m = i < n ? map[i] : -1;
+    uint64_t w[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = 0;
+    if (m >= 0) rec_load<W>(rec, m, w);
+    const uint32_t vb = ps.voff >= 0 ? (uint32_t)rec_get<W>(w, ps.voff, 1) : 0xFFu;
+#pragma unroll
+    for (int c = 0; c < kMaxGather; ++c) {
+      if (c >= g.ncols) break;
+      const bool ok = m >= 0 && ((vb >> c) & 1u);
+      if (i < n) {
+        if (g.width[c] == 0) {
+          static_cast<uint64_t*>(g.dst[c])[i] = ok ? rec_get<W>(w, ps.loff[c], 4) : 0;
+          g.start[c][i]                       = (int64_t)rec_get<W>(w, ps.off[c], 8);
+        } else {
+          store_width(g.dst[c], g.width[c], i, m >= 0 ? rec_get<W>(w, ps.off[c], g.width[c]) : 0);
+        }
+      }
+      if (g.dval[c] != nullptr) {
+        const uint64_t b = __ballot(ok);
+        if (lane == 0) {
+          g.dval[c][base >> 6]   = b;
+          const int64_t in_range = n - base < 64 ? n - base : 64;
+          nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(b);
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < kMaxGather; ++c)
+      if (c < g.ncols && nulls[c] != 0) atomicAdd(&null_count[c], (unsigned long long)nulls[c]);
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ void gather_store(const void* src, void* dst, int32_t m, int64_t i)
@@ -449,11 +543,14 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
 // row offsets staged in LDS.  Words shared with a neighbouring wave's range are written with
 // byte stores, so no two waves write the same byte.
 constexpr int kStrBlock = 256;
+// `row_start`, when given, holds each output row's source start (read in order instead of the
+// map and the source offsets).
 __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* __restrict__ src_off,
                                                                 const uint8_t* __restrict__ src_chars,
                                                                 const int32_t* __restrict__ map, int64_t n,
                                                                 const int64_t* __restrict__ dst_off,
-                                                                uint8_t* __restrict__ dst_chars)
+                                                                uint8_t* __restrict__ dst_chars,
+                                                                const int64_t* __restrict__ row_start)
 {
   __shared__ int64_t s_doff[kStrBlock / 64][65];
   __shared__ int64_t s_soff[kStrBlock / 64][64];
@@ -464,8 +561,12 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
     const int nrows    = (int)(n - base < 0 ? 0 : (n - base < 64 ? n - base : 64));
     if (lane < nrows) {
       s_doff[wv][lane] = dst_off[base + lane];
-      const int32_t m  = map[base + lane];
-      s_soff[wv][lane] = m >= 0 ? src_off[m] : 0;
+      if (row_start != nullptr) {
+        s_soff[wv][lane] = row_start[base + lane];
+      } else {
+        const int32_t m  = map[base + lane];
+        s_soff[wv][lane] = m >= 0 ? src_off[m] : 0;
+      }
     }
     if (lane == 0 && nrows > 0) s_doff[wv][nrows] = dst_off[base + nrows];
     __syncthreads();
@@ -505,7 +606,7 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
 }
 
 void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
-                    bool force_nullable, Alloc& a, ldf_column* out)
+                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map)
 {
   hipStream_t s = a.stream();
   int done      = 0;
@@ -535,12 +636,67 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
           g.dst[j] = o->data;
         }
       }
-      if (n_out > 0) {
-        ProfScope prof("gather", s);
-        hipLaunchKernelGGL(k_gather_multi, dim3(grid_for(n_out, 256, 256 * 16)), dim3(256), 0, s, g, map, n_out,
-                           cnt.get());
+      // A random map reads each source row through one packed record when that saves random reads
+      // (two or more per output row otherwise) and the output reads most source rows.
+      const int64_t n_src = src[c0]->length;
+      PackSet ps{};
+      ps.voff = -1;
+      int reads = 0, nf = 0, widths[2 * kMaxGather + 1], offs[2 * kMaxGather + 1];
+      for (int j = 0; j < nc; ++j) {
+        if (g.width[j] > 0) {
+          widths[nf++] = g.width[j];
+          reads += 1;
+        } else {
+          widths[nf++] = 8;
+          widths[nf++] = 4;
+          reads += 2;  // the offsets here, the start again for the characters
+        }
+        if (g.sval[j] != nullptr) reads += 1;
+      }
+      bool anyval = false;
+      for (int j = 0; j < nc; ++j) anyval |= g.sval[j] != nullptr;
+      if (anyval) widths[nf++] = 1;
+      ps.stride         = rec_layout(widths, nf, offs);
+      const bool packed = random_map && n_out > 0 && n_src > 0 && n_out >= n_src / 4 && reads >= 2 && ps.stride > 0;
+      Buf<unsigned char> recs;
+      std::vector<Buf<int64_t>> starts;
+      if (packed) {
+        ps.ncols = nc;
+        int f    = 0;
+        for (int j = 0; j < nc; ++j) {
+          ps.off[j] = offs[f++];
+          if (g.width[j] == 0) {
+            ps.loff[j] = offs[f++];
+            starts.emplace_back(a, n_out);
+            g.start[j] = starts.back().get();
+          }
+        }
+        if (anyval) ps.voff = offs[f++];
+        recs = Buf<unsigned char>(a, (size_t)n_src * (size_t)ps.stride);
+        hipLaunchKernelGGL(k_pack_rows, dim3(grid_for(n_src, 256, 256 * 16)), dim3(256), 0, s, g, ps, n_src,
+                           recs.get());
         LDF_HIP_TRY(hipGetLastError());
       }
+      if (n_out > 0) {
+        ProfScope prof("gather", s);
+        const dim3 grid(grid_for(n_out, 256, 256 * 16));
+        if (!packed)
+          hipLaunchKernelGGL(k_gather_multi, grid, dim3(256), 0, s, g, map, n_out, cnt.get());
+        else if (ps.stride == 8)
+          hipLaunchKernelGGL(k_gather_packed<1>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
+                             n_out, cnt.get());
+        else if (ps.stride == 16)
+          hipLaunchKernelGGL(k_gather_packed<2>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
+                             n_out, cnt.get());
+        else if (ps.stride == 32)
+          hipLaunchKernelGGL(k_gather_packed<4>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
+                             n_out, cnt.get());
+        else
+          hipLaunchKernelGGL(k_gather_packed<8>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
+                             n_out, cnt.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      recs.reset();
       // strings: offsets from the lengths, then the characters
       size_t li = lens.size();
       for (int j = nc - 1; j >= 0; --j) {
@@ -559,11 +715,12 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
           ProfScope prof("gather_str", s);
           hipLaunchKernelGGL(k_gather_str_chars, dim3(grid_for(n_out, kStrBlock, 256 * 16)), dim3(kStrBlock), 0, s,
                              v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, o->offsets,
-                             static_cast<uint8_t*>(o->data));
+                             static_cast<uint8_t*>(o->data), (const int64_t*)g.start[j]);
           LDF_HIP_TRY(hipGetLastError());
         }
       }
       lens.clear();
+      starts.clear();
       uint64_t hc[kMaxGather];
       LDF_HIP_TRY(hipMemcpyAsync(hc, cnt.get(), sizeof(hc), hipMemcpyDeviceToHost, s));
       LDF_HIP_TRY(hipStreamSynchronize(s));

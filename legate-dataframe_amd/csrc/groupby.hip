@@ -44,6 +44,7 @@
 #include "ldf_agg.hpp"
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
+#include "ldf_rec.hpp"
 
 namespace ldf {
 
@@ -627,69 +628,6 @@ struct PartBufs {
   const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
 };
 
-// A record being built in registers: eight named words (no private array, which a dynamically
-// indexed access would place in scratch memory).  `off` is the same for every lane, so put()
-// branches on it with scalar branches; values come zero-extended from their column.
-struct RecWords {
-  uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0, w7 = 0;
-  __device__ __forceinline__ void put(int off, uint64_t bits)
-  {
-    const uint64_t sv = bits << ((off & 7) * 8);
-    switch (off >> 3) {
-      case 0: w0 |= sv; break;
-      case 1: w1 |= sv; break;
-      case 2: w2 |= sv; break;
-      case 3: w3 |= sv; break;
-      case 4: w4 |= sv; break;
-      case 5: w5 |= sv; break;
-      case 6: w6 |= sv; break;
-      default: w7 |= sv; break;
-    }
-  }
-  __device__ __forceinline__ void store(unsigned char* base, int stride, int64_t i) const
-  {
-    unsigned char* d = base + i * (int64_t)stride;
-    if (stride == 8) {
-      *reinterpret_cast<uint64_t*>(d) = w0;
-      return;
-    }
-    uint4* q = reinterpret_cast<uint4*>(d);
-    q[0]     = make_uint4((uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32));
-    if (stride >= 32) q[1] = make_uint4((uint32_t)w2, (uint32_t)(w2 >> 32), (uint32_t)w3, (uint32_t)(w3 >> 32));
-    if (stride == 64) {
-      q[2] = make_uint4((uint32_t)w4, (uint32_t)(w4 >> 32), (uint32_t)w5, (uint32_t)(w5 >> 32));
-      q[3] = make_uint4((uint32_t)w6, (uint32_t)(w6 >> 32), (uint32_t)w7, (uint32_t)(w7 >> 32));
-    }
-  }
-};
-
-template <int W>
-__device__ __forceinline__ uint64_t rec_get(const uint64_t (&w)[W], int off, int width)
-{
-  const int wi = off >> 3;
-  uint64_t x   = 0;
-#pragma unroll
-  for (int k = 0; k < W; ++k) x |= w[k] & (0ull - (uint64_t)(k == wi));
-  x >>= (off & 7) * 8;
-  return width >= 8 ? x : x & ((1ull << (8 * width)) - 1);
-}
-
-template <int W>
-__device__ __forceinline__ void rec_load(const unsigned char* base, int64_t i, uint64_t (&w)[W])
-{
-  const unsigned char* d = base + i * (int64_t)(8 * W);
-  if (W == 1) {
-    w[0] = *reinterpret_cast<const uint64_t*>(d);
-    return;
-  }
-  const uint4* q = reinterpret_cast<const uint4*>(d);
-#pragma unroll
-  for (int k = 0; k < W / 2; ++k) {
-    const uint4 x = q[k];
-    w[2 * k]      = (uint64_t)x.x | ((uint64_t)x.y << 32);
-    w[2 * k + 1]  = (uint64_t)x.z | ((uint64_t)x.w << 32);
-  }
-}
 
 struct HotParts {  // per-block partials of the hot slots, index blk * H + h (FSUM: 3 words)
   uint32_t* rows;

@@ -427,7 +427,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   gather_columns(lsrc.data(), n_lhs_out, lmap.get(), n_out, jt == LDF_JOIN_FULL, a, out->cols);
   try {
     gather_columns(rsrc.data(), n_rhs_out, rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
-                   out->cols + n_lhs_out);
+                   out->cols + n_lhs_out, /*random_map=*/true);
   } catch (...) {
     for (int i = 0; i < n_lhs_out; ++i) free_column(&out->cols[i], a);
     out->ncols = 0;

@@ -275,10 +275,12 @@ uint64_t read_u64(const uint64_t* dev, hipStream_t s);
 void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out, bool force_nullable,
                    Alloc& a, ldf_column* out);
 // The same for `ncols` columns sharing one map (read once per row for up to kMaxGather columns).
+// `random_map` says the map visits source rows in no particular order (a build side's rows, a
+// permutation): the source rows may then be packed into records read once per output row.
 // On error every column this call allocated is freed.
 constexpr int kMaxGather = 8;
 void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
-                    bool force_nullable, Alloc& a, ldf_column* out);
+                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map = false);
 void alloc_column(int32_t type, int64_t n, bool nullable, int64_t string_bytes, Alloc& a,
                   ldf_column* out);
 void free_column(ldf_column* c, Alloc& a);

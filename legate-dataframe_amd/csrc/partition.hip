@@ -512,7 +512,7 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       LDF_HIP_TRY(hipGetLastError());
       std::vector<const ldf_column_view*> src(table->ncols);
       for (int i = 0; i < table->ncols; ++i) src[i] = &table->cols[i];
-      gather_columns(src.data(), table->ncols, map.get(), n, false, a, out->cols);
+      gather_columns(src.data(), table->ncols, map.get(), n, false, a, out->cols, /*random_map=*/true);
     }
 
     Buf<uint64_t> offs(a, (size_t)nparts + 1);
