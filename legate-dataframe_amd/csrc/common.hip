@@ -538,13 +538,27 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
 }
 
 // Characters of gathered strings.  Each wave owns 64 consecutive output rows, whose characters
-// form one contiguous destination range [D0, D1); lanes write it as aligned 4-byte words
-// (256 contiguous bytes per wave store), finding each byte's row by binary search over the 65
-// row offsets staged in LDS.  Words shared with a neighbouring wave's range are written with
-// byte stores, so no two waves write the same byte.
+// form one contiguous destination range [D0, D1).
+//  * Rows of at most kShortStr bytes (the usual case): each lane copies its own row into the
+//    wave's LDS window -- aligned source dwords (an aligned dword holding a byte of the string
+//    never leaves that byte's page), one LDS byte store per character -- and the wave then writes
+//    the window as aligned 16-byte stores.
+//  * A wave with a longer row: lanes fill aligned 4-byte destination words, finding each byte's
+//    row by binary search over the 65 row offsets staged in LDS.
+// Destination pieces shared with a neighbouring wave's range are written with byte stores, so no
+// two waves write the same byte.  `row_start`, when given, holds each output row's source start
+// (read in order instead of the map and the source offsets).
 constexpr int kStrBlock = 256;
-// `row_start`, when given, holds each output row's source start (read in order instead of the
-// map and the source offsets).
+constexpr int kStrWin   = 2048;  // LDS bytes per wave
+constexpr int kShortStr = 256;
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* __restrict__ src_off,
                                                                 const uint8_t* __restrict__ src_chars,
                                                                 const int32_t* __restrict__ map, int64_t n,
@@ -554,8 +568,10 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
 {
   __shared__ int64_t s_doff[kStrBlock / 64][65];
   __shared__ int64_t s_soff[kStrBlock / 64][64];
+  __shared__ __align__(16) uint8_t s_win[kStrBlock / 64][kStrWin];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t align = (int64_t)((uintptr_t)dst_chars & 3);
+  const int64_t align4  = (int64_t)((uintptr_t)dst_chars & 3);
+  const int64_t align16 = (int64_t)((uintptr_t)dst_chars & 15);
   for (int64_t bbase = (int64_t)blockIdx.x * kStrBlock; bbase < n; bbase += (int64_t)gridDim.x * kStrBlock) {
     const int64_t base = bbase + wv * 64;
     const int nrows    = (int)(n - base < 0 ? 0 : (n - base < 64 ? n - base : 64));
@@ -572,32 +588,70 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
     __syncthreads();
     if (nrows > 0) {
       const int64_t D0 = s_doff[wv][0], D1 = s_doff[wv][nrows];
-      const int64_t W0 = ((D0 + align) & ~(int64_t)3) - align;
-      for (int64_t wd = W0 + 4 * lane; wd < D1; wd += 256) {
-        const int64_t d0 = wd > D0 ? wd : D0;
-        int lo = 0, hi = nrows - 1;  // last row whose first byte is <= d0
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_doff[wv][mid] <= d0) lo = mid;
-          else hi = mid - 1;
-        }
-        uint32_t word = 0, bytes = 0;
-        int r         = lo;
+      const int64_t myb = lane < nrows ? s_doff[wv][lane] : D1;
+      const int64_t mye = lane < nrows ? s_doff[wv][lane + 1] : D1;
+      const int64_t mys = lane < nrows ? s_soff[wv][lane] : 0;
+      int64_t longest   = mye - myb;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int64_t d = wd + b;
-          if (d >= D0 && d < D1) {
-            while (d >= s_doff[wv][r + 1]) ++r;
-            word |= (uint32_t)src_chars[s_soff[wv][r] + (d - s_doff[wv][r])] << (8 * b);
-            bytes |= 1u << b;
+      for (int o = 32; o > 0; o >>= 1) {
+        const int64_t t = __shfl_xor(longest, o);
+        longest         = t > longest ? t : longest;
+      }
+      if (longest <= kShortStr) {
+        for (int64_t W = ((D0 + align16) & ~(int64_t)15) - align16; W < D1; W += kStrWin) {
+          const int64_t We = W + kStrWin;
+          const int64_t lo = myb > W ? myb : W, hi = mye < We ? mye : We;
+          if (lo < hi) {
+            const uintptr_t u0 = (uintptr_t)(src_chars + mys + (lo - myb));
+            const uintptr_t u1 = u0 + (uintptr_t)(hi - lo);
+            const int64_t off  = (lo - W) - (int64_t)u0;  // window index of source byte address u
+            for (uintptr_t u = u0 & ~(uintptr_t)3; u < u1; u += 4) {
+              const uint32_t word = *reinterpret_cast<const uint32_t*>(u);
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+                if (u + b >= u0 && u + b < u1) s_win[wv][(int64_t)(u + b) + off] = (uint8_t)(word >> (8 * b));
+            }
           }
+          wave_lds_sync();
+          const int64_t lim = We < D1 ? We : D1;
+          for (int64_t c = W + 16 * lane; c < lim; c += 16 * 64) {
+            const int64_t c0 = c > D0 ? c : D0, c1 = c + 16 < lim ? c + 16 : lim;
+            if (c0 == c && c1 == c + 16) {
+              *reinterpret_cast<uint4*>(dst_chars + c) = *reinterpret_cast<const uint4*>(&s_win[wv][c - W]);
+            } else {
+              for (int64_t d = c0; d < c1; ++d) dst_chars[d] = s_win[wv][d - W];
+            }
+          }
+          wave_lds_sync();
         }
-        if (bytes == 0xFu) {
-          *reinterpret_cast<uint32_t*>(dst_chars + wd) = word;
-        } else {
+      } else {
+        const int64_t W0 = ((D0 + align4) & ~(int64_t)3) - align4;
+        for (int64_t wd = W0 + 4 * lane; wd < D1; wd += 256) {
+          const int64_t d0 = wd > D0 ? wd : D0;
+          int lo = 0, hi = nrows - 1;  // last row whose first byte is <= d0
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_doff[wv][mid] <= d0) lo = mid;
+            else hi = mid - 1;
+          }
+          uint32_t word = 0, bytes = 0;
+          int r         = lo;
 #pragma unroll
-          for (int b = 0; b < 4; ++b)
-            if ((bytes >> b) & 1u) dst_chars[wd + b] = (uint8_t)(word >> (8 * b));
+          for (int b = 0; b < 4; ++b) {
+            const int64_t d = wd + b;
+            if (d >= D0 && d < D1) {
+              while (d >= s_doff[wv][r + 1]) ++r;
+              word |= (uint32_t)src_chars[s_soff[wv][r] + (d - s_doff[wv][r])] << (8 * b);
+              bytes |= 1u << b;
+            }
+          }
+          if (bytes == 0xFu) {
+            *reinterpret_cast<uint32_t*>(dst_chars + wd) = word;
+          } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+              if ((bytes >> b) & 1u) dst_chars[wd + b] = (uint8_t)(word >> (8 * b));
+          }
         }
       }
     }

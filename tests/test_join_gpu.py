@@ -226,3 +226,27 @@ def test_join_many_output_columns_vs_oracle(gpu, how):
     exp = join_via_oracle(lhs, rhs, ["lk"], ["rk"], how, True)
     assert got.num_rows == exp.num_rows
     assert sorted_rows(_norm(got)) == sorted_rows(_norm(exp))
+
+
+@pytest.mark.parametrize("how", ["inner", "left"])
+def test_join_string_lengths_vs_oracle(gpu, how):
+    """String payloads of every length class: empty, short (the row-per-lane copy through LDS
+    windows, rows crossing window edges), up to 256 bytes, and multi-kilobyte rows (the
+    per-word copy), on both sides (probe side in order, build side through packed records)."""
+    rng = np.random.default_rng(33)
+
+    def strs(n):
+        lens = rng.choice([0, 1, 3, 7, 17, 31, 64, 200, 256, 257, 1000, 4099], n,
+                          p=[.08, .1, .1, .1, .15, .15, .1, .08, .04, .04, .04, .02])
+        return [bytes(rng.integers(33, 127, ln, dtype=np.uint8)).decode() for ln in lens]
+
+    def t(n, p):
+        return pa.table({f"{p}k": pa.array(rng.integers(0, 500, n), pa.int64()),
+                         f"{p}s": pa.array(strs(n), mask=rng.random(n) < 0.05),
+                         f"{p}v": pa.array(rng.integers(0, 1 << 40, n), pa.int64())})
+
+    lhs, rhs = t(6000, "l"), t(2500, "r")
+    got = gpu_join(lhs, rhs, ["lk"], ["rk"], how)
+    exp = join_via_oracle(lhs, rhs, ["lk"], ["rk"], how, True)
+    assert got.num_rows == exp.num_rows
+    assert sorted_rows(_norm(got)) == sorted_rows(_norm(exp))
