@@ -402,11 +402,13 @@ struct GatherSet {
 // one per column (and per string offset).
 struct PackSet {
   int ncols;
-  int32_t off[kMaxGather];   // value, or string start
+  int32_t off[kMaxGather];   // value, or string start (low 56 bits of an 8-byte field)
   int32_t loff[kMaxGather];  // string length
-  int32_t voff;              // validity bits (bit c = column c), -1 when no column has validity
+  int32_t voff;              // validity bits (bit c = column c), -1 when no column has validity;
+                             // with a string column, the top byte of the first one's start field
   int32_t stride;
 };
+constexpr uint64_t kStartMask = (1ull << 56) - 1;
 
 __global__ __launch_bounds__(256) void k_pack_rows(GatherSet g, PackSet ps, int64_t n, unsigned char* __restrict__ rec)
 {
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
       if (i < n) {
         if (g.width[c] == 0) {
           static_cast<uint64_t*>(g.dst[c])[i] = ok ? rec_get<W>(w, ps.loff[c], 4) : 0;
-          g.start[c][i]                       = (int64_t)rec_get<W>(w, ps.off[c], 8);
+          g.start[c][i]                       = (int64_t)(rec_get<W>(w, ps.off[c], 8) & kStartMask);
         } else {
           store_width(g.dst[c], g.width[c], i, m >= 0 ? rec_get<W>(w, ps.off[c], g.width[c]) : 0);
         }
@@ -707,9 +709,13 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         }
         if (g.sval[j] != nullptr) reads += 1;
       }
-      bool anyval = false;
-      for (int j = 0; j < nc; ++j) anyval |= g.sval[j] != nullptr;
-      if (anyval) widths[nf++] = 1;
+      bool anyval = false, anystr = false;
+      for (int j = 0; j < nc; ++j) {
+        anyval |= g.sval[j] != nullptr;
+        anystr |= g.width[j] == 0;
+      }
+      // the validity bits ride in the top byte of a string start (starts stay below 2^56)
+      if (anyval && !anystr) widths[nf++] = 1;
       ps.stride         = rec_layout(widths, nf, offs);
       const bool packed = random_map && n_out > 0 && n_src > 0 && n_out >= n_src / 4 && reads >= 2 && ps.stride > 0;
       Buf<unsigned char> recs;
@@ -723,9 +729,10 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
             ps.loff[j] = offs[f++];
             starts.emplace_back(a, n_out);
             g.start[j] = starts.back().get();
+            if (anyval && ps.voff < 0) ps.voff = ps.off[j] + 7;
           }
         }
-        if (anyval) ps.voff = offs[f++];
+        if (anyval && !anystr) ps.voff = offs[f++];
         recs = Buf<unsigned char>(a, (size_t)n_src * (size_t)ps.stride);
         hipLaunchKernelGGL(k_pack_rows, dim3(grid_for(n_src, 256, 256 * 16)), dim3(256), 0, s, g, ps, n_src,
                            recs.get());

@@ -21,6 +21,7 @@
 // offsets in LDS), so one hot key with 10^5 matches does not serialise on one thread.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ldf_common.hpp"
@@ -49,9 +50,15 @@ __global__ void k_jslot_fill(JSlot* __restrict__ tab, int64_t n)
 }
 
 // ---- slot lookup ------------------------------------------------------------------------
+// Linear probing wraps inside a tile of tmask + 1 slots (the whole table when tmask = mask), so
+// a tile can be built on its own in LDS.  Probe sequences stop after tmask + 1 slots: a full tile
+// returns kTileFull from the build (the caller rebuilds with one tile) and "absent" from a probe.
+constexpr int64_t kTileFull = -2;
+__device__ __forceinline__ uint64_t tile_next(uint64_t i, uint64_t tmask) { return (i & ~tmask) | ((i + 1) & tmask); }
+
 template <bool GENERIC>
 __device__ __forceinline__ int64_t build_find_or_insert(const DKeys& bk, int64_t r, JSlot* tab, uint64_t mask,
-                                                        uint64_t cap, bool nulls_equal)
+                                                        uint64_t tmask, uint64_t cap, bool nulls_equal)
 {
   if (!GENERIC) {
     const DCol& c = bk.c[0];
@@ -59,22 +66,23 @@ __device__ __forceinline__ int64_t build_find_or_insert(const DKeys& bk, int64_t
     unsigned long long k = load_bits(c.data, c.width, r);
     if (k == kEmpty) return (int64_t)cap;
     uint64_t i = mix64(k) & mask;
-    while (true) {
+    for (uint64_t probes = 0; probes <= tmask; ++probes) {
       unsigned long long cur = atomic_load_relaxed(&tab[i].key);
       if (cur == k) return (int64_t)i;
       if (cur == kEmpty) {
         unsigned long long old = atomicCAS(&tab[i].key, kEmpty, k);
         if (old == kEmpty || old == k) return (int64_t)i;
       }
-      i = (i + 1) & mask;
+      i = tile_next(i, tmask);
     }
+    return kTileFull;
   } else {
     if (!nulls_equal && any_key_null(bk, r)) return -1;
     uint64_t h                = row_hash64(bk, r);
     uint32_t fp               = (uint32_t)(h >> 32);
     unsigned long long packed = ((unsigned long long)fp << 32) | (uint32_t)r;
     uint64_t i                = h & mask;
-    while (true) {
+    for (uint64_t probes = 0; probes <= tmask; ++probes) {
       unsigned long long cur = atomic_load_relaxed(&tab[i].key);
       if (cur == kEmpty) {
         unsigned long long old = atomicCAS(&tab[i].key, kEmpty, packed);
@@ -82,15 +90,16 @@ __device__ __forceinline__ int64_t build_find_or_insert(const DKeys& bk, int64_t
         cur = old;
       }
       if ((uint32_t)(cur >> 32) == fp && rows_equal(bk, r, bk, (int64_t)(uint32_t)cur)) return (int64_t)i;
-      i = (i + 1) & mask;
+      i = tile_next(i, tmask);
     }
+    return kTileFull;
   }
 }
 
 // Returns the slot of the lhs row's key, or -1 when no build row has that key.
 template <bool GENERIC>
 __device__ __forceinline__ int64_t probe_find(const DKeys& pk, const DKeys& bk, int64_t r, const JSlot* tab,
-                                              uint64_t mask, uint64_t cap, bool nulls_equal)
+                                              uint64_t mask, uint64_t tmask, uint64_t cap, bool nulls_equal)
 {
   if (!GENERIC) {
     const DCol& c = pk.c[0];
@@ -98,33 +107,40 @@ __device__ __forceinline__ int64_t probe_find(const DKeys& pk, const DKeys& bk, 
     unsigned long long k = load_bits(c.data, c.width, r);
     if (k == kEmpty) return (int64_t)cap;
     uint64_t i = mix64(k) & mask;
-    while (true) {
+    for (uint64_t probes = 0; probes <= tmask; ++probes) {
       unsigned long long cur = tab[i].key;
       if (cur == k) return (int64_t)i;
       if (cur == kEmpty) return -1;
-      i = (i + 1) & mask;
+      i = tile_next(i, tmask);
     }
+    return -1;
   } else {
     if (!nulls_equal && any_key_null(pk, r)) return -1;
     uint64_t h  = row_hash64(pk, r);
     uint32_t fp = (uint32_t)(h >> 32);
     uint64_t i  = h & mask;
-    while (true) {
+    for (uint64_t probes = 0; probes <= tmask; ++probes) {
       unsigned long long cur = tab[i].key;
       if (cur == kEmpty) return -1;
       if ((uint32_t)(cur >> 32) == fp && rows_equal(pk, r, bk, (int64_t)(uint32_t)cur)) return (int64_t)i;
-      i = (i + 1) & mask;
+      i = tile_next(i, tmask);
     }
+    return -1;
   }
 }
 
 // ---- build ------------------------------------------------------------------------------
 template <bool GENERIC>
 __global__ __launch_bounds__(256) void k_join_build(DKeys bk, int64_t nb, JSlot* __restrict__ tab, uint64_t mask,
-                                                    uint64_t cap, bool nulls_equal, int32_t* __restrict__ slot_of_row)
+                                                    uint64_t tmask, uint64_t cap, bool nulls_equal,
+                                                    int32_t* __restrict__ slot_of_row, int32_t* __restrict__ err)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nb; r += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s = build_find_or_insert<GENERIC>(bk, r, tab, mask, cap, nulls_equal);
+    int64_t s = build_find_or_insert<GENERIC>(bk, r, tab, mask, tmask, cap, nulls_equal);
+    if (s == kTileFull) {
+      *err = 1;
+      s    = -1;
+    }
     if (s >= 0) atomicAdd(&tab[s].cnt, 1u);
     slot_of_row[r] = (int32_t)s;
   }
@@ -153,6 +169,160 @@ __global__ void k_join_group_rows(const int32_t* __restrict__ slot_of_row, int64
   }
 }
 
+// ---- tiled build (single fixed-width key) -----------------------------------------------
+// Build rows are grouped by the tile of their key's home slot (plus two bins: the key equal to
+// the empty marker, and the null key when nulls are equal): per-block histograms, a scan, and a
+// scatter of 16-byte (key, row) records.  Then one block per tile inserts its rows' keys into an
+// LDS copy of the tile with LDS atomics, counts the rows per slot, writes the finished slots
+// {key, cnt, start} and places its rows grouped by slot -- in the same row range the grouping
+// gave the tile, which is exactly where the CSR layout wants them.
+constexpr int kBTile    = 4096;   // slots per tile
+constexpr int kBMaxBins = 32770;  // tiles + 2 special bins (a table of up to 2^27 slots)
+constexpr int kBHist    = 1024;   // threads of the histogram / scatter blocks
+
+__device__ __forceinline__ int jb_bin(const DCol& c, int64_t r, uint64_t mask, int tbits, int ntiles, bool nulls_equal,
+                                      unsigned long long& k)
+{
+  if (!valid_at(c.valid, r)) return nulls_equal ? ntiles + 1 : -1;
+  k = load_bits(c.data, c.width, r);
+  if (k == kEmpty) return ntiles;
+  return (int)((mix64(k) & mask) >> tbits);
+}
+
+__global__ __launch_bounds__(kBHist) void k_jb_hist(DCol c, int64_t nb, int64_t rpb, uint64_t mask, int tbits,
+                                                    int ntiles, bool nulls_equal, uint32_t* __restrict__ counts)
+{
+  extern __shared__ uint32_t hist[];
+  const int nbins = ntiles + 2;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const int64_t begin = (int64_t)blockIdx.x * rpb;
+  const int64_t end   = begin + rpb < nb ? begin + rpb : nb;
+  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
+    unsigned long long k;
+    const int bin = jb_bin(c, r, mask, tbits, ntiles, nulls_equal, k);
+    if (bin >= 0) atomicAdd(&hist[bin], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) counts[(int64_t)i * gridDim.x + blockIdx.x] = hist[i];
+}
+
+__global__ __launch_bounds__(kBHist) void k_jb_scatter(DCol c, int64_t nb, int64_t rpb, uint64_t mask, int tbits,
+                                                       int ntiles, bool nulls_equal, const uint64_t* __restrict__ offs,
+                                                       uint4* __restrict__ recs, int32_t* __restrict__ slot_of_row)
+{
+  extern __shared__ uint32_t cur[];
+  const int nbins = ntiles + 2;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) cur[i] = (uint32_t)offs[(int64_t)i * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int64_t begin = (int64_t)blockIdx.x * rpb;
+  const int64_t end   = begin + rpb < nb ? begin + rpb : nb;
+  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
+    unsigned long long k = 0;
+    const int bin        = jb_bin(c, r, mask, tbits, ntiles, nulls_equal, k);
+    if (bin < 0) {
+      slot_of_row[r] = -1;
+      continue;
+    }
+    const uint32_t pos = atomicAdd(&cur[bin], 1u);
+    recs[pos]          = make_uint4((uint32_t)k, (uint32_t)(k >> 32), (uint32_t)r, 0u);
+  }
+}
+
+// One block per bin: bins < ntiles are table tiles of `tile` slots; bin ntiles is slot cap (the
+// empty-marker key), bin ntiles + 1 is slot cap + 1 (the null key).
+__global__ __launch_bounds__(256) void k_jb_tiles(const uint4* __restrict__ recs, const uint64_t* __restrict__ offs,
+                                                  int nblk_hist, int ntiles, int tile, uint64_t cap,
+                                                  JSlot* __restrict__ tab, uint32_t* __restrict__ rows,
+                                                  int32_t* __restrict__ slot_of_row, int32_t* __restrict__ err)
+{
+  __shared__ unsigned long long tkeys[kBTile];
+  __shared__ uint32_t tcnt[kBTile];
+  __shared__ uint32_t tpos[kBTile];
+  __shared__ uint64_t smem[17];
+  __shared__ int bad;
+  const int t       = blockIdx.x;
+  const int64_t r0  = (int64_t)offs[(int64_t)t * nblk_hist];
+  const int64_t r1  = (int64_t)offs[(int64_t)(t + 1) * nblk_hist];
+  if (t >= ntiles) {  // a special slot: all the bin's rows share it
+    const int64_t slot = (int64_t)cap + (t - ntiles);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+      const uint32_t row = recs[i].z;
+      rows[i]            = row;
+      slot_of_row[row]   = (int32_t)slot;
+    }
+    if (threadIdx.x == 0) {
+      tab[slot].key   = kEmpty;
+      tab[slot].cnt   = (uint32_t)(r1 - r0);
+      tab[slot].start = (uint32_t)r0;
+    }
+    return;
+  }
+  const uint32_t tmask = (uint32_t)tile - 1;
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    tkeys[i] = kEmpty;
+    tcnt[i]  = 0;
+  }
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  // keys into the tile
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    const uint4 rc             = recs[i];
+    const unsigned long long k = (unsigned long long)rc.x | ((unsigned long long)rc.y << 32);
+    uint32_t j                 = (uint32_t)mix64(k) & tmask;
+    int slot                   = -1;
+    for (int probes = 0; probes < tile; ++probes) {
+      unsigned long long cur = tkeys[j];
+      if (cur == kEmpty) cur = atomicCAS(&tkeys[j], kEmpty, k);
+      if (cur == kEmpty || cur == k) {
+        slot = (int)j;
+        break;
+      }
+      j = (j + 1) & tmask;
+    }
+    if (slot < 0) bad = 1;
+    else atomicAdd(&tcnt[slot], 1u);
+  }
+  __syncthreads();
+  if (bad) {
+    if (threadIdx.x == 0) atomicOr(err, 1);
+    return;
+  }
+  // row starts per slot: exclusive scan of the counts, tile / blockDim consecutive slots per thread
+  const int per = tile / (int)blockDim.x > 0 ? tile / (int)blockDim.x : 1;
+  const int my0 = threadIdx.x * per;
+  uint64_t sum  = 0;
+  for (int q = 0; q < per; ++q)
+    if (my0 + q < tile) sum += tcnt[my0 + q];
+  uint64_t total;
+  uint64_t run = block_exclusive_scan(sum, smem, &total);
+  for (int q = 0; q < per; ++q)
+    if (my0 + q < tile) {
+      tpos[my0 + q] = (uint32_t)run;
+      run += tcnt[my0 + q];
+    }
+  __syncthreads();
+  JSlot* base = tab + (int64_t)t * tile;
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    JSlot sl;
+    sl.key   = tkeys[i];
+    sl.cnt   = tcnt[i];
+    sl.start = (uint32_t)(r0 + tpos[i]);
+    base[i]  = sl;
+  }
+  __syncthreads();
+  // rows grouped by slot
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    const uint4 rc             = recs[i];
+    const unsigned long long k = (unsigned long long)rc.x | ((unsigned long long)rc.y << 32);
+    uint32_t j                 = (uint32_t)mix64(k) & tmask;
+    while (tkeys[j] != k) j = (j + 1) & tmask;  // present: inserted above
+    const uint32_t pos = atomicAdd(&tpos[j], 1u);
+    rows[r0 + pos]     = rc.z;
+    slot_of_row[rc.z]  = (int32_t)((int64_t)t * tile + j);
+  }
+}
+
 // ---- probe ------------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t out_rows_for(uint32_t cnt, int jt)
 {
@@ -168,7 +338,7 @@ __device__ __forceinline__ uint64_t out_rows_for(uint32_t cnt, int jt)
 // info[r] = start | (cnt << 32) of the lhs row's build rows (cnt = 0: no match)
 template <bool GENERIC>
 __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int64_t np, const JSlot* __restrict__ tab,
-                                                        uint64_t mask, uint64_t cap, bool nulls_equal, int jt,
+                                                        uint64_t mask, uint64_t tmask, uint64_t cap, bool nulls_equal, int jt,
                                                         uint64_t* __restrict__ info, uint8_t* __restrict__ matched,
                                                         uint64_t* __restrict__ block_tot)
 {
@@ -179,7 +349,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
   for (int j = 0; j < kJItems; ++j) {
     int64_t r = base + j * kJBlock + threadIdx.x;
     if (r < np) {
-      int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, cap, nulls_equal);
+      int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, tmask, cap, nulls_equal);
       uint32_t cnt = 0, st = 0;
       if (s >= 0) {
         cnt = tab[s].cnt;
@@ -331,20 +501,61 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "join: build side too large for one call (2^29 rows)");
   int64_t nslot = (int64_t)cap + 2;
   Buf<JSlot> tab(a, nslot);
-  hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
-  LDF_HIP_TRY(hipGetLastError());
-
-  // build
   Buf<int32_t> slot_of_row(a, nb > 0 ? nb : 1);
   Buf<uint32_t> rows(a, nb > 0 ? nb : 1);
-  if (nb > 0) {
+  // Single fixed-width key: tiles of kBTile slots built in LDS (probing wraps inside a tile).
+  // Composite keys, tables over kBMaxBins - 2 tiles, or a tile that fills up: the global build,
+  // probing over the whole table.
+  const int tile    = (int)std::min<uint64_t>(cap, (uint64_t)kBTile);
+  const int ntiles  = (int)std::min<uint64_t>(cap / (uint64_t)tile, (uint64_t)INT32_MAX);
+  uint64_t tmask    = mask;
+  bool built        = false;
+  Buf<int32_t> err(a, 1);
+  if (!generic && nb > 0 && (int64_t)ntiles + 2 <= kBMaxBins) {
     ProfScope prof("join_build", s);
+    static bool attr = false;
+    if (!attr) {
+      LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_jb_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+      LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_jb_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+      attr = true;
+    }
+    int tbits = 0;
+    while ((1 << tbits) < tile) ++tbits;
+    const int nbins   = ntiles + 2;
+    const int nblkh   = 256;  // one 1024-thread block per CU (the histogram takes the LDS)
+    const int64_t rpb = (nb + nblkh - 1) / nblkh;
+    Buf<uint32_t> counts(a, (size_t)nbins * nblkh);
+    Buf<uint64_t> offs(a, (size_t)nbins * nblkh + 1);
+    Buf<uint4> recs(a, (size_t)nb);
+    fill_u32(reinterpret_cast<uint32_t*>(err.get()), 0, 1, s);
+    hipLaunchKernelGGL(k_jb_hist, dim3(nblkh), dim3(kBHist), nbins * 4, s, bk.c[0], nb, rpb, mask, tbits, ntiles,
+                       nulls_equal, counts.get());
+    LDF_HIP_TRY(hipGetLastError());
+    exclusive_scan_u32_to_u64(counts.get(), offs.get(), (int64_t)nbins * nblkh, a);
+    hipLaunchKernelGGL(k_jb_scatter, dim3(nblkh), dim3(kBHist), nbins * 4, s, bk.c[0], nb, rpb, mask, tbits, ntiles,
+                       nulls_equal, (const uint64_t*)offs.get(), recs.get(), slot_of_row.get());
+    LDF_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_jb_tiles, dim3(nbins), dim3(256), 0, s, (const uint4*)recs.get(), (const uint64_t*)offs.get(),
+                       nblkh, ntiles, tile, cap, tab.get(), rows.get(), slot_of_row.get(), err.get());
+    LDF_HIP_TRY(hipGetLastError());
+    int32_t herr = 0;
+    LDF_HIP_TRY(hipMemcpyAsync(&herr, err.get(), sizeof(herr), hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    built = herr == 0;
+    if (built) tmask = (uint64_t)tile - 1;
+  }
+  if (!built && nb > 0) {
+    ProfScope prof("join_build", s);
+    tmask = mask;
+    fill_u32(reinterpret_cast<uint32_t*>(err.get()), 0, 1, s);
+    hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
+    LDF_HIP_TRY(hipGetLastError());
     if (generic)
-      hipLaunchKernelGGL(k_join_build<true>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, cap,
-                         nulls_equal, slot_of_row.get());
+      hipLaunchKernelGGL(k_join_build<true>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, tmask,
+                         cap, nulls_equal, slot_of_row.get(), err.get());
     else
-      hipLaunchKernelGGL(k_join_build<false>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, cap,
-                         nulls_equal, slot_of_row.get());
+      hipLaunchKernelGGL(k_join_build<false>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bk, nb, tab.get(), mask, tmask,
+                         cap, nulls_equal, slot_of_row.get(), err.get());
     LDF_HIP_TRY(hipGetLastError());
     {
       Buf<uint32_t> cnt(a, nslot);
@@ -359,6 +570,9 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
                          cnt.get(), rows.get());
       LDF_HIP_TRY(hipGetLastError());
     }
+  } else if (nb == 0) {
+    hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
+    LDF_HIP_TRY(hipGetLastError());
   }
 
   // probe: count
@@ -375,10 +589,10 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   {
   ProfScope prof("join_count", s);
   if (generic)
-    hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
+    hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
                        nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
   else
-    hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, cap,
+    hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
                        nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
   LDF_HIP_TRY(hipGetLastError());
   }

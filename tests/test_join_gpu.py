@@ -250,3 +250,33 @@ def test_join_string_lengths_vs_oracle(gpu, how):
     exp = join_via_oracle(lhs, rhs, ["lk"], ["rk"], how, True)
     assert got.num_rows == exp.num_rows
     assert sorted_rows(_norm(got)) == sorted_rows(_norm(exp))
+
+
+def _fmix64(k: np.ndarray) -> np.ndarray:
+    """MurmurHash3's 64-bit finaliser (the join table's slot hash), in wrapping uint64."""
+    k = k.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xFF51AFD7ED558CCD)
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xC4CEB9FE1A85EC53)
+        k ^= k >> np.uint64(33)
+    return k
+
+
+@pytest.mark.parametrize("how", ["inner", "full"])
+def test_join_build_tile_overflow_falls_back(gpu, how):
+    """5000 distinct build keys chosen to share one 4096-slot tile of the 16384-slot table: the
+    tiled LDS build cannot hold them, and the join rebuilds over the whole table (same result)."""
+    rng = np.random.default_rng(5)
+    cand = rng.integers(0, 1 << 40, 200_000, dtype=np.int64)
+    tile0 = cand[((_fmix64(cand) & np.uint64(16383)) >> np.uint64(12)) == 0]
+    keys = np.unique(tile0)[:5000]
+    assert keys.size == 5000
+    rhs = pa.table({"rk": pa.array(rng.permutation(keys)), "rv": pa.array(rng.integers(0, 1000, 5000))})
+    pk = np.concatenate([rng.choice(keys, 6000), rng.integers(0, 1 << 40, 2000)])
+    lhs = pa.table({"lk": pa.array(pk), "lv": pa.array(rng.integers(0, 1000, pk.size))})
+    got = gpu_join(lhs, rhs, ["lk"], ["rk"], how)
+    exp = join_via_oracle(lhs, rhs, ["lk"], ["rk"], how, True)
+    assert got.num_rows == exp.num_rows
+    assert sorted_rows(_norm(got)) == sorted_rows(_norm(exp))
