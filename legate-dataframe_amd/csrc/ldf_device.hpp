@@ -105,6 +105,20 @@ __device__ inline uint32_t murmur3_bytes(const uint8_t* p, int64_t len, uint32_t
   return fmix32(h);
 }
 
+// a mod d for 32-bit a and a divisor fixed per launch, by multiplication (Lemire, Kaser and
+// Kurz, "Faster Remainder by Direct Computation", 2019): with m = floor((2^64 - 1) / d) + 1,
+// a mod d = high 64 bits of ((m * a mod 2^64) * d); exact for every a and every d >= 1.
+struct FastMod {
+  uint64_t m;
+  uint32_t d;
+};
+inline FastMod fastmod_make(uint32_t d) { return FastMod{~0ull / d + 1, d}; }
+__device__ __forceinline__ uint32_t fastmod(uint32_t a, const FastMod& f)
+{
+  const uint64_t low = f.m * (uint64_t)a;
+  return (uint32_t)__umul64hi(low, (uint64_t)f.d);
+}
+
 // boost-style combine used by libcudf's row hasher.
 __device__ __forceinline__ uint32_t hash_combine32(uint32_t lhs, uint32_t rhs)
 {

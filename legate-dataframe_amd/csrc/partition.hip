@@ -61,7 +61,7 @@ __global__ void k_hp_hash(HashCols hc, int64_t n, uint32_t* __restrict__ out)
 }
 
 __global__ __launch_bounds__(kHpThreads) void k_hp_hist(HashCols hc, int64_t n, int64_t rows_per_block, uint32_t nparts,
-                                                        uint32_t* __restrict__ counts)
+                                                        FastMod fm, uint32_t* __restrict__ counts)
 {
   extern __shared__ uint32_t hist[];
   for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) hist[p] = 0;
@@ -69,8 +69,39 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_hist(HashCols hc, int64_t n, 
   int64_t begin = blockIdx.x * rows_per_block;
   int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
   for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
-    uint32_t p = partition_row_hash(hc, r) % nparts;
+    uint32_t p = fastmod(partition_row_hash(hc, r), fm);
     atomicAdd(&hist[p], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) counts[(int64_t)p * gridDim.x + blockIdx.x] = hist[p];
+}
+
+// The histogram of one fixed-width non-null column hashed on itself: eight rows per thread
+// have their loads in flight together.
+template <typename T>
+__global__ __launch_bounds__(kHpThreads) void k_hp_hist_v(const T* __restrict__ src, int64_t n, uint32_t seed,
+                                                          int64_t rows_per_block, uint32_t nparts, FastMod fm,
+                                                          uint32_t* __restrict__ counts)
+{
+  extern __shared__ uint32_t hist[];
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const int64_t begin = blockIdx.x * rows_per_block;
+  const int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
+  constexpr int R     = 8;
+  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
+    T v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int64_t r = r0 + (int64_t)j * blockDim.x;
+      v[j]            = r < end ? src[r] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (r0 + (int64_t)j * blockDim.x >= end) break;
+      const uint32_t p = fastmod(hash_combine32(seed, murmur3_fixed((uint64_t)v[j], (int)sizeof(T), seed)), fm);
+      atomicAdd(&hist[p], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) counts[(int64_t)p * gridDim.x + blockIdx.x] = hist[p];
@@ -86,7 +117,7 @@ struct ScatterCols {
 // LDS layout (dynamic): cursor u64[P] | wtab u16[W][P] | total u32[P] | start u32[P] |
 //                       stage_row u16[kHpSub] | stage_part u16[kHpSub]
 __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t n, int64_t rows_per_block, uint32_t nparts,
-                                                           int pbits, const uint64_t* __restrict__ offsets,
+                                                           FastMod fm, int pbits, const uint64_t* __restrict__ offsets,
                                                            ScatterCols sc, int64_t* __restrict__ perm)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -120,7 +151,7 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t 
       const int local = wave * kHpPerWave + g * kWave + lane;
       const bool active = local < sub_n;
       uint32_t p = 0;
-      if (active) p = partition_row_hash(hc, sub + local) % nparts;
+      if (active) p = fastmod(partition_row_hash(hc, sub + local), fm);
       uint64_t same = __ballot(active);
       for (int b = 0; b < pbits; ++b) {
         uint64_t bb = __ballot(active && ((p >> b) & 1));
@@ -219,7 +250,7 @@ constexpr int kVGroups  = kVPerWave / kWave;
 
 template <typename T>
 __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restrict__ src, int64_t n, uint32_t seed,
-                                                            int64_t rows_per_block, uint32_t nparts, int pbits,
+                                                            int64_t rows_per_block, uint32_t nparts, FastMod fm, int pbits,
                                                             const uint64_t* __restrict__ offsets, T* __restrict__ dst)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -268,7 +299,7 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
       const bool active = local < sub_n;
       uint32_t p        = 0;
       if (active)
-        p = hash_combine32(seed, murmur3_fixed((uint64_t)my_val[g], (int)sizeof(T), seed)) % nparts;
+        p = fastmod(hash_combine32(seed, murmur3_fixed((uint64_t)my_val[g], (int)sizeof(T), seed)), fm);
       uint64_t same = __ballot(active);
       for (int b = 0; b < pbits; ++b) {
         uint64_t bb = __ballot(active && ((p >> b) & 1));
@@ -387,7 +418,8 @@ static void launch_scatter_v(const ldf_column_view& c, int64_t n, uint32_t seed,
     attr = true;
   }
   hipLaunchKernelGGL(k_hp_scatter_v<T>, dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)), s,
-                     static_cast<const T*>(c.data), n, seed, rpb, nparts, pbits, scan, static_cast<T*>(dst));
+                     static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
+                     static_cast<T*>(dst));
   LDF_HIP_TRY(hipGetLastError());
 }
 
@@ -458,11 +490,30 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
 
     Buf<uint32_t> counts(a, (size_t)nparts * nblk);
     Buf<uint64_t> scan(a, (size_t)nparts * nblk + 1);
+    // one fixed-width non-null column hashed on itself: the value-staged kernels
+    const int w0      = type_width(table->cols[0].type_id);
+    const bool single = direct && table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
+                        scatter_v_lds_bytes((uint32_t)nparts, w0) <= kHpVLdsMax;
+    const FastMod fm  = fastmod_make((uint32_t)nparts);
     {
-    ProfScope prof("hp_hist", s);
-    hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), nparts * sizeof(uint32_t), s, hc, n, rpb,
-                       (uint32_t)nparts, counts.get());
-    LDF_HIP_TRY(hipGetLastError());
+      ProfScope prof("hp_hist", s);
+      const size_t hl = nparts * sizeof(uint32_t);
+      const void* d0  = table->cols[0].data;
+      if (!single)
+        hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), hl, s, hc, n, rpb, (uint32_t)nparts, fm, counts.get());
+      else if (w0 == 1)
+        hipLaunchKernelGGL(k_hp_hist_v<uint8_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint8_t*)d0, n, seed, rpb,
+                           (uint32_t)nparts, fm, counts.get());
+      else if (w0 == 2)
+        hipLaunchKernelGGL(k_hp_hist_v<uint16_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint16_t*)d0, n, seed, rpb,
+                           (uint32_t)nparts, fm, counts.get());
+      else if (w0 == 4)
+        hipLaunchKernelGGL(k_hp_hist_v<uint32_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed, rpb,
+                           (uint32_t)nparts, fm, counts.get());
+      else
+        hipLaunchKernelGGL(k_hp_hist_v<uint64_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed, rpb,
+                           (uint32_t)nparts, fm, counts.get());
+      LDF_HIP_TRY(hipGetLastError());
     }
     exclusive_scan_u32_to_u64(counts.get(), scan.get(), (int64_t)nparts * nblk, a);
 
@@ -489,9 +540,6 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       lds_attr = true;
     }
     ProfScope prof("hp_scatter", s);
-    // one fixed-width non-null column hashed on itself: the value-staged kernel
-    const bool single = direct && table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
-                        scatter_v_lds_bytes((uint32_t)nparts, sc.width[0]) <= kHpVLdsMax;
     if (single) {
       const uint64_t* sp = (const uint64_t*)scan.get();
       switch (sc.width[0]) {
@@ -501,7 +549,7 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
         default: launch_scatter_v<uint64_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
       }
     } else {
-      hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, pbits,
+      hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, fm, pbits,
                          (const uint64_t*)scan.get(), sc, perm.get());
       LDF_HIP_TRY(hipGetLastError());
     }
