@@ -1311,12 +1311,15 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
     }
   }
   __syncthreads();
-  // store this block's groups in their global slots
+  // Store this block's groups in its own dense slots, after the hash table and its two special
+  // slots: every key outside the hot set lives in exactly one (partition, sub-pass) block, so the
+  // slot needs no hashing and no atomics.
+  const int64_t dbase = (int64_t)t.cap + 2 + ((int64_t)part * q.S + sub) * q.tcap;
   for (int h = threadIdx.x; h < q.tcap; h += blockDim.x) {
     const unsigned long long k = keys[h];
     if (k == kGEmpty) continue;
-    const int64_t s = gb_insert_bits(k, true, t);
-    if (s < 0) continue;
+    const int64_t s = dbase + h;
+    t.slots[s]      = k;
     const uint32_t nr = reinterpret_cast<const uint32_t*>(lds + q.rows_off)[h];
     const uint32_t fg = reinterpret_cast<const uint32_t*>(lds + q.flags_off)[h];
     for (int j = 0; j < p.nacc; ++j) {
@@ -1443,7 +1446,7 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
 // the LDS set (through `fine`, within the partition's range), each group is deduplicated in the
 // LDS set, and each new pair counts for its key in an LDS key table that holds all of the
 // partition's keys; every key's count is added to its slot at the end.
-constexpr int kColdKeySlots = 2048;
+constexpr int kColdKeySlots = 4096;
 constexpr int kColdGroups   = 64;
 
 // the (key, value) pair of partition row r when its value is valid
@@ -1455,7 +1458,9 @@ __device__ __forceinline__ bool cold_pair(const PartBufs& pb, int dcol, int dwid
   return true;
 }
 
-__global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol, int dwidth, KeyTab t,
+// The partition's keys and their dense slots (written by k_gb_part3) are loaded first; a key that
+// spilled to the global hash table is looked up there at the end.
+__global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol, int dwidth, KeyTab t, int S, int tcap,
                                                            unsigned long long* __restrict__ fine,
                                                            unsigned long long* __restrict__ dist,
                                                            int32_t* __restrict__ err)
@@ -1463,6 +1468,7 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   __shared__ unsigned long long set[kPairSet];
   __shared__ unsigned long long kkeys[kColdKeySlots];
   __shared__ uint32_t kcnt[kColdKeySlots];
+  __shared__ int32_t kslot[kColdKeySlots];
   __shared__ uint32_t gh[kColdGroups];
   __shared__ unsigned long long gcur[kColdGroups];
   __shared__ int64_t gstart[kColdGroups + 1];
@@ -1479,9 +1485,26 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
     kkeys[i] = kGEmpty;
     kcnt[i]  = 0;
+    kslot[i] = -1;
   }
   if (threadIdx.x < kColdGroups) gh[threadIdx.x] = 0;
   if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < S * tcap; q += blockDim.x) {
+    const int64_t sl           = (int64_t)t.cap + 2 + (int64_t)part * S * tcap + q;
+    const unsigned long long k = t.slots[sl];
+    if (k == kGEmpty) continue;
+    uint32_t i = (uint32_t)(mix64(k) & (kColdKeySlots - 1));
+    int tries  = 0;
+    for (; tries < kColdKeySlots; ++tries) {
+      if (atomicCAS(&kkeys[i], kGEmpty, k) == kGEmpty) {
+        kslot[i] = (int32_t)(sl - ((int64_t)t.cap + 2));
+        break;
+      }
+      i = (i + 1) & (kColdKeySlots - 1);
+    }
+    if (tries == kColdKeySlots) bad |= 4;
+  }
   __syncthreads();
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
     unsigned long long pr;
@@ -1531,7 +1554,7 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   }
   for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
     if (kcnt[i] == 0) continue;
-    const int64_t sl = gb_find_bits(kkeys[i], t);
+    const int64_t sl = kslot[i] >= 0 ? (int64_t)t.cap + 2 + kslot[i] : gb_find_bits(kkeys[i], t);
     if (sl >= 0) atomicAdd(&dist[sl], (unsigned long long)kcnt[i]);
     else bad |= 8;
   }
@@ -2122,9 +2145,45 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     if (low.acc[i].kind == ACC_DISTINCT) dacc = i;
   const int32_t dist_lds_off = dacc >= 0 ? plan.a[dacc].lds_off : -1;
 
+  // The partitioned path's plan: partition tables (k_gb_part3) sized from the accumulators' LDS
+  // bytes per slot, the partition count, and the dense slots that hold the keys outside the hot
+  // set (one region of table_slots per (partition, sub-pass) block, after the hash table).
+  PartPlan q;
+  std::memset(&q, 0, sizeof(q));
+  int64_t dense = 0;
+  if (use_v3 && n > 0) {
+    int per_slot = 8 + 4 + 4;
+    for (int i = 0; i < nacc; ++i) {
+      switch (plan.a[i].kind) {
+        case ACC_VCOUNT:
+        case ACC_DISTINCT: per_slot += 4; break;
+        case ACC_ISUM:
+        case ACC_MIN:
+        case ACC_MAX: per_slot += 8; break;
+        case ACC_FSUM: per_slot += 48; break;
+        default: break;
+      }
+    }
+    q.tcap = 256;
+    while (q.tcap * 2 * per_slot + 64 * (nacc + 3) <= 150 * 1024 && q.tcap < 8192) q.tcap *= 2;
+    // Partitions: the fewest (2^10 .. 2^12) whose keys fit one table each -- the sample's
+    // estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load -- as far as the
+    // main pass's LDS cursors allow; keys beyond that take sub-passes over a partition.
+    const int hc_bytes = dpart ? kHPairParts * 8 : 0;
+    int pbits          = kMinPartBits;
+    while (pbits < kMaxPartBits && 1.25 * (double)est / (double)(1 << pbits) > 0.45 * q.tcap &&
+           plan.lds_bytes + (1 << (pbits + 1)) * 8 + hc_bytes <= 150 * 1024)
+      ++pbits;
+    plan.part_bits             = pbits;
+    const int nparts           = 1 << pbits;
+    const double keys_per_part = 1.25 * (double)est / nparts;
+    q.S                        = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
+    dense                      = (int64_t)nparts * q.S * q.tcap;
+  }
+
   for (;;) {
-    LDF_EXPECTS(cap + 2 < (uint64_t)INT32_MAX, "groupby: too many groups");
-    const int64_t nslot = (int64_t)cap + 2;
+    LDF_EXPECTS(cap + 2 + dense < (uint64_t)INT32_MAX, "groupby: too many groups");
+    const int64_t nslot = (int64_t)cap + 2 + dense;
     Buf<unsigned long long> slots(a, nslot);
     fill_u64((uint64_t*)slots.get(), kGEmpty, nslot, s);
     // status: [0] nkeys, [1] overflow (int), [2.. 2+nacc) err flags (int), then null counts
@@ -2209,35 +2268,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         plan.dpart = 1;
         plan.dcol  = plan.a[dacc].col;
       }
-      // partition tables (k_gb_part3): LDS slots per block from the accumulators' bytes per slot
-      PartPlan q;
-      std::memset(&q, 0, sizeof(q));
-      int per_slot = 8 + 4 + 4;
-      for (int i = 0; i < nacc; ++i) {
-        switch (plan.a[i].kind) {
-          case ACC_VCOUNT:
-          case ACC_DISTINCT: per_slot += 4; break;
-          case ACC_ISUM:
-          case ACC_MIN:
-          case ACC_MAX: per_slot += 8; break;
-          case ACC_FSUM: per_slot += 48; break;
-          default: break;
-        }
-      }
-      q.tcap = 256;
-      while (q.tcap * 2 * per_slot + 64 * (nacc + 3) <= 150 * 1024 && q.tcap < 8192) q.tcap *= 2;
-      // Partitions: the fewest (2^10 .. 2^12) whose keys fit one table each -- the sample's
-      // estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load -- as far as the
-      // main pass's LDS cursors allow; keys beyond that take sub-passes over a partition.
-      const int hc_bytes = dpart ? kHPairParts * 8 : 0;
-      int pbits          = kMinPartBits;
-      while (pbits < kMaxPartBits && 1.25 * (double)est / (double)(1 << pbits) > 0.45 * q.tcap &&
-             plan.lds_bytes + (1 << (pbits + 1)) * 8 + hc_bytes <= 150 * 1024)
-        ++pbits;
-      plan.part_bits             = pbits;
-      const int nparts           = 1 << pbits;
-      const double keys_per_part = 1.25 * (double)est / nparts;
-      q.S                        = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
+      const int hc_bytes         = dpart ? kHPairParts * 8 : 0;
+      const int nparts           = 1 << plan.part_bits;
       const int stage_bytes      = nparts * 8 + hc_bytes;
       const int main_lds    = plan.lds_bytes + stage_bytes;
       int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
@@ -2412,7 +2444,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         const AccDev& d = plan.a[dacc];
         if (ncold > 0) {
           hipLaunchKernelGGL(k_gb_cold_distinct, dim3(nparts), dim3(1024), 0, s, pb, plan.dcol,
-                             plan.col[plan.dcol].width, t, hfine.get(), d.g0, d.err);
+                             plan.col[plan.dcol].width, t, q.S, q.tcap, hfine.get(), d.g0, d.err);
           LDF_HIP_TRY(hipGetLastError());
         }
         if (nhp > 0) {
