@@ -735,9 +735,8 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
 
 // Classifies every row: codes[r] = its hot index, or kCodeCold | its partition; counts
 // counts[part * nblk + blk] of the rows whose key is not hot and, with dpart, hcounts[hpart * nblk +
-// blk] of the hot rows' distinct pairs.  kHistSplit blocks share each main-pass block's rows and
-// add into zeroed counts.
-constexpr int kHistSplit = 2;
+// blk] of the hot rows' distinct pairs (one block per main-pass block; plain stores).
+constexpr int kHistSplit = 1;
 constexpr uint32_t kCodeCold = 0x8000u;
 __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
                                                          uint32_t* hcounts, uint16_t* __restrict__ codes)
@@ -787,11 +786,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
-    if (hist[i]) atomicAdd(&counts[(int64_t)i * nmain + mblk], hist[i]);
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) counts[(int64_t)i * nmain + mblk] = hist[i];
   if (p.dpart)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
-      if (hhist[i]) atomicAdd(&hcounts[(int64_t)i * nmain + mblk], hhist[i]);
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * nmain + mblk] = hhist[i];
 }
 
 // LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
@@ -1384,7 +1381,7 @@ __device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned
 __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long long* __restrict__ pairs,
                                                           const uint64_t* __restrict__ hoffsets, int nblk,
                                                           unsigned long long* __restrict__ fine, int hot_cap,
-                                                          HotList hot, unsigned long long* __restrict__ dist,
+                                                          HotList hot, uint32_t* __restrict__ part_cnt,
                                                           int32_t* __restrict__ err)
 {
   __shared__ unsigned long long set[kPairSet];
@@ -1436,9 +1433,23 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < nhot; i += blockDim.x)
-    if (cnt[i]) atomicAdd(&dist[hot.gslot[i]], (unsigned long long)cnt[i]);
+  // this pair partition's new-pair counts per hot key (summed per key by k_gb_hot_dist_sum)
+  for (int i = threadIdx.x; i < nhot; i += blockDim.x) part_cnt[(int64_t)q * hot_cap + i] = cnt[i];
   if (threadIdx.x == 0 && bad) atomicOr(err, 1);
+}
+
+// One block per hot key: its distinct count is the sum over the pair partitions.
+__global__ __launch_bounds__(256) void k_gb_hot_dist_sum(const uint32_t* __restrict__ part_cnt, int nq, int hot_cap,
+                                                         HotList hot, unsigned long long* __restrict__ dist)
+{
+  __shared__ uint64_t smem[17];
+  const int nhot = (int)min(*hot.count, (unsigned int)hot_cap);
+  for (int h = blockIdx.x; h < nhot; h += gridDim.x) {
+    uint64_t s = 0;
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) s += part_cnt[(int64_t)q * hot_cap + h];
+    const uint64_t tot = block_reduce_sum(s, smem);
+    if (threadIdx.x == 0) dist[hot.gslot[h]] += tot;
+  }
 }
 
 // One block per cold partition (whose rows hold every row of their keys): the (key, value)
@@ -1554,7 +1565,11 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   }
   for (int i = threadIdx.x; i < kColdKeySlots; i += blockDim.x) {
     if (kcnt[i] == 0) continue;
-    const int64_t sl = kslot[i] >= 0 ? (int64_t)t.cap + 2 + kslot[i] : gb_find_bits(kkeys[i], t);
+    if (kslot[i] >= 0) {  // this block is the only writer of the key's count
+      dist[(int64_t)t.cap + 2 + kslot[i]] = kcnt[i];
+      continue;
+    }
+    const int64_t sl = gb_find_bits(kkeys[i], t);  // a key that spilled to the hash table
     if (sl >= 0) atomicAdd(&dist[sl], (unsigned long long)kcnt[i]);
     else bad |= 8;
   }
@@ -2294,11 +2309,9 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           attr = true;
         }
-        fill_u32(pcounts.get(), 0, (int64_t)nparts * nblk, s);
         if (dpart) {
           hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
           hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
-          fill_u32(hcounts.get(), 0, (int64_t)kHPairParts * nblk, s);
         }
         codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
@@ -2448,8 +2461,12 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           LDF_HIP_TRY(hipGetLastError());
         }
         if (nhp > 0) {
+          Buf<uint32_t> part_cnt(a, (size_t)kHPairParts * hot_cap);
           hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, s, (const unsigned long long*)hpairs.get(),
-                             (const uint64_t*)hoffs.get(), nblk, hfine.get(), hot_cap, hot, d.g0, d.err);
+                             (const uint64_t*)hoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
+          LDF_HIP_TRY(hipGetLastError());
+          hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, s, (const uint32_t*)part_cnt.get(),
+                             kHPairParts, hot_cap, hot, d.g0);
           LDF_HIP_TRY(hipGetLastError());
         }
       }
