@@ -626,6 +626,10 @@ struct PartBufs {
   int32_t nblk;
   unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
   const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
+  // dpart with one sub-pass: k_gb_part3 appends each partition's (key << 32 | value) pairs of rows
+  // with a valid value at the start of the partition's range, and their number per partition
+  unsigned long long* cpairs;
+  uint32_t* cpair_count;
 };
 
 
@@ -1217,6 +1221,7 @@ template <int W>
 __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, PartBufs pb, KeyTab t)
 {
   extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ uint32_t s_pcur;
   const int xcd   = blockIdx.x & 7;
   const int rest  = blockIdx.x >> 3;
   const int sub   = rest % q.S;
@@ -1228,6 +1233,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
     reinterpret_cast<uint32_t*>(lds + q.rows_off)[i]  = 0;
     reinterpret_cast<uint32_t*>(lds + q.flags_off)[i] = 0;
   }
+  if (threadIdx.x == 0) s_pcur = 0;
   for (int j = 0; j < p.nacc; ++j) {
     const AccDev& a = p.a[j];
     if (q.acc_off[j] < 0) continue;
@@ -1300,6 +1306,24 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
       }
     }
     acc_part_rows<R>(p, q, lds, slot, vr, fl);
+    if (pb.cpairs != nullptr) {  // the rows' COUNT_DISTINCT pairs, appended wave by wave
+      const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const bool has = slot[j] != -2 && ((fl[j] >> p.dcol) & 1);
+        const uint64_t m = __ballot(has);
+        if (m == 0) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base    = 0;
+        if (lane == leader) base = atomicAdd(&s_pcur, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, kWave);
+        if (has) {
+          const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+          pb.cpairs[begin + base + (uint32_t)__popcll(m & below)] =
+              ((unsigned long long)(uint32_t)kr[j] << 32) | (uint32_t)pick(vr[j], p.dcol);
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if (slot[j] != -1) continue;
@@ -1308,6 +1332,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
     }
   }
   __syncthreads();
+  if (pb.cpairs != nullptr && threadIdx.x == 0) pb.cpair_count[part] = s_pcur;
   // Store this block's groups in its own dense slots, after the hash table and its two special
   // slots: every key outside the hot set lives in exactly one (partition, sub-pass) block, so the
   // slot needs no hashing and no atomics.
@@ -1460,9 +1485,14 @@ __global__ __launch_bounds__(256) void k_gb_hot_dist_sum(const uint32_t* __restr
 constexpr int kColdKeySlots = 4096;
 constexpr int kColdGroups   = 64;
 
-// the (key, value) pair of partition row r when its value is valid
+// the (key, value) pair of partition row r when its value is valid (from the appended pairs when
+// k_gb_part3 wrote them: then every index below the partition's pair count holds one)
 __device__ __forceinline__ bool cold_pair(const PartBufs& pb, int dcol, int dwidth, int64_t r, unsigned long long& pr)
 {
+  if (pb.cpairs != nullptr) {
+    pr = pb.cpairs[r];
+    return true;
+  }
   const unsigned char* d = pb.rec + r * (int64_t)pb.stride;
   if (!((d[pb.flags_off] >> dcol) & 1)) return false;
   pr = (load_bits(d + pb.key_off, pb.key_width, 0) << 32) | (uint32_t)load_bits(d + pb.val_off[dcol], dwidth, 0);
@@ -1486,7 +1516,7 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   __shared__ int bad;
   const int part   = blockIdx.x;
   const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
-  const int64_t r1 = (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
+  const int64_t r1 = pb.cpairs != nullptr ? r0 + pb.cpair_count[part] : (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
   // groups of at most 0.7 * kPairSet pairs when every row has a pair
   const int ng = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
   if (ng > kColdGroups) {  // more rows than this kernel plans for: the caller falls back
@@ -2274,7 +2304,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     Buf<uint32_t> hcounts;
     Buf<uint16_t> codes;
     Buf<uint64_t> hoffs;
-    Buf<unsigned long long> hpairs, hfine;
+    Buf<unsigned long long> hpairs, hfine, cpairs;
+    Buf<uint32_t> cpair_count;
     plan.dpart = 0;
     plan.dcol  = 0;
     if (dacc >= 0) plan.a[dacc].lds_off = (use_v3 && dpart) ? -1 : dist_lds_off;
@@ -2387,6 +2418,12 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
         pb.hpairs   = hpairs.get();
         pb.hoffsets = hoffs.get();
+        if (q.S == 1 && ncold > 0) {
+          cpairs         = Buf<unsigned long long>(a, (size_t)ncold);
+          cpair_count    = Buf<uint32_t>(a, (size_t)nparts);
+          pb.cpairs      = cpairs.get();
+          pb.cpair_count = cpair_count.get();
+        }
       }
       // 4. per-block partials of the hot slots
       HotParts hp;
