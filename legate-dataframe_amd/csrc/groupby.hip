@@ -113,6 +113,8 @@ struct GbPlan {
   int32_t dpart;  // COUNT_DISTINCT through pair partitions (partitioned path, one such accumulator)
   int32_t dcol;   // its value column (index into col)
   int32_t part_bits;  // partitioned path: 2^part_bits partitions
+  int32_t acc_end;    // LDS bytes of the hot accumulators (the hot key table follows them)
+  int32_t stage;      // k_gb_main3 packs partition records through LDS (0: in registers)
 };
 
 struct HotList {
@@ -604,6 +606,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
 // LDS table.  Each key therefore lives in exactly one place -- a hot LDS slot, a partition
 // block's LDS slot, or (only when a partition block's table is full) the global table -- and
 // the final values are stored, not added, into the global slot arrays.
+constexpr int kMain3Lds    = 160 * 1024 - 256;  // k_gb_main3 dynamic LDS limit (a CU holds 160 KB; a little for its static LDS)
 constexpr int kMinPartBits = 10;
 constexpr int kMaxPartBits = 12;
 constexpr int kSub3     = kGbThreads * kGbRowsPerThread;  // rows staged per step
@@ -749,7 +752,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
   __shared__ int s_null_hot, s_sent_hot;
   __shared__ uint32_t hhist[kHPairParts];
   const int nparts = 1 << p.part_bits;
-  uint32_t* hist   = reinterpret_cast<uint32_t*>(lds + p.rows_off);  // after the hot key table
+  uint32_t* hist   = reinterpret_cast<uint32_t*>(lds + p.lds_bytes);  // after the hot table
   const int nhot   = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
   for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
@@ -885,20 +888,34 @@ __device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds
   }
 }
 
+// a field of `width` bytes into LDS (its offset is a multiple of its width)
+__device__ __forceinline__ void lds_put(unsigned char* at, int width, uint64_t bits)
+{
+  switch (width) {
+    case 1: *at = (uint8_t)bits; break;
+    case 2: *reinterpret_cast<uint16_t*>(at) = (uint16_t)bits; break;
+    case 4: *reinterpret_cast<uint32_t*>(at) = (uint32_t)bits; break;
+    default: *reinterpret_cast<uint64_t*>(at) = bits; break;
+  }
+}
+
 // Main pass: hot rows into LDS accumulators, other rows written straight into their partition:
 // each takes the next position of its partition's region with an LDS atomic on the region
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
-// accumulator is exact and order-independent).  LDS: [0, p.lds_bytes) hot table +
-// accumulators, then cursor u64[2^part_bits] (then hcursor u64[kHPairParts] with dpart).
+// accumulator is exact and order-independent).  LDS: [0, p.acc_end) hot accumulators, then
+// cursor u64[2^part_bits], hcursor u64[kHPairParts] (dpart), and with p.stage one record-sized
+// slot per thread where a record's fields are written before it is stored as 16-byte pieces.
 template <bool DPART>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp, const uint16_t* __restrict__ codes)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  unsigned long long* cursor  = reinterpret_cast<unsigned long long*>(lds + p.lds_bytes);
+  unsigned long long* cursor  = reinterpret_cast<unsigned long long*>(lds + p.acc_end);
   const int nparts            = 1 << p.part_bits;
   unsigned long long* hcursor = cursor + nparts;  // dpart only
+  unsigned char* stage        = reinterpret_cast<unsigned char*>(hcursor + (DPART ? kHPairParts : 0)) +
+                         (size_t)threadIdx.x * (size_t)pb.stride;
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
@@ -980,14 +997,30 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         ++di;
       }
       const int64_t dest = (int64_t)atomicAdd(&cursor[part[j]], 1ull);
-      RecWords w;
-      w.put(pb.key_off, kb[j]);
-      w.put(pb.flags_off, fl);
-      if (p.ncol > 0) w.put(pb.val_off[0], v[j].b0);
-      if (p.ncol > 1) w.put(pb.val_off[1], v[j].b1);
-      if (p.ncol > 2) w.put(pb.val_off[2], v[j].b2);
-      if (p.ncol > 3) w.put(pb.val_off[3], v[j].b3);
-      w.store(pb.rec, pb.stride, dest);
+      if (p.stage) {  // fields into this thread's LDS slot, then the record as 16-byte pieces
+        lds_put(stage + pb.key_off, pb.key_width, kb[j]);
+        stage[pb.flags_off] = (uint8_t)fl;
+        if (p.ncol > 0) lds_put(stage + pb.val_off[0], p.col[0].width, v[j].b0);
+        if (p.ncol > 1) lds_put(stage + pb.val_off[1], p.col[1].width, v[j].b1);
+        if (p.ncol > 2) lds_put(stage + pb.val_off[2], p.col[2].width, v[j].b2);
+        if (p.ncol > 3) lds_put(stage + pb.val_off[3], p.col[3].width, v[j].b3);
+        unsigned char* d = pb.rec + dest * (int64_t)pb.stride;
+        if (pb.stride == 8) {
+          *reinterpret_cast<uint64_t*>(d) = *reinterpret_cast<const uint64_t*>(stage);
+        } else {
+          for (int k = 0; k < pb.stride; k += 16)
+            *reinterpret_cast<uint4*>(d + k) = *reinterpret_cast<const uint4*>(stage + k);
+        }
+      } else {
+        RecWords w;
+        w.put(pb.key_off, kb[j]);
+        w.put(pb.flags_off, fl);
+        if (p.ncol > 0) w.put(pb.val_off[0], v[j].b0);
+        if (p.ncol > 1) w.put(pb.val_off[1], v[j].b1);
+        if (p.ncol > 2) w.put(pb.val_off[2], v[j].b2);
+        if (p.ncol > 3) w.put(pb.val_off[3], v[j].b3);
+        w.store(pb.rec, pb.stride, dest);
+      }
     }
   }
   __syncthreads();
@@ -2110,8 +2143,6 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     return o;
   };
   if (hot_cap > 0) {
-    plan.tab_off   = take(2 * hot_cap * 8);
-    plan.tidx_off  = take(2 * hot_cap * 2);
     plan.rows_off  = take(hot_cap * 4);
     plan.flags_off = take(hot_cap * 4);
   }
@@ -2135,6 +2166,12 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     if (d.kind == ACC_MIN || d.kind == ACC_MAX) d.flag_bit = flag_bits++;
   }
   LDF_EXPECTS(flag_bits <= 32, "groupby: too many MIN/MAX aggregations");
+  // the hot key table last: k_gb_main3 reads row classes and allocates only the accumulators
+  plan.acc_end = off;
+  if (hot_cap > 0) {
+    plan.tab_off  = take(2 * hot_cap * 8);
+    plan.tidx_off = take(2 * hot_cap * 2);
+  }
   plan.nacc      = nacc;
   plan.ncol      = (int32_t)colmap.size();
   plan.hot_cap   = hot_cap;
@@ -2214,10 +2251,19 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     // Partitions: the fewest (2^10 .. 2^12) whose keys fit one table each -- the sample's
     // estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load -- as far as the
     // main pass's LDS cursors allow; keys beyond that take sub-passes over a partition.
+    // k_gb_main3's LDS: hot accumulators, the partition cursors, the pair cursors, and (when it
+    // fits) one partition record per thread for packing
     const int hc_bytes = dpart ? kHPairParts * 8 : 0;
-    int pbits          = kMinPartBits;
+    int rec            = keys.c[0].width + 1;
+    for (int c = 0; c < plan.ncol; ++c) rec += plan.col[c].width;
+    int stride = 8;
+    while (stride < rec) stride *= 2;
+    const int stage_b = kGbThreads * stride;
+    plan.stage        = plan.acc_end + (1 << kMinPartBits) * 8 + hc_bytes + stage_b <= kMain3Lds ? 1 : 0;
+    const int main_fixed = plan.acc_end + hc_bytes + (plan.stage ? stage_b : 0);
+    int pbits            = kMinPartBits;
     while (pbits < kMaxPartBits && 1.25 * (double)est / (double)(1 << pbits) > 0.45 * q.tcap &&
-           plan.lds_bytes + (1 << (pbits + 1)) * 8 + hc_bytes <= 150 * 1024)
+           main_fixed + (1 << (pbits + 1)) * 8 <= kMain3Lds)
       ++pbits;
     plan.part_bits             = pbits;
     const int nparts           = 1 << pbits;
@@ -2316,8 +2362,11 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       }
       const int hc_bytes         = dpart ? kHPairParts * 8 : 0;
       const int nparts           = 1 << plan.part_bits;
-      const int stage_bytes      = nparts * 8 + hc_bytes;
-      const int main_lds    = plan.lds_bytes + stage_bytes;
+      int rec_bytes              = keys.c[0].width + 1;
+      for (int c = 0; c < plan.ncol; ++c) rec_bytes += plan.col[c].width;
+      int rec_stride = 8;
+      while (rec_stride < rec_bytes) rec_stride *= 2;
+      const int main_lds = plan.acc_end + nparts * 8 + hc_bytes + (plan.stage ? kGbThreads * rec_stride : 0);
       int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
       int nblk              = 256 * per_cu;
       int64_t rpb           = (n + nblk - 1) / nblk;
@@ -2332,8 +2381,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         static bool attr = false;
         if (!attr) {
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
@@ -2346,7 +2395,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         }
         codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
-        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk * kHistSplit), dim3(kGbThreads), plan.rows_off + nparts * 4, s, keys,
+        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk * kHistSplit), dim3(kGbThreads), plan.lds_bytes + nparts * 4, s, keys,
                            n, plan, hot, pcounts.get(), hcounts.get(), codes.get());
         LDF_HIP_TRY(hipGetLastError());
       }
