@@ -1670,6 +1670,65 @@ __global__ void k_gb_sample(DKeys keys, int64_t n, int64_t S, int64_t stride, Ke
   }
 }
 
+// The same sample for a single fixed-width key, counted first in each block's LDS table (4096
+// sampled rows per block), so the global table takes one insert and one add per distinct key
+// per block instead of one per sampled row (the heaviest keys would serialise on them).
+constexpr int kSampleBlockRows = 4096;
+constexpr int kSampleSlots     = 8192;
+__global__ __launch_bounds__(1024) void k_gb_sample_lds(DKeys keys, int64_t n, int64_t S, int64_t stride, KeyTab t,
+                                                        unsigned int* cnt)
+{
+  __shared__ unsigned long long tk[kSampleSlots];
+  __shared__ uint32_t tc[kSampleSlots];
+  __shared__ uint32_t s_null, s_empty;
+  for (int i = threadIdx.x; i < kSampleSlots; i += blockDim.x) {
+    tk[i] = kGEmpty;
+    tc[i] = 0;
+  }
+  if (threadIdx.x == 0) s_null = s_empty = 0;
+  __syncthreads();
+  const DCol& c      = keys.c[0];
+  const int64_t i0   = (int64_t)blockIdx.x * kSampleBlockRows;
+  const int64_t iend = i0 + kSampleBlockRows < S ? i0 + kSampleBlockRows : S;
+  for (int64_t i = i0 + threadIdx.x; i < iend; i += blockDim.x) {
+    const int64_t r = i * stride;
+    if (r >= n) continue;
+    if (!valid_at(c.valid, r)) {
+      atomicAdd(&s_null, 1u);
+      continue;
+    }
+    const unsigned long long k = load_bits(c.data, c.width, r);
+    if (k == kGEmpty) {
+      atomicAdd(&s_empty, 1u);
+      continue;
+    }
+    uint32_t j = (uint32_t)(mix64(k) & (kSampleSlots - 1));
+    while (true) {  // at most kSampleBlockRows keys in kSampleSlots slots: never full
+      unsigned long long cur = tk[j];
+      if (cur == kGEmpty) cur = atomicCAS(&tk[j], kGEmpty, k);
+      if (cur == kGEmpty || cur == k) break;
+      j = (j + 1) & (kSampleSlots - 1);
+    }
+    atomicAdd(&tc[j], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kSampleSlots; j += blockDim.x) {
+    if (tc[j] == 0) continue;
+    const int64_t sl = gb_insert_bits(tk[j], true, t);
+    if (sl >= 0) atomicAdd(&cnt[sl], tc[j]);
+  }
+  if (threadIdx.x == 0) {
+    if (s_null) {
+      const int64_t sl = gb_insert_bits(0, false, t);
+      if (sl >= 0) atomicAdd(&cnt[sl], s_null);
+    }
+    if (s_empty) {
+      const int64_t sl = gb_insert_bits(kGEmpty, true, t);
+      if (sl >= 0) atomicAdd(&cnt[sl], s_empty);
+    }
+  }
+}
+
 // stats[0] = distinct, [1] = singletons, [2] = doubletons, [3 + c] = keys seen c times (c < 64)
 __global__ __launch_bounds__(256) void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* stats)
 {
@@ -1877,7 +1936,7 @@ __global__ __launch_bounds__(256) void k_gb_finalize(OutPlan fp, const int32_t* 
 struct FillSeg {
   uint64_t off, len, val;
 };
-constexpr int kMaxFillSeg = 40;
+constexpr int kMaxFillSeg = 64;
 struct FillPlan {
   FillSeg s[kMaxFillSeg];
   int32_t n;
@@ -2075,7 +2134,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       if (generic)
         hipLaunchKernelGGL(k_gb_sample<true>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, st, scnt.get());
       else
-        hipLaunchKernelGGL(k_gb_sample<false>, dim3(grid_for(S, 256)), dim3(256), 0, s, keys, n, S, stride, st, scnt.get());
+        hipLaunchKernelGGL(k_gb_sample_lds, dim3((unsigned)((S + kSampleBlockRows - 1) / kSampleBlockRows)), dim3(1024), 0,
+                           s, keys, n, S, stride, st, scnt.get());
       LDF_HIP_TRY(hipGetLastError());
       hipLaunchKernelGGL(k_gb_sample_stats, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, scnt.get(), (int64_t)scap + 2,
                          smisc.get() + 2);
@@ -2288,13 +2348,18 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     FillPlan fp;
     fp.n            = 0;
     uint64_t total  = 0;
-    auto seg        = [&](uint64_t len, uint64_t val) {
+    // an array of `len` words whose first `fill` words start at `val` (the dense slots of the
+    // partitioned path are written whole by k_gb_part3 and need no initial value)
+    auto seg = [&](uint64_t len, uint64_t val, uint64_t fill) {
       uint64_t o = total;
       total += len;
-      if (fp.n > 0 && fp.s[fp.n - 1].val == val && fp.s[fp.n - 1].off + fp.s[fp.n - 1].len == o) fp.s[fp.n - 1].len += len;
-      else fp.s[fp.n++] = FillSeg{o, len, val};
+      if (fp.n > 0 && fp.s[fp.n - 1].val == val && fp.s[fp.n - 1].off + fp.s[fp.n - 1].len == o && fill == len)
+        fp.s[fp.n - 1].len += fill;
+      else if (fill > 0)
+        fp.s[fp.n++] = FillSeg{o, fill, val};
       return o;
     };
+    const uint64_t hashed = cap + 2;  // hash region + the two special slots
     uint64_t pcap_total = 0;
     std::vector<uint64_t> pcap(nacc, 0);
     for (int i = 0; i < nacc; ++i) {
@@ -2303,15 +2368,15 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       uint64_t init    = 0;
       if (h.kind == ACC_MIN) init = ~0ull;
       if (h.kind == ACC_PROD) init = fl ? 0x3ff0000000000000ull : 1ull;
-      off0[i] = seg(nslot, init);
+      off0[i] = seg(nslot, init, hashed);
       if (h.kind == ACC_FSUM) {
-        off1[i] = seg(nslot, 0);
-        off2[i] = seg(nslot, 0);
+        off1[i] = seg(nslot, 0, hashed);
+        off2[i] = seg(nslot, 0, hashed);
       }
-      if (h.kind == ACC_MIN || h.kind == ACC_MAX) offf[i] = seg((nslot + 1) / 2, 0);  // u32 flags
+      if (h.kind == ACC_MIN || h.kind == ACC_MAX) offf[i] = seg((nslot + 1) / 2, 0, (nslot + 1) / 2);  // u32 flags
       if (h.kind == ACC_DISTINCT && !use_v3) {
         pcap[i] = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
-        offp[i] = seg(pcap[i], kGEmpty);
+        offp[i] = seg(pcap[i], kGEmpty, pcap[i]);
         pcap_total += pcap[i];
       }
       LDF_EXPECTS(fp.n < kMaxFillSeg, "groupby: too many accumulator segments");
