@@ -2179,8 +2179,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
   int hot_cap = 0;
   if (!generic) {
     const int budget = 124 * 1024;  // leaves room for the partition staging of k_gb_main3
+    int most         = 2048;        // LDF_GROUPBY_HOT_CAP lowers the ceiling (a tuning knob)
+    if (const char* e = std::getenv("LDF_GROUPBY_HOT_CAP")) most = std::max(64, std::min(2048, std::atoi(e)));
     for (int h = 2048; h >= 64; h >>= 1)
-      if (h * (acc_bytes + 20) + 64 <= budget) {
+      if (h <= most && h * (acc_bytes + 20) + 64 <= budget) {
         hot_cap = h;
         break;
       }
