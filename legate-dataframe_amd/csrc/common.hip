@@ -72,11 +72,13 @@ void* pool_allocate(size_t bytes, hipStream_t s)
   std::lock_guard<std::mutex> lk(P.mu);
   auto fit = P.free_blocks.find({dev, s});
   if (fit != P.free_blocks.end()) {
-    auto cit = fit->second.find(cls);
-    if (cit != fit->second.end() && !cit->second.empty()) {
+    // the smallest cached class that holds the request, up to twice its size (sizes of
+    // consecutive calls vary, and exact classes alone would keep growing the cache)
+    for (auto cit = fit->second.lower_bound(cls); cit != fit->second.end() && cit->first <= 2 * cls; ++cit) {
+      if (cit->second.empty()) continue;
       void* p = cit->second.back();
       cit->second.pop_back();
-      P.live[p] = {dev, cls};
+      P.live[p] = {dev, cit->first};
       return p;
     }
   }
