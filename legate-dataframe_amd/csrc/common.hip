@@ -492,44 +492,52 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void gather_store(const void* src, void* dst, int32_t m, int64_t i)
-{
-  static_cast<T*>(dst)[i] = m >= 0 ? static_cast<const T*>(src)[m] : T(0);
-}
+// kGatherGroups groups of 64 output rows per wave and iteration: the map loads of all groups are
+// issued, then each column's value loads for all groups, before the stores that use them, so a
+// wave keeps several random reads in flight.
+constexpr int kGatherGroups = 4;
 
 __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t* __restrict__ map, int64_t n,
                                                       unsigned long long* __restrict__ null_count)
 {
-  const int lane = threadIdx.x & 63;
+  const int lane       = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   uint64_t nulls[kMaxGather];
 #pragma unroll
   for (int c = 0; c < kMaxGather; ++c) nulls[c] = 0;
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + lane;
-    const int32_t m = i < n ? map[i] : -1;
+       base += kGatherGroups * stride) {
+    int32_t m[kGatherGroups];
+#pragma unroll
+    for (int j = 0; j < kGatherGroups; ++j) {
+      const int64_t i = base + j * stride + lane;
+      m[j]            = i < n ? map[i] : -1;
+    }
 #pragma unroll
     for (int c = 0; c < kMaxGather; ++c) {
       if (c >= g.ncols) break;
-      const bool ok = m >= 0 && valid_at(g.sval[c], m);
-      if (i < n) {
-        switch (g.width[c]) {
-          case 0:
-            static_cast<uint64_t*>(g.dst[c])[i] = ok ? (uint64_t)(g.soff[c][m + 1] - g.soff[c][m]) : 0;
-            break;
-          case 1: gather_store<uint8_t>(g.src[c], g.dst[c], m, i); break;
-          case 2: gather_store<uint16_t>(g.src[c], g.dst[c], m, i); break;
-          case 4: gather_store<uint32_t>(g.src[c], g.dst[c], m, i); break;
-          default: gather_store<uint64_t>(g.src[c], g.dst[c], m, i); break;
-        }
+      uint64_t val[kGatherGroups];
+      bool ok[kGatherGroups];
+#pragma unroll
+      for (int j = 0; j < kGatherGroups; ++j) {
+        ok[j]  = m[j] >= 0 && valid_at(g.sval[c], m[j]);
+        val[j] = 0;
+        if (g.width[c] == 0) val[j] = ok[j] ? (uint64_t)(g.soff[c][m[j] + 1] - g.soff[c][m[j]]) : 0;
+        else if (m[j] >= 0) val[j] = load_bits(g.src[c], g.width[c], m[j]);
       }
-      if (g.dval[c] != nullptr) {
-        const uint64_t b = __ballot(ok);
-        if (lane == 0) {
-          g.dval[c][base >> 6]   = b;
-          const int64_t in_range = n - base < 64 ? n - base : 64;
-          nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(b);
+#pragma unroll
+      for (int j = 0; j < kGatherGroups; ++j) {
+        const int64_t gb = base + j * stride;
+        if (gb >= n) break;  // the same for the whole wave
+        const int64_t i = gb + lane;
+        if (i < n) store_width(g.dst[c], g.width[c], i, val[j]);
+        if (g.dval[c] != nullptr) {
+          const uint64_t bal = __ballot(ok[j]);
+          if (lane == 0) {
+            g.dval[c][gb >> 6]     = bal;
+            const int64_t in_range = n - gb < 64 ? n - gb : 64;
+            nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(bal);
+          }
         }
       }
     }
@@ -544,9 +552,9 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
 // Characters of gathered strings.  Each wave owns 64 consecutive output rows, whose characters
 // form one contiguous destination range [D0, D1).
 //  * Rows of at most kShortStr bytes (the usual case): each lane copies its own row into the
-//    wave's LDS window -- aligned source dwords (an aligned dword holding a byte of the string
-//    never leaves that byte's page), one LDS byte store per character -- and the wave then writes
-//    the window as aligned 16-byte stores.
+//    wave's LDS window -- aligned 16-byte source chunks, up to four loads in flight (an aligned
+//    chunk holding a byte of the string never leaves that byte's page), one LDS byte store per
+//    character -- and the wave then writes the window as aligned 16-byte stores.
 //  * A wave with a longer row: lanes fill aligned 4-byte destination words, finding each byte's
 //    row by binary search over the 65 row offsets staged in LDS.
 // Destination pieces shared with a neighbouring wave's range are written with byte stores, so no
@@ -576,9 +584,11 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t align4  = (int64_t)((uintptr_t)dst_chars & 3);
   const int64_t align16 = (int64_t)((uintptr_t)dst_chars & 15);
-  for (int64_t bbase = (int64_t)blockIdx.x * kStrBlock; bbase < n; bbase += (int64_t)gridDim.x * kStrBlock) {
-    const int64_t base = bbase + wv * 64;
-    const int nrows    = (int)(n - base < 0 ? 0 : (n - base < 64 ? n - base : 64));
+  // Waves work independently (all LDS state is per wave): wave w of the grid takes the 64-row
+  // groups w, w + waves, ...
+  const int64_t waves = (int64_t)gridDim.x * (kStrBlock / 64);
+  for (int64_t base = ((int64_t)blockIdx.x * (kStrBlock / 64) + wv) * 64; base < n; base += waves * 64) {
+    const int nrows = (int)(n - base < 64 ? n - base : 64);
     if (lane < nrows) {
       s_doff[wv][lane] = dst_off[base + lane];
       if (row_start != nullptr) {
@@ -588,9 +598,9 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
         s_soff[wv][lane] = m >= 0 ? src_off[m] : 0;
       }
     }
-    if (lane == 0 && nrows > 0) s_doff[wv][nrows] = dst_off[base + nrows];
-    __syncthreads();
-    if (nrows > 0) {
+    if (lane == 0) s_doff[wv][nrows] = dst_off[base + nrows];
+    wave_lds_sync();
+    {
       const int64_t D0 = s_doff[wv][0], D1 = s_doff[wv][nrows];
       const int64_t myb = lane < nrows ? s_doff[wv][lane] : D1;
       const int64_t mye = lane < nrows ? s_doff[wv][lane + 1] : D1;
@@ -609,11 +619,22 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
             const uintptr_t u0 = (uintptr_t)(src_chars + mys + (lo - myb));
             const uintptr_t u1 = u0 + (uintptr_t)(hi - lo);
             const int64_t off  = (lo - W) - (int64_t)u0;  // window index of source byte address u
-            for (uintptr_t u = u0 & ~(uintptr_t)3; u < u1; u += 4) {
-              const uint32_t word = *reinterpret_cast<const uint32_t*>(u);
+            // aligned 16-byte source chunks, four loads in flight per batch
+            for (uintptr_t a = u0 & ~(uintptr_t)15; a < u1; a += 64) {
+              uint4 q[4];
 #pragma unroll
-              for (int b = 0; b < 4; ++b)
-                if (u + b >= u0 && u + b < u1) s_win[wv][(int64_t)(u + b) + off] = (uint8_t)(word >> (8 * b));
+              for (int k = 0; k < 4; ++k)
+                if (a + 16 * k < u1) q[k] = *reinterpret_cast<const uint4*>(a + 16 * k);
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                if (a + 16 * k >= u1) break;
+                const uint32_t wd[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+                for (int b = 0; b < 16; ++b) {
+                  const uintptr_t u = a + 16 * k + b;
+                  if (u >= u0 && u < u1) s_win[wv][(int64_t)u + off] = (uint8_t)(wd[b >> 2] >> (8 * (b & 3)));
+                }
+              }
             }
           }
           wave_lds_sync();
@@ -659,7 +680,7 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();  // the wave's offsets are rewritten by its next group
   }
 }
 

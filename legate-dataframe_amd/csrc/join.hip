@@ -345,19 +345,79 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
   __shared__ uint64_t smem[17];
   int64_t base = (int64_t)blockIdx.x * kJRows;
   uint64_t sum = 0;
-#pragma unroll 2
-  for (int j = 0; j < kJItems; ++j) {
-    int64_t r = base + j * kJBlock + threadIdx.x;
-    if (r < np) {
-      int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, tmask, cap, nulls_equal);
-      uint32_t cnt = 0, st = 0;
-      if (s >= 0) {
-        cnt = tab[s].cnt;
-        st  = tab[s].start;
-        if (matched != nullptr && cnt > 0) matched[s] = 1;
+  if (!GENERIC) {
+    // All kJItems keys are read, then every row's home slot (one 16-byte load: key, count and
+    // start together), before any row is resolved: the loads of a thread are all in flight at
+    // once.  Only a row whose home slot holds another key probes further.
+    const DCol& c = pk.c[0];
+    int64_t slot[kJItems];
+    unsigned long long key[kJItems];
+#pragma unroll
+    for (int j = 0; j < kJItems; ++j) {
+      const int64_t r = base + j * kJBlock + threadIdx.x;
+      slot[j]         = -1;
+      key[j]          = 0;
+      if (r < np) {
+        if (!valid_at(c.valid, r)) {
+          slot[j] = nulls_equal ? (int64_t)cap + 1 : -1;
+        } else {
+          key[j]  = load_bits(c.data, c.width, r);
+          slot[j] = key[j] == kEmpty ? (int64_t)cap : (int64_t)(mix64(key[j]) & mask);
+        }
       }
+    }
+    uint4 home[kJItems];
+#pragma unroll
+    for (int j = 0; j < kJItems; ++j) {
+      home[j] = make_uint4(0, 0, 0, 0);
+      if (slot[j] >= 0) home[j] = *reinterpret_cast<const uint4*>(&tab[slot[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < kJItems; ++j) {
+      const int64_t r = base + j * kJBlock + threadIdx.x;
+      if (r >= np) continue;
+      int64_t s    = -1;
+      uint32_t cnt = 0, st = 0;
+      if (slot[j] >= 0) {
+        const unsigned long long hk = (unsigned long long)home[j].x | ((unsigned long long)home[j].y << 32);
+        if (slot[j] >= (int64_t)cap || hk == key[j]) {  // the two special slots hold their rows whatever the key
+          s   = slot[j];
+          cnt = home[j].z;
+          st  = home[j].w;
+        } else if (hk != kEmpty) {
+          uint64_t i = tile_next((uint64_t)slot[j], tmask);
+          for (uint64_t probes = 1; probes <= tmask; ++probes) {
+            const unsigned long long cur = tab[i].key;
+            if (cur == key[j]) {
+              s   = (int64_t)i;
+              cnt = tab[i].cnt;
+              st  = tab[i].start;
+              break;
+            }
+            if (cur == kEmpty) break;
+            i = tile_next(i, tmask);
+          }
+        }
+      }
+      if (matched != nullptr && cnt > 0) matched[s] = 1;
       info[r] = (uint64_t)st | ((uint64_t)cnt << 32);
       sum += out_rows_for(cnt, jt);
+    }
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < kJItems; ++j) {
+      int64_t r = base + j * kJBlock + threadIdx.x;
+      if (r < np) {
+        int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, tmask, cap, nulls_equal);
+        uint32_t cnt = 0, st = 0;
+        if (s >= 0) {
+          cnt = tab[s].cnt;
+          st  = tab[s].start;
+          if (matched != nullptr && cnt > 0) matched[s] = 1;
+        }
+        info[r] = (uint64_t)st | ((uint64_t)cnt << 32);
+        sum += out_rows_for(cnt, jt);
+      }
     }
   }
   uint64_t t = block_reduce_sum(sum, smem);
