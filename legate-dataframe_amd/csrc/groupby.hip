@@ -626,7 +626,7 @@ struct PartBufs {
   int32_t val_off[kMaxCols];
   int32_t flags_off;
   int32_t key_width;
-  int32_t nblk;
+  int32_t nblk;  // histogram blocks: the stride of offsets / hoffsets
   unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
   const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
   // dpart with one sub-pass: k_gb_part3 appends each partition's (key << 32 | value) pairs of rows
@@ -741,11 +741,16 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
 }
 
 // Classifies every row: codes[r] = its hot index, or kCodeCold | its partition; counts
-// counts[part * nblk + blk] of the rows whose key is not hot and, with dpart, hcounts[hpart * nblk +
-// blk] of the hot rows' distinct pairs (one block per main-pass block; plain stores).
-constexpr int kHistSplit = 1;
+// counts[part * nh + hb] of the rows whose key is not hot and, with dpart, hcounts[hpart * nh + hb]
+// of the hot rows' distinct pairs, for each of the nh = gridDim.x histogram blocks (plain
+// stores).  kHistSplit histogram blocks of kHistThreads cover one main-pass block's rows, in
+// order, so the main-pass block b's rows of a partition are the scanned range that starts at
+// histogram block b * kHistSplit.  The kernel's LDS holds only the hot key table and the
+// histograms, so several blocks share a CU.
+constexpr int kHistSplit   = 3;  // 3 blocks of 512 per CU (76 VGPRs): the grid is one round
+constexpr int kHistThreads = 512;
 constexpr uint32_t kCodeCold = 0x8000u;
-__global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
+__global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
                                                          uint32_t* hcounts, uint16_t* __restrict__ codes)
 {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -757,11 +762,13 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
   for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
   hot_setup(p, lds, hot, nhot, false, true, &s_null_hot, &s_sent_hot);
-  const int mblk      = blockIdx.x / kHistSplit;  // main-pass block
-  const int64_t share = p.rows_per_block / kHistSplit;
-  const int64_t begin = (int64_t)mblk * p.rows_per_block + (int64_t)(blockIdx.x % kHistSplit) * share;
-  const int64_t end   = begin + share < n ? begin + share : n;
-  const int nmain     = (int)(gridDim.x / kHistSplit);
+  const int64_t mb0   = (int64_t)(blockIdx.x / kHistSplit) * p.rows_per_block;  // main-pass block's rows
+  const int64_t share = (p.rows_per_block + kHistSplit - 1) / kHistSplit;
+  const int sub       = (int)(blockIdx.x % kHistSplit);
+  const int64_t begin = mb0 + (int64_t)sub * share;
+  int64_t end         = mb0 + ((int64_t)(sub + 1) * share < p.rows_per_block ? (int64_t)(sub + 1) * share : p.rows_per_block);
+  end                 = end < n ? end : n;
+  const int nh        = (int)gridDim.x;
   const DCol& kc      = keys.c[0];
   const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
   constexpr int R = 8;  // rows per thread whose loads are in flight together
@@ -793,9 +800,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_hist3(DKeys keys, int64_t n, 
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) counts[(int64_t)i * nmain + mblk] = hist[i];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) counts[(int64_t)i * nh + blockIdx.x] = hist[i];
   if (p.dpart)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * nmain + mblk] = hhist[i];
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * nh + blockIdx.x] = hhist[i];
 }
 
 // LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
@@ -918,10 +925,10 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
                          (size_t)threadIdx.x * (size_t)pb.stride;
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + blockIdx.x];
+  const int64_t hb0 = (int64_t)blockIdx.x * kHistSplit;  // first histogram block of this block's rows
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + hb0];
   if (DPART)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x)
-      hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + blockIdx.x];
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + hb0];
   hot_setup(p, lds, hot, nhot, true, false, &s_null_hot, &s_sent_hot);  // rows come classified
 
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
@@ -2441,9 +2448,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       nblk                  = (int)((n + rpb - 1) / rpb);
       LDF_EXPECTS(rpb < INT32_MAX, "groupby: more than 2^31 rows per block of the main pass");
       plan.rows_per_block   = rpb;
-      // 1. histogram of the non-hot rows per (partition, block)
-      pcounts = Buf<uint32_t>(a, (size_t)nparts * nblk);
-      poffs   = Buf<uint64_t>(a, (size_t)nparts * nblk + 1);
+      // 1. histogram of the non-hot rows per (partition, histogram block)
+      const int nh = nblk * kHistSplit;
+      pcounts      = Buf<uint32_t>(a, (size_t)nparts * nh);
+      poffs        = Buf<uint64_t>(a, (size_t)nparts * nh + 1);
       {
         static bool attr = false;
         if (!attr) {
@@ -2457,21 +2465,25 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           attr = true;
         }
         if (dpart) {
-          hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
-          hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
+          hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nh);
+          hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nh + 1);
         }
         codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
-        hipLaunchKernelGGL(k_gb_hist3, dim3(nblk * kHistSplit), dim3(kGbThreads), plan.lds_bytes + nparts * 4, s, keys,
-                           n, plan, hot, pcounts.get(), hcounts.get(), codes.get());
+        GbPlan hplan    = plan;  // the hot key table at the start of LDS, the histograms after it
+        hplan.tab_off   = 0;
+        hplan.tidx_off  = 2 * plan.hot_cap * 8;
+        hplan.lds_bytes = (hplan.tidx_off + 2 * plan.hot_cap * 2 + 15) & ~15;
+        hipLaunchKernelGGL(k_gb_hist3, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n, hplan,
+                           hot, pcounts.get(), hcounts.get(), codes.get());
         LDF_HIP_TRY(hipGetLastError());
       }
-      exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nblk, a);
-      if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nblk, a);
+      exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);
+      if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nh, a);
       uint64_t totals[2] = {0, 0};
-      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)nparts * nblk, 8, hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)nparts * nh, 8, hipMemcpyDeviceToHost, s));
       if (dpart)
-        LDF_HIP_TRY(hipMemcpyAsync(&totals[1], hoffs.get() + (size_t)kHPairParts * nblk, 8, hipMemcpyDeviceToHost, s));
+        LDF_HIP_TRY(hipMemcpyAsync(&totals[1], hoffs.get() + (size_t)kHPairParts * nh, 8, hipMemcpyDeviceToHost, s));
       LDF_HIP_TRY(hipStreamSynchronize(s));
       const int64_t ncold = (int64_t)totals[0];
       const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart)
@@ -2507,7 +2519,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       PartBufs pb;
       std::memset(&pb, 0, sizeof(pb));
       pb.key_width = keys.c[0].width;
-      pb.nblk      = nblk;
+      pb.nblk      = nh;
       pb.offsets   = poffs.get();
       {  // record layout: fields widest first (each at a multiple of its width), flags byte last
         int32_t off = 0;
@@ -2616,7 +2628,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         if (nhp > 0) {
           Buf<uint32_t> part_cnt(a, (size_t)kHPairParts * hot_cap);
           hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, s, (const unsigned long long*)hpairs.get(),
-                             (const uint64_t*)hoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
+                             (const uint64_t*)hoffs.get(), nh, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
           LDF_HIP_TRY(hipGetLastError());
           hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, s, (const uint32_t*)part_cnt.get(),
                              kHPairParts, hot_cap, hot, d.g0);
