@@ -750,6 +750,13 @@ __device__ __forceinline__ int hot_lookup(const GbPlan& p, const unsigned char* 
 constexpr int kHistSplit   = 3;  // 3 blocks of 512 per CU (76 VGPRs): the grid is one round
 constexpr int kHistThreads = 512;
 constexpr uint32_t kCodeCold = 0x8000u;
+
+// Hot key table of k_gb_hist3 for keys of at most 4 bytes: one 8-byte LDS word per slot holding
+// (key << 32 | hot index), so a probe is one LDS read; 4 * hot_cap slots, at most 4096.
+constexpr int kHot32MaxSlots = 4096;
+__host__ __device__ inline int hot32_slots(int hot_cap) { return 4 * hot_cap < kHot32MaxSlots ? 4 * hot_cap : kHot32MaxSlots; }
+
+template <bool PACK32>
 __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
                                                          uint32_t* hcounts, uint16_t* __restrict__ codes)
 {
@@ -761,7 +768,29 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   const int nhot   = (int)min(*hot.count, (unsigned int)p.hot_cap);
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
   for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
-  hot_setup(p, lds, hot, nhot, false, true, &s_null_hot, &s_sent_hot);
+  const int t32             = hot32_slots(p.hot_cap);
+  unsigned long long* tab32 = reinterpret_cast<unsigned long long*>(lds);
+  if (PACK32) {
+    for (int i = threadIdx.x; i < t32; i += blockDim.x) tab32[i] = kGEmpty;
+    if (threadIdx.x == 0) {
+      s_null_hot = -1;
+      s_sent_hot = -1;
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < nhot; h += blockDim.x) {
+      const int sp = hot.special[h];
+      if (sp == 1) s_sent_hot = h;
+      else if (sp == 2) s_null_hot = h;
+      else {
+        const unsigned long long k = hot.keys[h];
+        uint32_t i                 = (uint32_t)(mix64(k) & (uint64_t)(t32 - 1));
+        while (atomicCAS(&tab32[i], kGEmpty, (k << 32) | (uint32_t)h) != kGEmpty) i = (i + 1) & (uint32_t)(t32 - 1);
+      }
+    }
+    __syncthreads();
+  } else {
+    hot_setup(p, lds, hot, nhot, false, true, &s_null_hot, &s_sent_hot);
+  }
   const int64_t mb0   = (int64_t)(blockIdx.x / kHistSplit) * p.rows_per_block;  // main-pass block's rows
   const int64_t share = (p.rows_per_block + kHistSplit - 1) / kHistSplit;
   const int sub       = (int)(blockIdx.x % kHistSplit);
@@ -771,30 +800,66 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   const int nh        = (int)gridDim.x;
   const DCol& kc      = keys.c[0];
   const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
-  constexpr int R = 8;  // rows per thread whose loads are in flight together
+  constexpr int R = PACK32 ? 4 : 8;  // rows per thread whose loads are in flight together
+  using KT = typename std::conditional<PACK32, uint32_t, uint64_t>::type;
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
-    uint64_t k[R], d[R];
+    KT k[R];
+    uint32_t d[R];  // pairs use the low 32 bits of the value
     bool kv[R], dv[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int64_t r = r0 + (int64_t)j * blockDim.x;
       kv[j]           = r < end && valid_at(kc.valid, r);
-      k[j]            = kv[j] ? load_bits(kc.data, kc.width, r) : 0;
+      k[j]            = kv[j] ? (KT)load_bits(kc.data, kc.width, r) : (KT)0;
       dv[j]           = p.dpart && r < end && valid_at(dc.valid, r);
-      d[j]            = dv[j] ? load_bits(dc.data, dc.width, r) : 0;
+      d[j]            = dv[j] ? (uint32_t)load_bits(dc.data, dc.width, r) : 0u;
+    }
+    int hh[R];
+    uint32_t cpj[R];
+    if (PACK32) {  // every row's first probe is issued before any is resolved
+      unsigned long long e[R];
+      uint32_t slot[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint64_t hk = mix64((uint64_t)k[j]);
+        slot[j]           = (uint32_t)(hk & (uint64_t)(t32 - 1));
+        cpj[j]            = cold_part(hk, p.part_bits);
+        e[j]              = tab32[slot[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        uint32_t i = slot[j];
+        hh[j]      = -1;
+        while (e[j] != kGEmpty) {
+          if ((e[j] >> 32) == k[j]) {
+            hh[j] = (int)(uint32_t)e[j];
+            break;
+          }
+          i    = (i + 1) & (uint32_t)(t32 - 1);
+          e[j] = tab32[i];
+        }
+        if (!kv[j]) hh[j] = s_null_hot;
+      }
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int64_t r = r0 + (int64_t)j * blockDim.x;
       if (r >= end) break;
-      const uint64_t hk = mix64(k[j]);
-      const int h       = hot_lookup(p, lds, k[j], hk, kv[j], s_null_hot, s_sent_hot);
-      const uint32_t cp = cold_part(hk, p.part_bits);
+      int h;
+      uint32_t cp;
+      if (PACK32) {
+        h  = hh[j];
+        cp = cpj[j];
+      } else {
+        const uint64_t hk = mix64((uint64_t)k[j]);
+        h                 = hot_lookup(p, lds, (uint64_t)k[j], hk, kv[j], s_null_hot, s_sent_hot);
+        cp                = cold_part(hk, p.part_bits);
+      }
       codes[r]          = (uint16_t)(h >= 0 ? (uint32_t)h : (kCodeCold | cp));
       if (h < 0) {
         atomicAdd(&hist[cp], 1u);
       } else if (dv[j]) {
-        const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | (uint32_t)d[j];
+        const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | d[j];
         atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
       }
     }
@@ -2455,7 +2520,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       {
         static bool attr = false;
         if (!attr) {
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
@@ -2471,11 +2537,16 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
         GbPlan hplan    = plan;  // the hot key table at the start of LDS, the histograms after it
+        const bool pack = keys.c[0].width <= 4;
         hplan.tab_off   = 0;
         hplan.tidx_off  = 2 * plan.hot_cap * 8;
-        hplan.lds_bytes = (hplan.tidx_off + 2 * plan.hot_cap * 2 + 15) & ~15;
-        hipLaunchKernelGGL(k_gb_hist3, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n, hplan,
-                           hot, pcounts.get(), hcounts.get(), codes.get());
+        hplan.lds_bytes = pack ? hot32_slots(plan.hot_cap) * 8 : (hplan.tidx_off + 2 * plan.hot_cap * 2 + 15) & ~15;
+        if (pack)
+          hipLaunchKernelGGL(k_gb_hist3<true>, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n,
+                             hplan, hot, pcounts.get(), hcounts.get(), codes.get());
+        else
+          hipLaunchKernelGGL(k_gb_hist3<false>, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n,
+                             hplan, hot, pcounts.get(), hcounts.get(), codes.get());
         LDF_HIP_TRY(hipGetLastError());
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);
