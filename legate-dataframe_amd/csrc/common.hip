@@ -585,26 +585,37 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
   const int64_t align4  = (int64_t)((uintptr_t)dst_chars & 3);
   const int64_t align16 = (int64_t)((uintptr_t)dst_chars & 15);
   // Waves work independently (all LDS state is per wave): wave w of the grid takes the 64-row
-  // groups w, w + waves, ...
-  const int64_t waves = (int64_t)gridDim.x * (kStrBlock / 64);
-  for (int64_t base = ((int64_t)blockIdx.x * (kStrBlock / 64) + wv) * 64; base < n; base += waves * 64) {
+  // groups w, w + waves, ...  The group headers are software-pipelined: while a group's
+  // characters are copied, the next group's destination offsets and source starts and the
+  // following group's map entries are already in flight.
+  const int64_t step = (int64_t)gridDim.x * kStrBlock;
+  int64_t base       = ((int64_t)blockIdx.x * (kStrBlock / 64) + wv) * 64;
+  auto map_at = [&](int64_t b) -> int32_t { return b + lane < n ? map[b + lane] : -1; };
+  auto start_of = [&](int32_t m) -> int64_t { return m >= 0 ? src_off[m] : 0; };
+  int64_t hb = 0, he = 0, hs = 0;  // this lane's destination range and source start
+  int32_t m_next = -1;             // the next group's map entry of this lane
+  if (base < n && base + lane < n) {
+    hb = dst_off[base + lane];
+    he = dst_off[base + lane + 1];
+    hs = row_start != nullptr ? row_start[base + lane] : start_of(map[base + lane]);
+  }
+  if (row_start == nullptr) m_next = map_at(base + step);
+  for (; base < n; base += step) {
     const int nrows = (int)(n - base < 64 ? n - base : 64);
-    if (lane < nrows) {
-      s_doff[wv][lane] = dst_off[base + lane];
-      if (row_start != nullptr) {
-        s_soff[wv][lane] = row_start[base + lane];
-      } else {
-        const int32_t m  = map[base + lane];
-        s_soff[wv][lane] = m >= 0 ? src_off[m] : 0;
-      }
+    const int64_t nb = base + step;
+    int64_t nhb = 0, nhe = 0, nhs = 0;
+    int32_t m_after = -1;
+    if (nb + lane < n) {
+      nhb = dst_off[nb + lane];
+      nhe = dst_off[nb + lane + 1];
+      nhs = row_start != nullptr ? row_start[nb + lane] : start_of(m_next);
     }
-    if (lane == 0) s_doff[wv][nrows] = dst_off[base + nrows];
-    wave_lds_sync();
+    if (row_start == nullptr) m_after = map_at(nb + step);
     {
-      const int64_t D0 = s_doff[wv][0], D1 = s_doff[wv][nrows];
-      const int64_t myb = lane < nrows ? s_doff[wv][lane] : D1;
-      const int64_t mye = lane < nrows ? s_doff[wv][lane + 1] : D1;
-      const int64_t mys = lane < nrows ? s_soff[wv][lane] : 0;
+      const int64_t D0 = __shfl(hb, 0), D1 = __shfl(he, nrows - 1);
+      const int64_t myb = lane < nrows ? hb : D1;
+      const int64_t mye = lane < nrows ? he : D1;
+      const int64_t mys = lane < nrows ? hs : 0;
       int64_t longest   = mye - myb;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
@@ -650,6 +661,12 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
           wave_lds_sync();
         }
       } else {
+        if (lane < nrows) {
+          s_doff[wv][lane] = myb;
+          s_soff[wv][lane] = mys;
+        }
+        if (lane == 0) s_doff[wv][nrows] = D1;
+        wave_lds_sync();
         const int64_t W0 = ((D0 + align4) & ~(int64_t)3) - align4;
         for (int64_t wd = W0 + 4 * lane; wd < D1; wd += 256) {
           const int64_t d0 = wd > D0 ? wd : D0;
@@ -680,7 +697,11 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
         }
       }
     }
-    wave_lds_sync();  // the wave's offsets are rewritten by its next group
+    wave_lds_sync();  // the wave's LDS state is rewritten by its next group
+    hb     = nhb;
+    he     = nhe;
+    hs     = nhs;
+    m_next = m_after;
   }
 }
 
