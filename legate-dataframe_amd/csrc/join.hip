@@ -231,11 +231,25 @@ __global__ __launch_bounds__(kBHist) void k_jb_scatter(DCol c, int64_t nb, int64
 
 // One block per bin: bins < ntiles are table tiles of `tile` slots; bin ntiles is slot cap (the
 // empty-marker key), bin ntiles + 1 is slot cap + 1 (the null key).
+// Probe filter of a tiled table: kFiltWords 32-bit words per tile, two bits per key in one word
+// (a blocked Bloom filter), chosen by hash bits 32 and up, which the slot index (bits below 30)
+// does not use.  A probe key whose two bits are not both set is not in the table, and the probe
+// reads one filter word (a table of 2^27 slots has a 16 MB filter, mostly cache-resident)
+// instead of a slot line of the 2 GB table.  The tile's block builds its words in LDS.
+constexpr int kFiltWords = 128;
+__device__ __forceinline__ uint32_t filt_word(uint64_t hk, int tbits, uint64_t mask)
+{
+  return (uint32_t)((((hk & mask) >> tbits) * kFiltWords) + ((hk >> 32) & (kFiltWords - 1)));
+}
+__device__ __forceinline__ uint32_t filt_bits(uint64_t hk) { return (1u << ((hk >> 40) & 31)) | (1u << ((hk >> 45) & 31)); }
+
 __global__ __launch_bounds__(256) void k_jb_tiles(const uint4* __restrict__ recs, const uint64_t* __restrict__ offs,
                                                   int nblk_hist, int ntiles, int tile, uint64_t cap,
                                                   JSlot* __restrict__ tab, uint32_t* __restrict__ rows,
-                                                  int32_t* __restrict__ slot_of_row, int32_t* __restrict__ err)
+                                                  int32_t* __restrict__ slot_of_row, uint32_t* __restrict__ filt,
+                                                  int32_t* __restrict__ err)
 {
+  __shared__ uint32_t tfilt[kFiltWords];
   __shared__ unsigned long long tkeys[kBTile];
   __shared__ uint32_t tcnt[kBTile];
   __shared__ uint32_t tpos[kBTile];
@@ -263,14 +277,17 @@ __global__ __launch_bounds__(256) void k_jb_tiles(const uint4* __restrict__ recs
     tkeys[i] = kEmpty;
     tcnt[i]  = 0;
   }
+  for (int i = threadIdx.x; i < kFiltWords; i += blockDim.x) tfilt[i] = 0;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  // keys into the tile
+  // keys into the tile (and their filter bits)
   for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     const uint4 rc             = recs[i];
     const unsigned long long k = (unsigned long long)rc.x | ((unsigned long long)rc.y << 32);
-    uint32_t j                 = (uint32_t)mix64(k) & tmask;
-    int slot                   = -1;
+    const uint64_t hk          = mix64(k);
+    atomicOr(&tfilt[(hk >> 32) & (kFiltWords - 1)], filt_bits(hk));
+    uint32_t j = (uint32_t)hk & tmask;
+    int slot   = -1;
     for (int probes = 0; probes < tile; ++probes) {
       unsigned long long cur = tkeys[j];
       if (cur == kEmpty) cur = atomicCAS(&tkeys[j], kEmpty, k);
@@ -288,6 +305,7 @@ __global__ __launch_bounds__(256) void k_jb_tiles(const uint4* __restrict__ recs
     if (threadIdx.x == 0) atomicOr(err, 1);
     return;
   }
+  for (int i = threadIdx.x; i < kFiltWords; i += blockDim.x) filt[(int64_t)t * kFiltWords + i] = tfilt[i];
   // row starts per slot: exclusive scan of the counts, tile / blockDim consecutive slots per thread
   const int per = tile / (int)blockDim.x > 0 ? tile / (int)blockDim.x : 1;
   const int my0 = threadIdx.x * per;
@@ -339,6 +357,7 @@ __device__ __forceinline__ uint64_t out_rows_for(uint32_t cnt, int jt)
 template <bool GENERIC>
 __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int64_t np, const JSlot* __restrict__ tab,
                                                         uint64_t mask, uint64_t tmask, uint64_t cap, bool nulls_equal, int jt,
+                                                        const uint32_t* __restrict__ filt, int tbits,
                                                         uint64_t* __restrict__ info, uint8_t* __restrict__ matched,
                                                         uint64_t* __restrict__ block_tot)
 {
@@ -346,25 +365,42 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
   int64_t base = (int64_t)blockIdx.x * kJRows;
   uint64_t sum = 0;
   if (!GENERIC) {
-    // All kJItems keys are read, then every row's home slot (one 16-byte load: key, count and
-    // start together), before any row is resolved: the loads of a thread are all in flight at
-    // once.  Only a row whose home slot holds another key probes further.
+    // All kJItems keys are read, then their filter words (tiled tables), then the home slots of
+    // the rows the filter passes (one 16-byte load: key, count and start together), each step's
+    // loads all in flight before any is used.  Only a row whose home slot holds another key
+    // probes further.
     const DCol& c = pk.c[0];
     int64_t slot[kJItems];
     unsigned long long key[kJItems];
+    uint32_t fw[kJItems], fb[kJItems];
 #pragma unroll
     for (int j = 0; j < kJItems; ++j) {
       const int64_t r = base + j * kJBlock + threadIdx.x;
       slot[j]         = -1;
       key[j]          = 0;
+      fw[j]           = 0;
+      fb[j]           = 0;
       if (r < np) {
         if (!valid_at(c.valid, r)) {
           slot[j] = nulls_equal ? (int64_t)cap + 1 : -1;
         } else {
-          key[j]  = load_bits(c.data, c.width, r);
-          slot[j] = key[j] == kEmpty ? (int64_t)cap : (int64_t)(mix64(key[j]) & mask);
+          key[j]            = load_bits(c.data, c.width, r);
+          const uint64_t hk = mix64(key[j]);
+          slot[j]           = key[j] == kEmpty ? (int64_t)cap : (int64_t)(hk & mask);
+          if (filt != nullptr && key[j] != kEmpty) {
+            fw[j] = filt_word(hk, tbits, mask);
+            fb[j] = filt_bits(hk);
+          }
         }
       }
+    }
+    if (filt != nullptr) {
+      uint32_t word[kJItems];
+#pragma unroll
+      for (int j = 0; j < kJItems; ++j) word[j] = fb[j] != 0 ? filt[fw[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < kJItems; ++j)
+        if (fb[j] != 0 && (word[j] & fb[j]) != fb[j]) slot[j] = -1;  // not in the table
     }
     uint4 home[kJItems];
 #pragma unroll
@@ -571,6 +607,8 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   uint64_t tmask    = mask;
   bool built        = false;
   Buf<int32_t> err(a, 1);
+  Buf<uint32_t> filt;  // probe filter of the tiled build (ftbits = log2 tile; -1: none)
+  int ftbits = -1;
   if (!generic && nb > 0 && (int64_t)ntiles + 2 <= kBMaxBins) {
     ProfScope prof("join_build", s);
     static bool attr = false;
@@ -595,14 +633,18 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     hipLaunchKernelGGL(k_jb_scatter, dim3(nblkh), dim3(kBHist), nbins * 4, s, bk.c[0], nb, rpb, mask, tbits, ntiles,
                        nulls_equal, (const uint64_t*)offs.get(), recs.get(), slot_of_row.get());
     LDF_HIP_TRY(hipGetLastError());
+    filt = Buf<uint32_t>(a, (size_t)ntiles * kFiltWords);
     hipLaunchKernelGGL(k_jb_tiles, dim3(nbins), dim3(256), 0, s, (const uint4*)recs.get(), (const uint64_t*)offs.get(),
-                       nblkh, ntiles, tile, cap, tab.get(), rows.get(), slot_of_row.get(), err.get());
+                       nblkh, ntiles, tile, cap, tab.get(), rows.get(), slot_of_row.get(), filt.get(), err.get());
     LDF_HIP_TRY(hipGetLastError());
     int32_t herr = 0;
     LDF_HIP_TRY(hipMemcpyAsync(&herr, err.get(), sizeof(herr), hipMemcpyDeviceToHost, s));
     LDF_HIP_TRY(hipStreamSynchronize(s));
     built = herr == 0;
-    if (built) tmask = (uint64_t)tile - 1;
+    if (built) {
+      tmask = (uint64_t)tile - 1;
+      ftbits = tbits;
+    }
   }
   if (!built && nb > 0) {
     ProfScope prof("join_build", s);
@@ -650,10 +692,11 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   ProfScope prof("join_count", s);
   if (generic)
     hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
-                       nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
+                       nulls_equal, (int)jt, (const uint32_t*)nullptr, 0, info.get(), matched.get(), btot.get());
   else
     hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
-                       nulls_equal, (int)jt, info.get(), matched.get(), btot.get());
+                       nulls_equal, (int)jt, ftbits >= 0 ? (const uint32_t*)filt.get() : nullptr, ftbits, info.get(),
+                       matched.get(), btot.get());
   LDF_HIP_TRY(hipGetLastError());
   }
   exclusive_scan_u64(btot.get(), boff.get(), nblk, a);
