@@ -353,12 +353,15 @@ __device__ __forceinline__ uint64_t out_rows_for(uint32_t cnt, int jt)
   }
 }
 
-// info[r] = start | (cnt << 32) of the lhs row's build rows (cnt = 0: no match)
+// info[r] = start | (cnt << 32) of the lhs row's build rows (cnt = 0: no match).  With
+// `mark` (FULL) a matched slot gets kSlotMatched in its count word, stored only by probes that
+// find it unset: they have just read that word, so the flag costs no extra random read.
+constexpr uint32_t kSlotMatched = 0x80000000u;  // counts stay below 2^31 (build side < 2^29 rows)
 template <bool GENERIC>
-__global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int64_t np, const JSlot* __restrict__ tab,
+__global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int64_t np, JSlot* __restrict__ tab,
                                                         uint64_t mask, uint64_t tmask, uint64_t cap, bool nulls_equal, int jt,
                                                         const uint32_t* __restrict__ filt, int tbits,
-                                                        uint64_t* __restrict__ info, uint8_t* __restrict__ matched,
+                                                        uint64_t* __restrict__ info, bool mark,
                                                         uint64_t* __restrict__ block_tot)
 {
   __shared__ uint64_t smem[17];
@@ -413,12 +416,12 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
       const int64_t r = base + j * kJBlock + threadIdx.x;
       if (r >= np) continue;
       int64_t s    = -1;
-      uint32_t cnt = 0, st = 0;
+      uint32_t raw = 0, st = 0;  // raw: the count word with the matched flag
       if (slot[j] >= 0) {
         const unsigned long long hk = (unsigned long long)home[j].x | ((unsigned long long)home[j].y << 32);
         if (slot[j] >= (int64_t)cap || hk == key[j]) {  // the two special slots hold their rows whatever the key
           s   = slot[j];
-          cnt = home[j].z;
+          raw = home[j].z;
           st  = home[j].w;
         } else if (hk != kEmpty) {
           uint64_t i = tile_next((uint64_t)slot[j], tmask);
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
             const unsigned long long cur = tab[i].key;
             if (cur == key[j]) {
               s   = (int64_t)i;
-              cnt = tab[i].cnt;
+              raw = tab[i].cnt;
               st  = tab[i].start;
               break;
             }
@@ -435,7 +438,8 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
           }
         }
       }
-      if (matched != nullptr && cnt > 0) matched[s] = 1;
+      const uint32_t cnt = raw & ~kSlotMatched;
+      if (mark && cnt > 0 && !(raw & kSlotMatched)) tab[s].cnt = raw | kSlotMatched;  // same value from every writer
       info[r] = (uint64_t)st | ((uint64_t)cnt << 32);
       sum += out_rows_for(cnt, jt);
     }
@@ -447,9 +451,10 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
         int64_t s    = probe_find<GENERIC>(pk, bk, r, tab, mask, tmask, cap, nulls_equal);
         uint32_t cnt = 0, st = 0;
         if (s >= 0) {
-          cnt = tab[s].cnt;
-          st  = tab[s].start;
-          if (matched != nullptr && cnt > 0) matched[s] = 1;
+          const uint32_t raw = tab[s].cnt;
+          cnt                = raw & ~kSlotMatched;
+          st                 = tab[s].start;
+          if (mark && cnt > 0 && !(raw & kSlotMatched)) tab[s].cnt = raw | kSlotMatched;
         }
         info[r] = (uint64_t)st | ((uint64_t)cnt << 32);
         sum += out_rows_for(cnt, jt);
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_emit(const uint64_t* __restric
 
 // FULL join: build rows whose key no lhs row matched (or whose key is null with !nulls_equal).
 __global__ __launch_bounds__(kJBlock) void k_join_unmatched(const int32_t* __restrict__ slot_of_row, int64_t nb,
-                                                            const uint8_t* __restrict__ matched,
+                                                            const JSlot* __restrict__ tab,
                                                             const uint64_t* __restrict__ block_off, uint64_t base_out,
                                                             uint64_t* __restrict__ block_tot, int32_t* __restrict__ lmap,
                                                             int32_t* __restrict__ rmap)
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_unmatched(const int32_t* __res
     bool u    = false;
     if (r < nb) {
       int32_t sl = slot_of_row[r];
-      u          = (sl < 0) || (matched[sl] == 0);
+      u          = (sl < 0) || !(tab[sl].cnt & kSlotMatched);
     }
     f[j] = u;
     s += u ? 1 : 0;
@@ -683,20 +688,16 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   Buf<uint64_t> info(a, np > 0 ? np : 1);
   Buf<uint64_t> btot(a, nblk);
   Buf<uint64_t> boff(a, nblk + 1);
-  Buf<uint8_t> matched;
-  if (jt == LDF_JOIN_FULL) {
-    matched = Buf<uint8_t>(a, nslot);
-    fill_u8(matched.get(), 0, nslot, s);
-  }
+  const bool mark = jt == LDF_JOIN_FULL;  // matched slots flagged in their count words
   {
   ProfScope prof("join_count", s);
   if (generic)
     hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
-                       nulls_equal, (int)jt, (const uint32_t*)nullptr, 0, info.get(), matched.get(), btot.get());
+                       nulls_equal, (int)jt, (const uint32_t*)nullptr, 0, info.get(), mark, btot.get());
   else
     hipLaunchKernelGGL(k_join_count<false>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
                        nulls_equal, (int)jt, ftbits >= 0 ? (const uint32_t*)filt.get() : nullptr, ftbits, info.get(),
-                       matched.get(), btot.get());
+                       mark, btot.get());
   LDF_HIP_TRY(hipGetLastError());
   }
   exclusive_scan_u64(btot.get(), boff.get(), nblk, a);
@@ -707,7 +708,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   if (jt == LDF_JOIN_FULL) {
     utot = Buf<uint64_t>(a, ublk);
     uoff = Buf<uint64_t>(a, ublk + 1);
-    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, matched.get(),
+    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, (const JSlot*)tab.get(),
                        (const uint64_t*)nullptr, (uint64_t)0, utot.get(), (int32_t*)nullptr, (int32_t*)nullptr);
     LDF_HIP_TRY(hipGetLastError());
     exclusive_scan_u64(utot.get(), uoff.get(), ublk, a);
@@ -728,7 +729,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     LDF_HIP_TRY(hipGetLastError());
   }
   if (n2 > 0) {
-    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, matched.get(),
+    hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, (const JSlot*)tab.get(),
                        (const uint64_t*)uoff.get(), n1, utot.get(), lmap.get(), rmap.get());
     LDF_HIP_TRY(hipGetLastError());
   }
