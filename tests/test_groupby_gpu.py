@@ -80,12 +80,14 @@ def test_groupby_c2_mix_vs_oracle(gpu, seed):
 
 
 @pytest.mark.parametrize("keys_kind", ["uniform_300k", "zipf_1M", "few_16", "nullkeys_int64", "int64_wide",
-                                       "uniform_1200k", "uniform_wide_4M"])
+                                       "uniform_1200k", "uniform_wide_4M", "int16_signed", "int8_signed"])
 def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatch):
     """Inputs of >= 1M rows take the partitioned path (hot keys in LDS, the rest hash-partitioned
     and aggregated per partition in LDS); LDF_GROUPBY_PARTITIONED=0 forces the global-table path.
     Both must equal the oracle (integers exactly, FP64 sums within 1e-12).  The key counts span
-    2^10, 2^11 and 2^12 partitions, and (uniform_wide_4M, ~2.9M keys) sub-passes per partition."""
+    2^10, 2^11 and 2^12 partitions, and (uniform_wide_4M, ~2.9M keys) sub-passes per partition;
+    1- and 2-byte signed keys (negative values zero-extend the same way in the hot key table
+    and in the rows) take the packed hot-key lookup of the row classification."""
     rng = np.random.default_rng(21)
     n = 4_500_000 if keys_kind == "uniform_wide_4M" else 1_500_000
     if keys_kind == "uniform_1200k":
@@ -98,6 +100,10 @@ def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatc
         k = _c2_like(n, 3, n_keys=1_000_000)["k"]
     elif keys_kind == "few_16":
         k = pa.array(rng.integers(0, 16, n).astype(np.int32))
+    elif keys_kind == "int16_signed":
+        k = pa.array(rng.integers(-20_000, 20_000, n).astype(np.int16))
+    elif keys_kind == "int8_signed":
+        k = pa.array(rng.integers(-128, 128, n).astype(np.int8))
     elif keys_kind == "nullkeys_int64":
         k = pa.array(rng.integers(0, 50_000, n), pa.int64(), mask=rng.random(n) < 0.02)
     else:
@@ -108,7 +114,8 @@ def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatc
                   "u": pa.array(rng.integers(0, 10_000, n).astype(np.int32), mask=rng.random(n) < 0.01)})
     aggs = [("f", "sum", "fs"), ("f", "mean", "fm"), ("i", "sum", "is"), ("i", "min", "imin"), ("i", "max", "imax"),
             ("i", "count", "ic"), ("f", "max", "fmax"), (None, "count_all", "n")]
-    if keys_kind in ("uniform_300k", "zipf_1M", "few_16", "uniform_1200k", "uniform_wide_4M"):
+    if keys_kind in ("uniform_300k", "zipf_1M", "few_16", "uniform_1200k", "uniform_wide_4M", "int16_signed",
+                     "int8_signed"):
         aggs.append(("u", "count_distinct", "nu"))
     got = gpu_groupby(t, ["k"], aggs)
     monkeypatch.setenv("LDF_GROUPBY_PARTITIONED", "0")
