@@ -4,6 +4,8 @@
 // error / version entry points of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -208,11 +210,37 @@ void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s)
   LDF_HIP_TRY(hipGetLastError());
 }
 
+void* pinned_stage(size_t bytes)
+{
+  struct Stage {
+    void* p    = nullptr;
+    size_t cap = 0;
+  };
+  thread_local Stage st;  // kept for the thread's lifetime
+  if (bytes > st.cap) {
+    size_t cap = st.cap > 0 ? st.cap : 4096;
+    while (cap < bytes) cap *= 2;
+    void* p = nullptr;
+    LDF_HIP_TRY(hipHostMalloc(&p, cap, hipHostMallocDefault));
+    if (st.p != nullptr) (void)hipHostFree(st.p);
+    st.p   = p;
+    st.cap = cap;
+  }
+  return st.p;
+}
+
+void copy_to_host(void* dst, const void* dev, size_t bytes, hipStream_t s)
+{
+  void* stage = pinned_stage(bytes);
+  LDF_HIP_TRY(hipMemcpyAsync(stage, dev, bytes, hipMemcpyDeviceToHost, s));
+  LDF_HIP_TRY(hipStreamSynchronize(s));
+  std::memcpy(dst, stage, bytes);
+}
+
 uint64_t read_u64(const uint64_t* dev, hipStream_t s)
 {
   uint64_t h = 0;
-  LDF_HIP_TRY(hipMemcpyAsync(&h, dev, sizeof(h), hipMemcpyDeviceToHost, s));
-  LDF_HIP_TRY(hipStreamSynchronize(s));
+  copy_to_host(&h, dev, sizeof(h), s);
   return h;
 }
 
@@ -827,8 +855,7 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
       lens.clear();
       starts.clear();
       uint64_t hc[kMaxGather];
-      LDF_HIP_TRY(hipMemcpyAsync(hc, cnt.get(), sizeof(hc), hipMemcpyDeviceToHost, s));
-      LDF_HIP_TRY(hipStreamSynchronize(s));
+      copy_to_host(hc, cnt.get(), sizeof(hc), s);
       for (int j = 0; j < nc; ++j) out[c0 + j].null_count = out[c0 + j].validity ? (int64_t)hc[j] : 0;
     }
   } catch (...) {

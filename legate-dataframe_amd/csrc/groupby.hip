@@ -2215,8 +2215,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     }
   }
   unsigned long long stats[3 + 64] = {};
-  LDF_HIP_TRY(hipMemcpyAsync(stats, smisc.get() + 2, sizeof(stats), hipMemcpyDeviceToHost, s));
-  LDF_HIP_TRY(hipStreamSynchronize(s));
+  copy_to_host(stats, smisc.get() + 2, sizeof(stats), s);
   uint64_t est;
   {
     double d = (double)stats[0], f1 = (double)stats[1];
@@ -2552,10 +2551,15 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);
       if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nh, a);
       uint64_t totals[2] = {0, 0};
-      LDF_HIP_TRY(hipMemcpyAsync(&totals[0], poffs.get() + (size_t)nparts * nh, 8, hipMemcpyDeviceToHost, s));
-      if (dpart)
-        LDF_HIP_TRY(hipMemcpyAsync(&totals[1], hoffs.get() + (size_t)kHPairParts * nh, 8, hipMemcpyDeviceToHost, s));
-      LDF_HIP_TRY(hipStreamSynchronize(s));
+      {  // both totals queued into the pinned stage, one synchronisation
+        uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
+        st[1]        = 0;
+        LDF_HIP_TRY(hipMemcpyAsync(st, poffs.get() + (size_t)nparts * nh, 8, hipMemcpyDeviceToHost, s));
+        if (dpart) LDF_HIP_TRY(hipMemcpyAsync(st + 1, hoffs.get() + (size_t)kHPairParts * nh, 8, hipMemcpyDeviceToHost, s));
+        LDF_HIP_TRY(hipStreamSynchronize(s));
+        totals[0] = st[0];
+        totals[1] = st[1];
+      }
       const int64_t ncold = (int64_t)totals[0];
       const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart)
       {  // LDF_GROUPBY_TRACE=1: one line per call with the partitioned path's plan
@@ -2727,8 +2731,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     exclusive_scan_u32_to_u64(occ.get(), pos.get(), nslot, a);
     LDF_HIP_TRY(hipMemcpyAsync(status.get() + nstatus - 1, pos.get() + nslot, 8, hipMemcpyDeviceToDevice, s));
     std::vector<unsigned long long> hs(nstatus);
-    LDF_HIP_TRY(hipMemcpyAsync(hs.data(), status.get(), nstatus * 8, hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipStreamSynchronize(s));
+    copy_to_host(hs.data(), status.get(), nstatus * 8, s);
     if (use_v3 && dpart && dacc >= 0 && (int32_t)hs[2 + dacc]) {  // a dedupe table overflowed
       dpart = false;
       continue;
@@ -2845,8 +2848,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       }
       flush_plan();
       std::vector<unsigned long long> nc(nkeys + naggs);
-      LDF_HIP_TRY(hipMemcpyAsync(nc.data(), nullcnt, nc.size() * 8, hipMemcpyDeviceToHost, s));
-      LDF_HIP_TRY(hipStreamSynchronize(s));
+      copy_to_host(nc.data(), nullcnt, nc.size() * 8, s);
       for (int i = generic ? nkeys : 0; i < nkeys + naggs; ++i)
         out->cols[i].null_count = out->cols[i].validity ? (int64_t)nc[i] : 0;
     } catch (...) {

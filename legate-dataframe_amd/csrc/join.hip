@@ -643,8 +643,7 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
                        nblkh, ntiles, tile, cap, tab.get(), rows.get(), slot_of_row.get(), filt.get(), err.get());
     LDF_HIP_TRY(hipGetLastError());
     int32_t herr = 0;
-    LDF_HIP_TRY(hipMemcpyAsync(&herr, err.get(), sizeof(herr), hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipStreamSynchronize(s));
+    copy_to_host(&herr, err.get(), sizeof(herr), s);
     built = herr == 0;
     if (built) {
       tmask = (uint64_t)tile - 1;

@@ -268,6 +268,12 @@ void exclusive_scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, int64_t n, 
 void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
+// Small device-to-host readbacks go through a per-thread pinned buffer (grown on demand and kept
+// for the thread's lifetime): copies into pinned memory skip the runtime's staging of pageable
+// memory, so several can be queued before one synchronisation.  The buffer holds one readback
+// at a time: consume it before the next call that uses it.
+void* pinned_stage(size_t bytes);
+void copy_to_host(void* dst, const void* dev, size_t bytes, hipStream_t s);  // then synchronises s
 uint64_t read_u64(const uint64_t* dev, hipStream_t s);
 
 // Gather `src` rows by `map` (int32 row ids; a negative id produces a null row).

@@ -568,8 +568,7 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
                        (uint32_t)nparts, (uint32_t)nblk, offs.get());
     LDF_HIP_TRY(hipGetLastError());
     std::vector<uint64_t> h(nparts + 1);
-    LDF_HIP_TRY(hipMemcpyAsync(h.data(), offs.get(), (nparts + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipStreamSynchronize(s));
+    copy_to_host(h.data(), offs.get(), (nparts + 1) * sizeof(uint64_t), s);
     for (int p = 0; p <= nparts; ++p) offsets[p] = (int64_t)h[p];
   });
 }
