@@ -734,7 +734,8 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
 }
 
 void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
-                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map)
+                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map,
+                    unsigned long long* null_counts)
 {
   hipStream_t s = a.stream();
   int done      = 0;
@@ -744,8 +745,13 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
       const int nc = ncols - c0 < kMaxGather ? ncols - c0 : kMaxGather;
       GatherSet g{};
       g.ncols = nc;
-      Buf<unsigned long long> cnt(a, kMaxGather);
-      fill_u64(reinterpret_cast<uint64_t*>(cnt.get()), 0, kMaxGather, s);
+      Buf<unsigned long long> own;  // this batch's null counts, unless the caller collects them
+      unsigned long long* cnt_p = null_counts != nullptr ? null_counts + c0 : nullptr;
+      if (cnt_p == nullptr) {
+        own   = Buf<unsigned long long>(a, kMaxGather);
+        cnt_p = own.get();
+        fill_u64(reinterpret_cast<uint64_t*>(cnt_p), 0, kMaxGather, s);
+      }
       for (int j = 0; j < nc; ++j) {
         const ldf_column_view& v = *src[c0 + j];
         ldf_column* o            = &out[c0 + j];
@@ -814,19 +820,19 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         ProfScope prof("gather", s);
         const dim3 grid(grid_for(n_out, 256, 256 * 16));
         if (!packed)
-          hipLaunchKernelGGL(k_gather_multi, grid, dim3(256), 0, s, g, map, n_out, cnt.get());
+          hipLaunchKernelGGL(k_gather_multi, grid, dim3(256), 0, s, g, map, n_out, cnt_p);
         else if (ps.stride == 8)
           hipLaunchKernelGGL(k_gather_packed<1>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
-                             n_out, cnt.get());
+                             n_out, cnt_p);
         else if (ps.stride == 16)
           hipLaunchKernelGGL(k_gather_packed<2>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
-                             n_out, cnt.get());
+                             n_out, cnt_p);
         else if (ps.stride == 32)
           hipLaunchKernelGGL(k_gather_packed<4>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
-                             n_out, cnt.get());
+                             n_out, cnt_p);
         else
           hipLaunchKernelGGL(k_gather_packed<8>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
-                             n_out, cnt.get());
+                             n_out, cnt_p);
         LDF_HIP_TRY(hipGetLastError());
       }
       recs.reset();
@@ -854,8 +860,9 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
       }
       lens.clear();
       starts.clear();
+      if (null_counts != nullptr) continue;  // the caller reads them back
       uint64_t hc[kMaxGather];
-      copy_to_host(hc, cnt.get(), sizeof(hc), s);
+      copy_to_host(hc, cnt_p, sizeof(hc), s);
       for (int j = 0; j < nc; ++j) out[c0 + j].null_count = out[c0 + j].validity ? (int64_t)hc[j] : 0;
     }
   } catch (...) {

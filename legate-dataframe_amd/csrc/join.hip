@@ -614,7 +614,10 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   Buf<int32_t> err(a, 1);
   Buf<uint32_t> filt;  // probe filter of the tiled build (ftbits = log2 tile; -1: none)
   int ftbits = -1;
+  bool tiled = false;
   if (!generic && nb > 0 && (int64_t)ntiles + 2 <= kBMaxBins) {
+    if (jt == LDF_JOIN_FULL)  // rows of a tile that fills keep -1: the unmatched pass stays in bounds
+      fill_u32(reinterpret_cast<uint32_t*>(slot_of_row.get()), 0xFFFFFFFFu, nb, s);
     ProfScope prof("join_build", s);
     static bool attr = false;
     if (!attr) {
@@ -642,15 +645,14 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     hipLaunchKernelGGL(k_jb_tiles, dim3(nbins), dim3(256), 0, s, (const uint4*)recs.get(), (const uint64_t*)offs.get(),
                        nblkh, ntiles, tile, cap, tab.get(), rows.get(), slot_of_row.get(), filt.get(), err.get());
     LDF_HIP_TRY(hipGetLastError());
-    int32_t herr = 0;
-    copy_to_host(&herr, err.get(), sizeof(herr), s);
-    built = herr == 0;
-    if (built) {
-      tmask = (uint64_t)tile - 1;
-      ftbits = tbits;
-    }
+    // A tile that filled up is found when the output size is read back (no synchronisation
+    // here); the probe of such a table reads only in-bounds slots, and its result is discarded.
+    built  = true;
+    tiled  = true;
+    tmask  = (uint64_t)tile - 1;
+    ftbits = tbits;
   }
-  if (!built && nb > 0) {
+  auto global_build = [&]() {
     ProfScope prof("join_build", s);
     tmask = mask;
     fill_u32(reinterpret_cast<uint32_t*>(err.get()), 0, 1, s);
@@ -676,6 +678,9 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
                          cnt.get(), rows.get());
       LDF_HIP_TRY(hipGetLastError());
     }
+  };
+  if (!built && nb > 0) {
+    global_build();
   } else if (nb == 0) {
     hipLaunchKernelGGL(k_jslot_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab.get(), nslot);
     LDF_HIP_TRY(hipGetLastError());
@@ -688,7 +693,14 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   Buf<uint64_t> btot(a, nblk);
   Buf<uint64_t> boff(a, nblk + 1);
   const bool mark = jt == LDF_JOIN_FULL;  // matched slots flagged in their count words
-  {
+  int64_t ublk = (nb + kJRows - 1) / kJRows;
+  if (ublk < 1) ublk = 1;
+  Buf<uint64_t> utot, uoff;
+  if (jt == LDF_JOIN_FULL) {
+    utot = Buf<uint64_t>(a, ublk);
+    uoff = Buf<uint64_t>(a, ublk + 1);
+  }
+  auto probe = [&]() {
   ProfScope prof("join_count", s);
   if (generic)
     hipLaunchKernelGGL(k_join_count<true>, dim3(nblk), dim3(kJBlock), 0, s, pk, bk, np, tab.get(), mask, tmask, cap,
@@ -698,22 +710,35 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
                        nulls_equal, (int)jt, ftbits >= 0 ? (const uint32_t*)filt.get() : nullptr, ftbits, info.get(),
                        mark, btot.get());
   LDF_HIP_TRY(hipGetLastError());
-  }
   exclusive_scan_u64(btot.get(), boff.get(), nblk, a);
-
-  int64_t ublk = (nb + kJRows - 1) / kJRows;
-  if (ublk < 1) ublk = 1;
-  Buf<uint64_t> utot, uoff;
   if (jt == LDF_JOIN_FULL) {
-    utot = Buf<uint64_t>(a, ublk);
-    uoff = Buf<uint64_t>(a, ublk + 1);
     hipLaunchKernelGGL(k_join_unmatched, dim3(ublk), dim3(kJBlock), 0, s, slot_of_row.get(), nb, (const JSlot*)tab.get(),
                        (const uint64_t*)nullptr, (uint64_t)0, utot.get(), (int32_t*)nullptr, (int32_t*)nullptr);
     LDF_HIP_TRY(hipGetLastError());
     exclusive_scan_u64(utot.get(), uoff.get(), ublk, a);
   }
-  uint64_t n1 = read_u64(boff.get() + nblk, s);
-  uint64_t n2 = (jt == LDF_JOIN_FULL) ? read_u64(uoff.get() + ublk, s) : 0;
+  };
+  // output sizes (and the tiled build's overflow flag) read back with one synchronisation
+  uint64_t n1 = 0, n2 = 0;
+  auto read_sizes = [&](bool with_err) -> bool {
+    uint64_t* st = static_cast<uint64_t*>(pinned_stage(24));
+    st[0] = st[1] = st[2] = 0;
+    LDF_HIP_TRY(hipMemcpyAsync(st, boff.get() + nblk, 8, hipMemcpyDeviceToHost, s));
+    if (jt == LDF_JOIN_FULL) LDF_HIP_TRY(hipMemcpyAsync(st + 1, uoff.get() + ublk, 8, hipMemcpyDeviceToHost, s));
+    if (with_err) LDF_HIP_TRY(hipMemcpyAsync(st + 2, err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    n1 = st[0];
+    n2 = st[1];
+    return st[2] == 0;
+  };
+  probe();
+  if (!read_sizes(tiled)) {  // a tile filled up: the global build, and the probe again
+    tmask  = mask;
+    ftbits = -1;
+    global_build();
+    probe();
+    read_sizes(false);
+  }
   int64_t n_out = (int64_t)(n1 + n2);
 
   // emit gather maps
@@ -741,12 +766,23 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
   std::vector<const ldf_column_view*> lsrc(n_lhs_out), rsrc(n_rhs_out);
   for (int i = 0; i < n_lhs_out; ++i) lsrc[i] = &lhs->cols[lhs_out[i]];
   for (int i = 0; i < n_rhs_out; ++i) rsrc[i] = &rhs->cols[rhs_out[i]];
-  gather_columns(lsrc.data(), n_lhs_out, lmap.get(), n_out, jt == LDF_JOIN_FULL, a, out->cols);
+  // both sides' null counts land in one device array, read back with one synchronisation
+  const int nout_cols = n_lhs_out + n_rhs_out;
+  Buf<unsigned long long> ncnt(a, nout_cols > 0 ? nout_cols : 1);
+  fill_u64(reinterpret_cast<uint64_t*>(ncnt.get()), 0, nout_cols > 0 ? nout_cols : 1, s);
+  gather_columns(lsrc.data(), n_lhs_out, lmap.get(), n_out, jt == LDF_JOIN_FULL, a, out->cols, false, ncnt.get());
+  int allocated = n_lhs_out;  // output columns this function owns (a failed gather frees its own)
   try {
     gather_columns(rsrc.data(), n_rhs_out, rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
-                   out->cols + n_lhs_out, /*random_map=*/true);
+                   out->cols + n_lhs_out, /*random_map=*/true, ncnt.get() + n_lhs_out);
+    allocated = nout_cols;
+    if (nout_cols > 0) {
+      std::vector<unsigned long long> hc(nout_cols);
+      copy_to_host(hc.data(), ncnt.get(), hc.size() * sizeof(unsigned long long), s);
+      for (int i = 0; i < nout_cols; ++i) out->cols[i].null_count = out->cols[i].validity ? (int64_t)hc[i] : 0;
+    }
   } catch (...) {
-    for (int i = 0; i < n_lhs_out; ++i) free_column(&out->cols[i], a);
+    for (int i = 0; i < allocated; ++i) free_column(&out->cols[i], a);
     out->ncols = 0;
     throw;
   }

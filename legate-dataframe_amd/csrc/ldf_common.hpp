@@ -283,10 +283,14 @@ void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out
 // The same for `ncols` columns sharing one map (read once per row for up to kMaxGather columns).
 // `random_map` says the map visits source rows in no particular order (a build side's rows, a
 // permutation): the source rows may then be packed into records read once per output row.
+// `null_counts` (device, ncols entries, zeroed by the caller): the output columns' null counts
+// are added there and not read back (out[i].null_count is left for the caller to set), so a
+// caller gathering several tables reads them all with one synchronisation.
 // On error every column this call allocated is freed.
 constexpr int kMaxGather = 8;
 void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t* map, int64_t n_out,
-                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map = false);
+                    bool force_nullable, Alloc& a, ldf_column* out, bool random_map = false,
+                    unsigned long long* null_counts = nullptr);
 void alloc_column(int32_t type, int64_t n, bool nullable, int64_t string_bytes, Alloc& a,
                   ldf_column* out);
 void free_column(ldf_column* c, Alloc& a);
