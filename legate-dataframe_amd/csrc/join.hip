@@ -229,8 +229,6 @@ __global__ __launch_bounds__(kBHist) void k_jb_scatter(DCol c, int64_t nb, int64
   }
 }
 
-// One block per bin: bins < ntiles are table tiles of `tile` slots; bin ntiles is slot cap (the
-// empty-marker key), bin ntiles + 1 is slot cap + 1 (the null key).
 // Probe filter of a tiled table: kFiltWords 32-bit words per tile, two bits per key in one word
 // (a blocked Bloom filter), chosen by hash bits 32 and up, which the slot index (bits below 30)
 // does not use.  A probe key whose two bits are not both set is not in the table, and the probe
@@ -243,6 +241,8 @@ __device__ __forceinline__ uint32_t filt_word(uint64_t hk, int tbits, uint64_t m
 }
 __device__ __forceinline__ uint32_t filt_bits(uint64_t hk) { return (1u << ((hk >> 40) & 31)) | (1u << ((hk >> 45) & 31)); }
 
+// One block per bin: bins < ntiles are table tiles of `tile` slots; bin ntiles is slot cap (the
+// empty-marker key), bin ntiles + 1 is slot cap + 1 (the null key).
 __global__ __launch_bounds__(256) void k_jb_tiles(const uint4* __restrict__ recs, const uint64_t* __restrict__ offs,
                                                   int nblk_hist, int ntiles, int tile, uint64_t cap,
                                                   JSlot* __restrict__ tab, uint32_t* __restrict__ rows,
