@@ -3,7 +3,8 @@
  *
  * This is the boundary the GPU task variants of legate-dataframe would bind: plain pointers
  * and sizes, no C++ types, no exceptions crossing it.  Every entry point enqueues its work on
- * the caller's HIP stream; the only host synchronisation is reading an output row count.
+ * the caller's HIP stream; the host synchronises only for small readbacks that the next
+ * allocation or the output needs (row counts, string sizes, sample statistics, null counts).
  *
  * Reference interfaces replaced (paths relative to the legate-dataframe source tree):
  *   ldf_join            <- JoinTask<bool>::gpu_variant (cpp/src/join.cu:187-244), which calls
