@@ -628,7 +628,8 @@ struct PartBufs {
   int32_t key_width;
   int32_t nblk;  // histogram blocks: the stride of offsets / hoffsets
   unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
-  const uint64_t* hoffsets;    // exclusive scan of hot pair counts[hpart][blk]
+  const uint64_t* hoffsets;    // hot pairs before each histogram block (exclusive scan)
+  uint32_t* pcounts;           // k_gb_main3's hot pairs per (pair partition, block)
   // dpart with one sub-pass: k_gb_part3 appends each partition's (key << 32 | value) pairs of rows
   // with a valid value at the start of the partition's range, and their number per partition
   unsigned long long* cpairs;
@@ -670,6 +671,7 @@ constexpr int kHPairParts = 1 << kHPairBits;
 constexpr int kHPairFine  = 16;
 constexpr int kPairSet    = 8192;  // LDS pair-set slots of the dedupe kernels (64 KB)
 __device__ __forceinline__ uint64_t pair_hash(unsigned long long pr) { return mix64(pr ^ 0x94d049bb133111ebull); }
+__device__ __forceinline__ uint32_t pair_part(unsigned long long pr) { return (uint32_t)(pair_hash(pr) >> (64 - kHPairBits)); }
 
 // Builds the LDS table of hot keys when `table` and clears the hot accumulators when `accs`.
 __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
@@ -762,12 +764,12 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  __shared__ uint32_t hhist[kHPairParts];
+  __shared__ uint64_t s_red[17];
   const int nparts = 1 << p.part_bits;
   uint32_t* hist   = reinterpret_cast<uint32_t*>(lds + p.lds_bytes);  // after the hot table
   const int nhot   = (int)min(*hot.count, (unsigned int)p.hot_cap);
+  uint64_t npairs  = 0;  // this thread's hot rows with a COUNT_DISTINCT value
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) hist[i] = 0;
-  for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hhist[i] = 0;
   const int t32             = hot32_slots(p.hot_cap);
   unsigned long long* tab32 = reinterpret_cast<unsigned long long*>(lds);
   if (PACK32) {
@@ -856,18 +858,13 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
         cp                = cold_part(hk, p.part_bits);
       }
       codes[r]          = (uint16_t)(h >= 0 ? (uint32_t)h : (kCodeCold | cp));
-      if (h < 0) {
-        atomicAdd(&hist[cp], 1u);
-      } else if (dv[j]) {
-        const unsigned long long pr = ((unsigned long long)(uint32_t)h << 32) | d[j];
-        atomicAdd(&hhist[pair_hash(pr) >> (64 - kHPairBits)], 1u);
-      }
+      if (h < 0) atomicAdd(&hist[cp], 1u);
+      else if (dv[j]) ++npairs;
     }
   }
-  __syncthreads();
+  const uint64_t block_pairs = block_reduce_sum(npairs, s_red);  // synchronises the block
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) counts[(int64_t)i * nh + blockIdx.x] = hist[i];
-  if (p.dpart)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcounts[(int64_t)i * nh + blockIdx.x] = hhist[i];
+  if (p.dpart && threadIdx.x == 0) hcounts[blockIdx.x] = (uint32_t)block_pairs;
 }
 
 // LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
@@ -975,25 +972,30 @@ __device__ __forceinline__ void lds_put(unsigned char* at, int width, uint64_t b
 // each takes the next position of its partition's region with an LDS atomic on the region
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
 // accumulator is exact and order-independent).  LDS: [0, p.acc_end) hot accumulators, then
-// cursor u64[2^part_bits], hcursor u64[kHPairParts] (dpart), and with p.stage one record-sized
-// slot per thread where a record's fields are written before it is stored as 16-byte pieces.
+// cursor u64[2^part_bits], and with p.stage one record-sized slot per thread where a record's
+// fields are written before it is stored as 16-byte pieces.  With DPART the hot rows'
+// (hot index, value) pairs are appended in row order to the block's range of pb.hpairs (one
+// LDS counter, coalesced stores); k_gb_pair_part then splits them by pair partition.
 template <bool DPART>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp, const uint16_t* __restrict__ codes)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  unsigned long long* cursor  = reinterpret_cast<unsigned long long*>(lds + p.acc_end);
-  const int nparts            = 1 << p.part_bits;
-  unsigned long long* hcursor = cursor + nparts;  // dpart only
-  unsigned char* stage        = reinterpret_cast<unsigned char*>(hcursor + (DPART ? kHPairParts : 0)) +
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + p.acc_end);
+  const int nparts           = 1 << p.part_bits;
+  uint32_t* phist            = reinterpret_cast<uint32_t*>(cursor + nparts);  // DPART: pairs per pair partition
+  unsigned char* stage = reinterpret_cast<unsigned char*>(phist + (DPART ? kHPairParts : 0)) +
                          (size_t)threadIdx.x * (size_t)pb.stride;
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
   const int64_t hb0 = (int64_t)blockIdx.x * kHistSplit;  // first histogram block of this block's rows
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + hb0];
+  __shared__ uint32_t s_pcur;  // hot pairs this block has appended
+  if (threadIdx.x == 0) s_pcur = 0;
   if (DPART)
-    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) hcursor[i] = pb.hoffsets[(int64_t)i * pb.nblk + hb0];
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) phist[i] = 0;
+  const int64_t pair0 = DPART ? (int64_t)pb.hoffsets[hb0] : 0;  // the block's first hot pair
   hot_setup(p, lds, hot, nhot, true, false, &s_null_hot, &s_sent_hot);  // rows come classified
 
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
@@ -1037,11 +1039,23 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
       hx[j]           = r >= end ? -2 : ((code[j] & kCodeCold) ? -1 : (int)code[j]);
     }
     acc_hot_rows<DPART, kGbRowsPerThread>(p, lds, keys, sub, hx, hot, v);
+    if (DPART) {  // hot pairs appended in row order, wave by wave (coalesced stores)
+      const int lane       = threadIdx.x & (kWave - 1);
+      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (int j = 0; j < kGbRowsPerThread; ++j) {
-      if (DPART && hx[j] >= 0 && ((v[j].valid >> p.dcol) & 1)) {
-        const unsigned long long pr = ((unsigned long long)(uint32_t)hx[j] << 32) | (uint32_t)pick(v[j], p.dcol);
-        pb.hpairs[atomicAdd(&hcursor[pair_hash(pr) >> (64 - kHPairBits)], 1ull)] = pr;
+      for (int j = 0; j < kGbRowsPerThread; ++j) {
+        const bool has   = hx[j] >= 0 && ((v[j].valid >> p.dcol) & 1);
+        const uint64_t m = __ballot(has);
+        if (m == 0) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t at      = 0;
+        if (lane == leader) at = atomicAdd(&s_pcur, (uint32_t)__popcll(m));
+        at = __shfl(at, leader, kWave);
+        if (has) {
+          const unsigned long long pr = ((unsigned long long)(uint32_t)hx[j] << 32) | (uint32_t)pick(v[j], p.dcol);
+          pb.hpairs[pair0 + at + (uint32_t)__popcll(m & below)] = pr;
+          atomicAdd(&phist[pair_part(pr)], 1u);
+        }
       }
     }
 #pragma unroll
@@ -1096,6 +1110,8 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
     }
   }
   __syncthreads();
+  if (DPART)  // this block's pairs per pair partition (partition-major, for the split's scan)
+    for (int i = threadIdx.x; i < kHPairParts; i += blockDim.x) pb.pcounts[(int64_t)i * gridDim.x + blockIdx.x] = phist[i];
 
   // hot partials of this block
   const int H = p.hot_cap;
@@ -1502,6 +1518,66 @@ __device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned
     i = (i + 1) & (kPairSet - 1);
   }
   return -1;
+}
+
+// The hot pairs, appended in row order by k_gb_main3 (block b's at [hoffs[b * split],
+// hoffs[(b + 1) * split])), split into the kHPairParts pair partitions, one block per main-pass
+// block (1024 threads = kHPairParts): k_gb_main3 counted its pairs per partition, and `off` is the
+// scan of those counts (partition-major).  Each tile of kPairTile pairs is sorted by partition in
+// LDS and every partition's pairs of the tile are stored as one contiguous run (order inside a
+// partition does not matter to the deduplication).
+constexpr int kPairTile = 16384;  // 16 pairs per partition and tile on average: runs of whole lines
+
+__global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long* __restrict__ src,
+                                                       const uint64_t* __restrict__ hoffs, int split,
+                                                       const uint64_t* __restrict__ off, unsigned long long* __restrict__ dst)
+{
+  __shared__ uint32_t tcnt[kHPairParts];
+  __shared__ uint32_t tstart[kHPairParts];
+  __shared__ unsigned long long cur[kHPairParts];
+  __shared__ unsigned long long stage[kPairTile];
+  __shared__ uint64_t smem[17];
+  constexpr int R  = kPairTile / kHPairParts;  // pairs per thread and tile
+  const int64_t b0 = (int64_t)hoffs[(int64_t)blockIdx.x * split];
+  const int64_t b1 = (int64_t)hoffs[(int64_t)(blockIdx.x + 1) * split];
+  cur[threadIdx.x] = off[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
+  for (int64_t t0 = b0; t0 < b1; t0 += kPairTile) {
+    const int tn     = (int)(b1 - t0 < kPairTile ? b1 - t0 : kPairTile);
+    tcnt[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long pr[R];
+    uint32_t part[R], rank[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = threadIdx.x + j * kHPairParts;
+      pr[j]       = i < tn ? src[t0 + i] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = threadIdx.x + j * kHPairParts;
+      part[j]     = pair_part(pr[j]);
+      rank[j]     = i < tn ? atomicAdd(&tcnt[part[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    uint64_t total;
+    tstart[threadIdx.x] = (uint32_t)block_exclusive_scan(tcnt[threadIdx.x], smem, &total);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = threadIdx.x + j * kHPairParts;
+      if (i < tn) {
+        stage[tstart[part[j]] + rank[j]] = pr[j];
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn; i += kHPairParts) {
+      const unsigned long long v = stage[i];
+      const uint32_t q           = pair_part(v);  // recomputed: no LDS for the staged partitions
+      dst[cur[q] + (uint64_t)(i - (int)tstart[q])] = v;
+    }
+    __syncthreads();
+    cur[threadIdx.x] += tcnt[threadIdx.x];
+  }
 }
 
 // One block per coarse hot-pair partition: split its pairs into kHPairFine fine partitions
@@ -2384,9 +2460,9 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     // Partitions: the fewest (2^10 .. 2^12) whose keys fit one table each -- the sample's
     // estimate runs low on skewed keys: plan for 1.25x as many at <= 45% load -- as far as the
     // main pass's LDS cursors allow; keys beyond that take sub-passes over a partition.
-    // k_gb_main3's LDS: hot accumulators, the partition cursors, the pair cursors, and (when it
-    // fits) one partition record per thread for packing
-    const int hc_bytes = dpart ? kHPairParts * 8 : 0;
+    // k_gb_main3's LDS: hot accumulators, the partition cursors, and (when it fits) one
+    // partition record per thread for packing
+    const int hc_bytes = dpart ? kHPairParts * 4 : 0;  // k_gb_main3's per-block pair histogram
     int rec            = keys.c[0].width + 1;
     for (int c = 0; c < plan.ncol; ++c) rec += plan.col[c].width;
     int stride = 8;
@@ -2487,7 +2563,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
     Buf<unsigned long long> hparts;
     Buf<uint32_t> hcounts;
     Buf<uint16_t> codes;
-    Buf<uint64_t> hoffs;
+    Buf<uint64_t> hoffs, hpoffs;  // hot pairs: per histogram block; per (pair partition, main block)
+    Buf<uint32_t> hpcnt;          // hot pairs per (pair partition, main block)
     Buf<unsigned long long> hpairs, hfine, cpairs;
     Buf<uint32_t> cpair_count;
     plan.dpart = 0;
@@ -2498,7 +2575,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         plan.dpart = 1;
         plan.dcol  = plan.a[dacc].col;
       }
-      const int hc_bytes         = dpart ? kHPairParts * 8 : 0;
+      const int hc_bytes         = dpart ? kHPairParts * 4 : 0;  // the per-block pair histogram
       const int nparts           = 1 << plan.part_bits;
       int rec_bytes              = keys.c[0].width + 1;
       for (int c = 0; c < plan.ncol; ++c) rec_bytes += plan.col[c].width;
@@ -2530,8 +2607,8 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
           attr = true;
         }
         if (dpart) {
-          hcounts = Buf<uint32_t>(a, (size_t)kHPairParts * nh);
-          hoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nh + 1);
+          hcounts = Buf<uint32_t>(a, (size_t)nh);  // hot pairs per histogram block
+          hoffs   = Buf<uint64_t>(a, (size_t)nh + 1);
         }
         codes = Buf<uint16_t>(a, (size_t)n);
         ProfScope prof("gb_hist", s);
@@ -2549,13 +2626,13 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         LDF_HIP_TRY(hipGetLastError());
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);
-      if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)kHPairParts * nh, a);
+      if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)nh, a);
       uint64_t totals[2] = {0, 0};
       {  // both totals queued into the pinned stage, one synchronisation
         uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
         st[1]        = 0;
         LDF_HIP_TRY(hipMemcpyAsync(st, poffs.get() + (size_t)nparts * nh, 8, hipMemcpyDeviceToHost, s));
-        if (dpart) LDF_HIP_TRY(hipMemcpyAsync(st + 1, hoffs.get() + (size_t)kHPairParts * nh, 8, hipMemcpyDeviceToHost, s));
+        if (dpart) LDF_HIP_TRY(hipMemcpyAsync(st + 1, hoffs.get() + (size_t)nh, 8, hipMemcpyDeviceToHost, s));
         LDF_HIP_TRY(hipStreamSynchronize(s));
         totals[0] = st[0];
         totals[1] = st[1];
@@ -2619,8 +2696,10 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       if (dpart) {
         hpairs      = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, nhp));
         hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
-        pb.hpairs   = hpairs.get();
+        pb.hpairs   = hfine.get();  // row order; split by pair partition into hpairs after the pass
         pb.hoffsets = hoffs.get();
+        hpcnt       = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
+        pb.pcounts  = hpcnt.get();
         if (q.S == 1 && ncold > 0) {
           cpairs         = Buf<unsigned long long>(a, (size_t)ncold);
           cpair_count    = Buf<uint32_t>(a, (size_t)nparts);
@@ -2656,6 +2735,15 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
       }
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
       LDF_HIP_TRY(hipGetLastError());
+      // the hot pairs by pair partition (before k_gb_cold_distinct reuses hfine)
+      if (dpart && nhp > 0) {
+        ProfScope prof("gb_pairs", s);
+        hpoffs = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
+        exclusive_scan_u32_to_u64(hpcnt.get(), hpoffs.get(), (int64_t)kHPairParts * nblk, a);
+        hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, s, (const unsigned long long*)hfine.get(),
+                           (const uint64_t*)hoffs.get(), kHistSplit, (const uint64_t*)hpoffs.get(), hpairs.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
       // 6. one block per (partition, sub-pass)
       int32_t loff = 0;
       auto ltake   = [&](int bytes) {
@@ -2703,7 +2791,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
         if (nhp > 0) {
           Buf<uint32_t> part_cnt(a, (size_t)kHPairParts * hot_cap);
           hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, s, (const unsigned long long*)hpairs.get(),
-                             (const uint64_t*)hoffs.get(), nh, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
+                             (const uint64_t*)hpoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
           LDF_HIP_TRY(hipGetLastError());
           hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, s, (const uint32_t*)part_cnt.get(),
                              kHPairParts, hot_cap, hot, d.g0);
