@@ -806,15 +806,13 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   using KT = typename std::conditional<PACK32, uint32_t, uint64_t>::type;
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
     KT k[R];
-    uint32_t d[R];  // pairs use the low 32 bits of the value
-    bool kv[R], dv[R];
+    bool kv[R], dv[R];  // dv: the row has a COUNT_DISTINCT value (only its presence is counted here)
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int64_t r = r0 + (int64_t)j * blockDim.x;
       kv[j]           = r < end && valid_at(kc.valid, r);
       k[j]            = kv[j] ? (KT)load_bits(kc.data, kc.width, r) : (KT)0;
       dv[j]           = p.dpart && r < end && valid_at(dc.valid, r);
-      d[j]            = dv[j] ? (uint32_t)load_bits(dc.data, dc.width, r) : 0u;
     }
     int hh[R];
     uint32_t cpj[R];
