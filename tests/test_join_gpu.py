@@ -264,10 +264,11 @@ def _fmix64(k: np.ndarray) -> np.ndarray:
     return k
 
 
-@pytest.mark.parametrize("how", ["inner", "full"])
+@pytest.mark.parametrize("how", ["inner", "left", "full", "semi", "anti"])
 def test_join_build_tile_overflow_falls_back(gpu, how):
     """5000 distinct build keys chosen to share one 4096-slot tile of the 16384-slot table: the
-    tiled LDS build cannot hold them, and the join rebuilds over the whole table (same result)."""
+    tiled LDS build cannot hold them, which is found when the output size is read back; the
+    join then rebuilds over the whole table and probes again (same result for every join type)."""
     rng = np.random.default_rng(5)
     cand = rng.integers(0, 1 << 40, 200_000, dtype=np.int64)
     tile0 = cand[((_fmix64(cand) & np.uint64(16383)) >> np.uint64(12)) == 0]
