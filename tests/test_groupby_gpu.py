@@ -238,6 +238,34 @@ def test_groupby_c2_full_size_properties(gpu):
     del ks, vs, so, seg
 
 
+def test_groupby_c2_full_size_count_distinct(gpu):
+    """COUNT_DISTINCT at the 100M-row, 1M-key Zipf(1.1) size (hot keys' pairs stored in row order
+    and split by pair partition in many tiles per block, other keys' pairs deduplicated per
+    partition): per key, the number of distinct (key, value) pairs from torch.unique."""
+    import torch
+
+    from legate_dataframe import _capi as C
+
+    n, nk = 100_000_000, 1_000_000
+    g = torch.Generator(device="cuda").manual_seed(7)
+    ranks = torch.arange(1, nk + 1, device="cuda", dtype=torch.float64)
+    cdf = torch.cumsum(ranks.pow(-1.1), 0)
+    cdf /= cdf[-1].clone()
+    k = torch.searchsorted(cdf, torch.rand(n, device="cuda", generator=g, dtype=torch.float64)).to(torch.int32)
+    k.clamp_(max=nk - 1)
+    del cdf
+    u = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int32)
+    dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
+    out = C.groupby([dc(k, C.INT32), dc(u, C.INT32)], [0], [(1, "count_distinct", None)])
+    keys = out[0].data.view(torch.int32)[: out[0].length].to(torch.int64)
+    got = out[1].data.view(torch.int64)[: out[0].length]
+    pairs = torch.unique((k.to(torch.int64) << 32) | u.to(torch.int64))
+    ref = torch.bincount(pairs >> 32, minlength=nk)
+    del pairs
+    assert torch.equal(got, ref[keys])
+    assert int(got.sum()) == int(ref.sum())
+
+
 def test_groupby_errors(gpu):
     from legate_dataframe import LogicalTable
     from legate_dataframe.lib.groupby_aggregation import groupby_aggregation
