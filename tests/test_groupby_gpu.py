@@ -238,22 +238,29 @@ def test_groupby_c2_full_size_properties(gpu):
     del ks, vs, so, seg
 
 
-def test_groupby_c2_full_size_count_distinct(gpu):
-    """COUNT_DISTINCT at the 100M-row, 1M-key Zipf(1.1) size (hot keys' pairs stored in row order
-    and split by pair partition in many tiles per block, other keys' pairs deduplicated per
-    partition): per key, the number of distinct (key, value) pairs from torch.unique."""
+@pytest.mark.parametrize("keys_kind", ["zipf_1M", "uniform_2M"])
+def test_groupby_c2_full_size_count_distinct(gpu, keys_kind):
+    """COUNT_DISTINCT at 100M rows: per key, the number of distinct (key, value) pairs from
+    torch.unique.  zipf_1M (1M keys, Zipf(1.1)): hot keys' pairs stored in row order and split by
+    pair partition in many tiles per block, the other keys' ~7.8K pairs per partition deduplicated
+    in two hash-filtered passes.  uniform_2M (2M keys, none hot): ~24K pairs per partition, split
+    into five groups through global memory before deduplication."""
     import torch
 
     from legate_dataframe import _capi as C
 
-    n, nk = 100_000_000, 1_000_000
+    n = 100_000_000
+    nk = 1_000_000 if keys_kind == "zipf_1M" else 2_000_000
     g = torch.Generator(device="cuda").manual_seed(7)
-    ranks = torch.arange(1, nk + 1, device="cuda", dtype=torch.float64)
-    cdf = torch.cumsum(ranks.pow(-1.1), 0)
-    cdf /= cdf[-1].clone()
-    k = torch.searchsorted(cdf, torch.rand(n, device="cuda", generator=g, dtype=torch.float64)).to(torch.int32)
-    k.clamp_(max=nk - 1)
-    del cdf
+    if keys_kind == "zipf_1M":
+        ranks = torch.arange(1, nk + 1, device="cuda", dtype=torch.float64)
+        cdf = torch.cumsum(ranks.pow(-1.1), 0)
+        cdf /= cdf[-1].clone()
+        k = torch.searchsorted(cdf, torch.rand(n, device="cuda", generator=g, dtype=torch.float64)).to(torch.int32)
+        k.clamp_(max=nk - 1)
+        del cdf
+    else:
+        k = torch.randint(0, nk, (n,), device="cuda", generator=g, dtype=torch.int32)
     u = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int32)
     dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
     out = C.groupby([dc(k, C.INT32), dc(u, C.INT32)], [0], [(1, "count_distinct", None)])
