@@ -672,6 +672,10 @@ constexpr int kHPairFine  = 16;
 constexpr int kPairSet    = 8192;  // LDS pair-set slots of the dedupe kernels (64 KB)
 __device__ __forceinline__ uint64_t pair_hash(unsigned long long pr) { return mix64(pr ^ 0x94d049bb133111ebull); }
 __device__ __forceinline__ uint32_t pair_part(unsigned long long pr) { return (uint32_t)(pair_hash(pr) >> (64 - kHPairBits)); }
+__device__ __forceinline__ uint32_t fine_part(unsigned long long pr)
+{
+  return (uint32_t)(pair_hash(pr) >> (64 - kHPairBits - 4)) & (kHPairFine - 1);
+}
 
 // Builds the LDS table of hot keys when `table` and clears the hot accumulators when `accs`.
 __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
@@ -1591,6 +1595,8 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
   __shared__ unsigned long long set[kPairSet];
   __shared__ uint32_t cnt[2048];
   __shared__ uint32_t fh[kHPairFine];
+  __shared__ uint32_t tcnt[kHPairFine];
+  __shared__ uint32_t tstart[kHPairFine];
   __shared__ unsigned long long fcur[kHPairFine];
   __shared__ int64_t fstart[kHPairFine + 1];
   __shared__ int bad;
@@ -1602,8 +1608,7 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
   if (threadIdx.x < kHPairFine) fh[threadIdx.x] = 0;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x)
-    atomicAdd(&fh[(pair_hash(pairs[r]) >> (64 - kHPairBits - 4)) & (kHPairFine - 1)], 1u);
+  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x) atomicAdd(&fh[fine_part(pairs[r])], 1u);
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t run = 0;
@@ -1615,9 +1620,47 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
     fstart[kHPairFine] = run;
   }
   __syncthreads();
-  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x) {
-    const unsigned long long pr = pairs[r];
-    fine[atomicAdd(&fcur[(pair_hash(pr) >> (64 - kHPairBits - 4)) & (kHPairFine - 1)], 1ull)] = pr;
+  // Tiles of kFineTile pairs are sorted by fine partition in LDS (in the set's memory, which the
+  // deduplication clears first) and each fine partition's run of a tile is stored contiguously.
+  constexpr int kFineTile = 4096;  // 256 pairs per fine partition and tile on average
+  constexpr int R         = kFineTile / 1024;  // pairs per thread and tile (2 blocks per CU)
+  for (int64_t t0 = b0; t0 < b1; t0 += kFineTile) {
+    const int tn = (int)(b1 - t0 < kFineTile ? b1 - t0 : kFineTile);
+    if (threadIdx.x < kHPairFine) tcnt[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long pr[R];
+    uint32_t fb[R], rank[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = threadIdx.x + j * 1024;
+      pr[j]       = i < tn ? pairs[t0 + i] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = threadIdx.x + j * 1024;
+      fb[j]       = fine_part(pr[j]);
+      rank[j]     = i < tn ? atomicAdd(&tcnt[fb[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int f = 0; f < kHPairFine; ++f) {
+        tstart[f] = run;
+        run += tcnt[f];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (threadIdx.x + j * 1024 < tn) set[tstart[fb[j]] + rank[j]] = pr[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn; i += 1024) {
+      const unsigned long long v = set[i];
+      const uint32_t f           = fine_part(v);  // recomputed: no LDS for the staged partitions
+      fine[fcur[f] + (uint64_t)(i - (int)tstart[f])] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kHPairFine) fcur[threadIdx.x] += tcnt[threadIdx.x];
   }
   __syncthreads();
   for (int f = 0; f < kHPairFine; ++f) {
