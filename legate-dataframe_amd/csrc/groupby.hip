@@ -1742,9 +1742,6 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   __shared__ uint32_t gh[kColdGroups];
   __shared__ unsigned long long gcur[kColdGroups];
   __shared__ int64_t gstart[kColdGroups + 1];
-  __shared__ uint32_t tcnt[kColdGroups];
-  __shared__ uint32_t tstart[kColdGroups];
-  __shared__ uint32_t s_tn;
   __shared__ int bad;
   const int part   = blockIdx.x;
   const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
@@ -1794,47 +1791,9 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
     gstart[ng] = run;
   }
   __syncthreads();
-  // Tiles of kColdTile rows: their pairs are sorted by group in LDS (in the set's memory, which
-  // the deduplication clears first) and each group's run of a tile is stored contiguously.
-  constexpr int kColdTile = 4096;
-  constexpr int R         = kColdTile / 1024;  // rows per thread and tile
-  for (int64_t t0 = r0; t0 < r1; t0 += kColdTile) {
-    if (threadIdx.x < kColdGroups) tcnt[threadIdx.x] = 0;
-    __syncthreads();
-    unsigned long long pr[R];
-    uint32_t gr[R], rank[R];
-    bool has[R];  // pr[j] is set only where has[j]
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int64_t r = t0 + threadIdx.x + j * 1024;
-      has[j]          = r < r1 && cold_pair(pb, dcol, dwidth, r, pr[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      gr[j]   = has[j] ? cold_group(pair_hash(pr[j]), ng) : 0u;
-      rank[j] = has[j] ? atomicAdd(&tcnt[gr[j]], 1u) : 0u;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t run = 0;
-      for (int g = 0; g < ng; ++g) {
-        tstart[g] = run;
-        run += tcnt[g];
-      }
-      s_tn = run;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j)
-      if (has[j]) set[tstart[gr[j]] + rank[j]] = pr[j];
-    __syncthreads();
-    for (int i = threadIdx.x; i < (int)s_tn; i += 1024) {
-      const unsigned long long v = set[i];
-      const uint32_t g           = cold_group(pair_hash(v), ng);  // recomputed: no LDS for it
-      fine[gcur[g] + (uint64_t)(i - (int)tstart[g])] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < ng) gcur[threadIdx.x] += tcnt[threadIdx.x];
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    unsigned long long pr;
+    if (cold_pair(pb, dcol, dwidth, r, pr)) fine[atomicAdd(&gcur[cold_group(pair_hash(pr), ng)], 1ull)] = pr;
   }
   __syncthreads();
   for (int g = 0; g < ng; ++g) {
