@@ -17,6 +17,9 @@
  *                          HASH_MURMUR3 and DEFAULT_HASH_SEED, then split at the offsets).
  *   ldf_murmur3_hash    <- the row hash used by that partition step (libcudf MurmurHash3_x86_32
  *                          per column, combined with hash_combine).
+ *   ldf_distinct        <- DistinctTask::gpu_variant (cpp/src/stream_compaction.cu:38-72):
+ *                          cudf::distinct on the key columns, every column returned
+ *                          (client: cpp/src/stream_compaction.cpp:120-146).
  *
  * Column layout in HBM (Arrow-compatible, see ldf_column_view):
  *   fixed width : `data` = length * width bytes, little endian.
@@ -191,6 +194,18 @@ int ldf_hash_partition(const ldf_table_view* table,
                        void* stream,
                        ldf_table_out* out,
                        int64_t* offsets);
+
+/* Distinct rows by the key columns `keys` (cudf::distinct with KEEP_ANY, null_equality::EQUAL,
+ * cpp/src/stream_compaction.cu:38-72): one row per distinct key tuple -- its first row, so the
+ * output rows keep their input order -- with every column of `table` in order.  A null key equals
+ * a null key; floating-point keys compare bitwise (as the groupby and the join do).
+ * `out->cols` must have room for table->ncols columns; 1 to 8 key columns. */
+int ldf_distinct(const ldf_table_view* table,
+                 const int32_t* keys,
+                 int32_t nkeys,
+                 const ldf_allocator* alloc,
+                 void* stream,
+                 ldf_table_out* out);
 
 /* Row hash of the selected columns into a device uint32 array of table rows (the value the
  * partition step reduces mod nparts). */

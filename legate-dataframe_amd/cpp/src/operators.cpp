@@ -1,4 +1,4 @@
-// join / groupby_aggregation / hash_partition: the client half of the reference operators
+// join / groupby_aggregation / hash_partition / distinct: the client half of the reference operators
 // (validation, output schema, name -> index mapping) on top of the C ABI.
 #include <algorithm>
 #include <map>
@@ -8,6 +8,7 @@
 
 #include <legate_dataframe/core/runtime.hpp>
 #include <legate_dataframe/groupby_aggregation.hpp>
+#include <legate_dataframe/stream_compaction.hpp>
 #include <legate_dataframe/hash_partition.hpp>
 #include <legate_dataframe/join.hpp>
 
@@ -220,6 +221,25 @@ std::pair<LogicalTable, std::vector<int64_t>> hash_partition(const LogicalTable&
   std::vector<DataType> types;
   for (auto& c : table.get_columns()) types.push_back(c.type());
   return {LogicalTable(adopt_all(out_cols, types, ncols), table.get_column_name_vector()), offsets};
+}
+
+// ---------------------------------------------------------------------------------------------
+// distinct (reference: cpp/src/stream_compaction.cpp:120-146 and cpp/src/stream_compaction.cu:38-72)
+// ---------------------------------------------------------------------------------------------
+LogicalTable distinct(const LogicalTable& tbl, const std::vector<std::string>& keys, bool high_cardinality)
+{
+  (void)high_cardinality;  // only selects a pre-shuffle local distinct in the reference
+  std::vector<int32_t> key_idx;
+  for (const std::string& key : keys) key_idx.push_back((int32_t)tbl.get_column_names().at(key));
+  std::vector<ldf_column_view> tv;
+  ldf_table_view t = tbl.view(tv);
+  int ncols        = tbl.num_columns();
+  std::vector<ldf_column> out_cols(std::max(1, ncols));
+  ldf_table_out out{ncols, out_cols.data()};
+  detail::check_status(ldf_distinct(&t, key_idx.data(), (int32_t)key_idx.size(), nullptr, get_stream(), &out));
+  std::vector<DataType> types;
+  for (auto& c : tbl.get_columns()) types.push_back(c.type());
+  return LogicalTable(adopt_all(out_cols, types, ncols), tbl.get_column_name_vector());
 }
 
 }  // namespace legate::dataframe

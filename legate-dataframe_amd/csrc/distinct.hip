@@ -1,0 +1,198 @@
+// Distinct rows of a table by a set of key columns.
+//
+// Reference: DistinctTask::gpu_variant (cpp/src/stream_compaction.cu:38-72) runs
+// cudf::distinct(table, keys, KEEP_ANY, null_equality::EQUAL, nan_equality::ALL_EQUAL) on the
+// rows a hash repartition brought to this rank; the client (cpp/src/stream_compaction.cpp:120-146)
+// returns every column of the table, in the table's order.  On one GPU the repartition keeps every
+// row, so `high_cardinality` (which only decides whether a local distinct runs before the
+// shuffle) changes nothing.
+//
+// One row per distinct key tuple is kept: the first row of each key (KEEP_ANY allows any; the
+// first makes the output deterministic), and the output rows keep their input order.  Keys compare
+// as in the groupby and the join: a null equals a null, floating-point keys compare bitwise.
+//
+// Layout in HBM: an open-addressing set of 2^k >= 2n 64-bit slots, each (high 32 bits of the
+// row's 64-bit key hash << 32 | row index), installed with one atomicCAS; a row whose key is
+// already present lowers the slot's row with atomicMin (same hash bits, so the minimum is the
+// smaller row), so once every row is in, each key's slot holds its first row.  A row's slot index
+// (4 bytes per row) lets the marking pass test "slot holds me" without probing again.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "ldf_common.hpp"
+#include "ldf_device.hpp"
+
+namespace ldf {
+
+namespace {
+
+constexpr unsigned long long kDEmpty = ~0ull;  // row index 0xFFFFFFFF never occurs (< 2^31 rows)
+constexpr int kDThreads              = 256;
+constexpr int kDRowsPerThread        = 16;
+constexpr int kDBlockRows            = kDThreads * kDRowsPerThread;
+
+// Inserts every row into the set and records its slot; err = 1 when a probe sequence found no
+// room (cannot happen at load <= 1/2, kept as a guard).
+__global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t n, unsigned long long* __restrict__ set,
+                                                               uint64_t mask, uint32_t* __restrict__ slot_of_row,
+                                                               int32_t* __restrict__ err)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h              = row_hash64(k, r);
+    const unsigned long long mine = ((h >> 32) << 32) | (uint64_t)(uint32_t)r;
+    uint64_t i                    = h & mask;
+    bool placed                   = false;
+    for (uint64_t probes = 0; probes <= mask; ++probes) {
+      unsigned long long cur = set[i];
+      if (cur == kDEmpty) {
+        cur = atomicCAS(&set[i], kDEmpty, mine);
+        if (cur == kDEmpty) {
+          placed = true;
+          break;
+        }
+      }
+      // cur holds a row of the slot's key (its row may still be lowered, never its key)
+      if ((cur >> 32) == (h >> 32) && rows_equal(k, (int64_t)(uint32_t)cur, k, r)) {
+        atomicMin(&set[i], mine);
+        placed = true;
+        break;
+      }
+      i = (i + 1) & mask;
+    }
+    slot_of_row[r] = (uint32_t)i;
+    if (!placed) *err = 1;
+  }
+}
+
+// keep[r] = 1 when row r is its key's first row; per-block counts of kept rows.  Block b covers
+// rows [b * kDBlockRows, (b + 1) * kDBlockRows), thread t the kDRowsPerThread rows from
+// b * kDBlockRows + t * kDRowsPerThread, so a block scan of the per-thread counts orders the
+// kept rows by row index.
+__global__ __launch_bounds__(kDThreads) void k_distinct_mark(int64_t n, const unsigned long long* __restrict__ set,
+                                                             const uint32_t* __restrict__ slot_of_row,
+                                                             uint8_t* __restrict__ keep, uint32_t* __restrict__ blk_cnt)
+{
+  __shared__ uint64_t smem[17];
+  const int64_t r0 = (int64_t)blockIdx.x * kDBlockRows + (int64_t)threadIdx.x * kDRowsPerThread;
+  uint32_t sl[kDRowsPerThread];
+#pragma unroll
+  for (int j = 0; j < kDRowsPerThread; ++j) sl[j] = r0 + j < n ? slot_of_row[r0 + j] : 0u;
+  unsigned long long v[kDRowsPerThread];
+#pragma unroll
+  for (int j = 0; j < kDRowsPerThread; ++j) v[j] = r0 + j < n ? set[sl[j]] : kDEmpty;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kDRowsPerThread; ++j) {
+    const int64_t r = r0 + j;
+    if (r >= n) continue;
+    const uint8_t kp = (uint32_t)v[j] == (uint32_t)r ? 1 : 0;
+    keep[r]          = kp;
+    cnt += kp;
+  }
+  const uint64_t tot = block_reduce_sum(cnt, smem);
+  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = (uint32_t)tot;
+}
+
+// map[blk_off[b] + rank] = r for every kept row r, in row order.
+__global__ __launch_bounds__(kDThreads) void k_distinct_emit(int64_t n, const uint8_t* __restrict__ keep,
+                                                             const uint64_t* __restrict__ blk_off,
+                                                             int32_t* __restrict__ map)
+{
+  __shared__ uint64_t smem[17];
+  const int64_t r0 = (int64_t)blockIdx.x * kDBlockRows + (int64_t)threadIdx.x * kDRowsPerThread;
+  uint8_t kp[kDRowsPerThread];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kDRowsPerThread; ++j) {
+    kp[j] = r0 + j < n ? keep[r0 + j] : 0;
+    cnt += kp[j];
+  }
+  uint64_t total;
+  uint64_t pos = blk_off[blockIdx.x] + block_exclusive_scan(cnt, smem, &total);
+#pragma unroll
+  for (int j = 0; j < kDRowsPerThread; ++j)
+    if (kp[j]) map[pos++] = (int32_t)(r0 + j);
+}
+
+}  // namespace
+
+static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t nkeys, Alloc& a, ldf_table_out* out)
+{
+  hipStream_t s = a.stream();
+  LDF_EXPECTS(t != nullptr && out != nullptr && (out->cols != nullptr || t->ncols == 0), "distinct: NULL argument");
+  LDF_EXPECTS(nkeys >= 1 && nkeys <= kMaxKeyCols, "distinct: between 1 and 8 key columns");
+  LDF_EXPECTS(keys != nullptr, "distinct: NULL key list");
+  LDF_EXPECTS(out->ncols >= t->ncols, "distinct: output table has too few column slots");
+  const int64_t n = t->ncols > 0 ? t->cols[0].length : 0;
+  for (int i = 0; i < t->ncols; ++i) validate_view(t->cols[i], n, "distinct");
+  LDF_EXPECTS(n < INT32_MAX, "distinct: limited to 2^31-1 rows per call");
+  DKeys k;
+  k.n = nkeys;
+  for (int i = 0; i < nkeys; ++i) {
+    if (keys[i] < 0 || keys[i] >= t->ncols) throw Error(LDF_E_INVALID, "distinct: key index out of range");
+    k.c[i] = to_dcol(t->cols[keys[i]]);
+  }
+
+  int64_t n_out = 0;
+  Buf<int32_t> map;
+  if (n > 0) {
+    const uint64_t cap  = next_pow2((uint64_t)(n > 32 ? 2 * n : 64));
+    const uint64_t mask = cap - 1;
+    Buf<unsigned long long> set(a, cap);
+    Buf<uint32_t> slot_of_row(a, n);
+    Buf<int32_t> err(a, 1);
+    fill_u64(reinterpret_cast<uint64_t*>(set.get()), kDEmpty, (int64_t)cap, s);
+    LDF_HIP_TRY(hipMemsetAsync(err.get(), 0, sizeof(int32_t), s));
+    {
+      ProfScope prof("distinct_insert", s);
+      hipLaunchKernelGGL(k_distinct_insert, dim3(grid_for(n, kDThreads)), dim3(kDThreads), 0, s, k, n, set.get(), mask,
+                         slot_of_row.get(), err.get());
+      LDF_HIP_TRY(hipGetLastError());
+    }
+    const int64_t nblk = (n + kDBlockRows - 1) / kDBlockRows;
+    Buf<uint8_t> keep(a, n);
+    Buf<uint32_t> blk_cnt(a, nblk);
+    Buf<uint64_t> blk_off(a, nblk + 1);
+    hipLaunchKernelGGL(k_distinct_mark, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, set.get(), slot_of_row.get(),
+                       keep.get(), blk_cnt.get());
+    LDF_HIP_TRY(hipGetLastError());
+    exclusive_scan_u32_to_u64(blk_cnt.get(), blk_off.get(), nblk, a);
+    uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
+    st[0] = st[1] = 0;
+    LDF_HIP_TRY(hipMemcpyAsync(st, blk_off.get() + nblk, 8, hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipMemcpyAsync(st + 1, err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    n_out             = (int64_t)st[0];
+    const bool failed = st[1] != 0;
+    if (failed) throw Error(LDF_E_INTERNAL, "distinct: hash set overflow");
+    set.reset();
+    slot_of_row.reset();
+    map = Buf<int32_t>(a, n_out > 0 ? n_out : 1);
+    if (n_out > 0) {
+      hipLaunchKernelGGL(k_distinct_emit, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),
+                         (const uint64_t*)blk_off.get(), map.get());
+      LDF_HIP_TRY(hipGetLastError());
+    }
+  } else {
+    map = Buf<int32_t>(a, 1);
+  }
+  // every column, gathered at the kept rows (in row order, so the reads are nearly sequential)
+  out->ncols = t->ncols;
+  std::vector<const ldf_column_view*> src(t->ncols);
+  for (int i = 0; i < t->ncols; ++i) src[i] = &t->cols[i];
+  gather_columns(src.data(), t->ncols, map.get(), n_out, false, a, out->cols);
+}
+
+}  // namespace ldf
+
+using namespace ldf;
+
+extern "C" int ldf_distinct(const ldf_table_view* table, const int32_t* keys, int32_t nkeys,
+                            const ldf_allocator* alloc, void* stream, ldf_table_out* out)
+{
+  return guarded([&] {
+    Alloc a(alloc, (hipStream_t)stream);
+    run_distinct(table, keys, nkeys, a, out);
+  });
+}

@@ -27,7 +27,7 @@ AGG = {"sum": 0, "product": 1, "min": 2, "max": 3, "count": 4, "mean": 5, "varia
 COUNT_MODE = {"only_valid": 0, "only_null": 1, "all": 2}
 
 EXPORTED = [
-    "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash",
+    "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_distinct",
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
     "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release",
@@ -108,6 +108,8 @@ def load(require: bool = True):
         L.ldf_hash_partition.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_uint32, P(Allocator), ctypes.c_void_p, P(TableOut),
                                          P(ctypes.c_int64)]
+        L.ldf_distinct.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, P(Allocator), ctypes.c_void_p,
+                                   P(TableOut)]
         L.ldf_murmur3_hash.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_uint32,
                                        ctypes.c_void_p, ctypes.c_void_p]
         L.ldf_bools_to_bitmask.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
@@ -137,7 +139,8 @@ def load(require: bool = True):
             getattr(L, f).restype = ctypes.c_int
         L.ldf_last_error.restype = ctypes.c_char_p
         L.ldf_version.restype = ctypes.c_char_p
-        for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_bools_to_bitmask",
+        for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_distinct",
+                  "ldf_bools_to_bitmask",
                   "ldf_bitmask_to_bools", "ldf_ranges_to_offsets", "ldf_offsets_to_ranges"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -343,6 +346,18 @@ def groupby(table: Sequence[DeviceColumn], keys, aggs, alloc: Optional[TorchAllo
     out = TableOut(ncols, cols)
     check(L.ldf_groupby(ctypes.byref(tv), int32_array(keys), len(keys), arr, len(aggs), ctypes.byref(alloc.struct),
                         stream_ptr(), ctypes.byref(out)))
+    return _adopt_columns(alloc, out)
+
+
+def distinct(table: Sequence[DeviceColumn], keys, alloc: Optional[TorchAllocator] = None):
+    """Distinct rows by the `keys` columns: every column, the first row of each key tuple."""
+    L = load()
+    alloc = alloc or TorchAllocator()
+    tv, _ta = make_table_view([c.view() for c in table])
+    outc = (Column * max(1, len(table)))()
+    out = TableOut(len(table), outc)
+    check(L.ldf_distinct(ctypes.byref(tv), int32_array(keys), len(keys), ctypes.byref(alloc.struct), stream_ptr(),
+                         ctypes.byref(out)))
     return _adopt_columns(alloc, out)
 
 

@@ -8,6 +8,7 @@
 #include <legate_dataframe/groupby_aggregation.hpp>
 #include <legate_dataframe/hash_partition.hpp>
 #include <legate_dataframe/join.hpp>
+#include <legate_dataframe/stream_compaction.hpp>
 
 namespace py = pybind11;
 using namespace legate::dataframe;
@@ -156,6 +157,14 @@ PYBIND11_MODULE(_ldf, m)
       return hash_partition(t, cols, nparts, seed);
     },
     py::arg("table"), py::arg("columns"), py::arg("num_partitions"), py::arg("seed") = 0);
+
+  m.def(
+    "distinct",
+    [](const LogicalTable& t, const std::vector<std::string>& keys, bool high_cardinality) {
+      py::gil_scoped_release nogil;
+      return distinct(t, keys, high_cardinality);
+    },
+    py::arg("tbl"), py::arg("keys"), py::arg("high_cardinality") = false);
 
   m.def("set_stream", [](uintptr_t s) { set_stream(reinterpret_cast<void*>(s)); });
   m.def("get_stream", []() { return reinterpret_cast<uintptr_t>(get_stream()); });

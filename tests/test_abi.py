@@ -64,6 +64,13 @@ def test_invalid_arguments_return_status_without_device():
     offs = (ctypes.c_int64 * 5000)()
     rc = L.ldf_hash_partition(ctypes.byref(tv), C.int32_array([0]), 1, 4096, 0, None, None, ctypes.byref(gout), offs)
     assert rc == -4
+    # distinct: no keys, a key index out of range, too few output slots
+    rc = L.ldf_distinct(ctypes.byref(tv), C.int32_array([]), 0, None, None, ctypes.byref(gout))
+    assert rc == -1 and "key" in L.ldf_last_error().decode()
+    rc = L.ldf_distinct(ctypes.byref(tv), C.int32_array([3]), 1, None, None, ctypes.byref(gout))
+    assert rc == -1 and "out of range" in L.ldf_last_error().decode()
+    rc = L.ldf_distinct(ctypes.byref(tv), C.int32_array([0]), 1, None, None, ctypes.byref(C.TableOut(0, outc)))
+    assert rc == -1 and "slots" in L.ldf_last_error().decode()
 
 
 def test_python_api_mirrors_reference_signatures():
@@ -80,6 +87,12 @@ def test_python_api_mirrors_reference_signatures():
                        "nulls_equal", "broadcast"]
     assert p["nulls_equal"].default is True
     assert list(inspect.signature(groupby_aggregation).parameters) == ["table", "keys", "column_aggregations"]
+    from legate_dataframe.lib.stream_compaction import distinct
+
+    p = inspect.signature(distinct).parameters
+    assert list(p) == ["tbl", "keys", "high_cardinality"]
+    assert p["keys"].default is None and p["high_cardinality"].default is False
+    assert p["high_cardinality"].kind is inspect.Parameter.KEYWORD_ONLY
 
 
 def test_groupby_python_validation_without_device():

@@ -1,0 +1,134 @@
+"""GPU distinct parity: ldf_distinct (through the C++ API and the C ABI) keeps one row per distinct
+key tuple -- the key's first row, rows in input order -- with every column.  Bar: bit-exact against
+the reference's own test case (python/tests/test_stream_compaction.py:97-121, compared sorted as
+the reference does and also in input order) and against a first-occurrence scan in Python (a dict
+over key tuples) on random tables with nulls, strings, composite keys and floating-point keys
+(bitwise: -0.0 and 0.0 are different keys, a NaN equals a NaN with the same bits); at 50M rows,
+against torch.unique's first index of each key."""
+
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+import pyarrow as pa
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_distinct(t: pa.Table, keys, high_cardinality=False) -> pa.Table:
+    from legate_dataframe import LogicalTable
+    from legate_dataframe.lib.stream_compaction import distinct
+
+    return distinct(LogicalTable.from_arrow(t), keys, high_cardinality=high_cardinality).to_arrow()
+
+
+def _bits(v):
+    return ("f", struct.pack("<d", v)) if isinstance(v, float) else v
+
+
+def first_rows(t: pa.Table, keys) -> list:
+    """Row index of each key tuple's first occurrence, in row order."""
+    cols = [[_bits(v) for v in t.column(k).to_pylist()] for k in keys]
+    seen, out = set(), []
+    for i, key in enumerate(zip(*cols)):
+        if key not in seen:
+            seen.add(key)
+            out.append(i)
+    return out
+
+
+def assert_rows_equal(got: pa.Table, exp: pa.Table):
+    assert got.column_names == exp.column_names
+    assert got.num_rows == exp.num_rows
+    for name in exp.column_names:
+        g = [_bits(v) for v in got.column(name).to_pylist()]
+        e = [_bits(v) for v in exp.column(name).to_pylist()]
+        assert g == e, name
+
+
+@pytest.mark.parametrize("keys, high_cardinality",
+                         [(["b", "a"], True), (["a", "b"], False), (["b", "c"], False), (None, False)])
+def test_distinct_reference_case(gpu, keys, high_cardinality):
+    df = pa.table({"a": [1, 1, 2, 3, 4], "b": [2, 2, 4, 4, None], "c": [10, 10, 20, 30, 40]})
+    res = gpu_distinct(df, keys, high_cardinality)
+    expect = pa.table({"a": [1, 2, 3, 4], "b": [2, 4, 4, None], "c": [10, 20, 30, 40]})
+    sort_by = [("a", "ascending"), ("b", "ascending"), ("c", "ascending")]
+    assert_rows_equal(res.sort_by(sort_by), expect.sort_by(sort_by))
+    assert_rows_equal(res, expect)  # first rows, in input order
+
+
+def _random_table(kind: str, n: int, seed: int) -> tuple[pa.Table, list]:
+    rng = np.random.default_rng(seed)
+    payload = {"p_f": pa.array(rng.uniform(-1, 1, n), mask=rng.random(n) < 0.05),
+               "p_s": pa.array([f"s{x}" for x in rng.integers(0, 1000, n)], mask=rng.random(n) < 0.05)}
+    if kind == "int64_nulls":
+        t = pa.table({"k": pa.array(rng.integers(0, 50_000, n), mask=rng.random(n) < 0.02), **payload})
+        return t, ["k"]
+    if kind == "int32_string":
+        words = np.array(["", "a", "bb", "ccc", "a much longer string value", "dd"])
+        t = pa.table({"k1": pa.array(rng.integers(0, 300, n).astype(np.int32), mask=rng.random(n) < 0.03),
+                      "k2": pa.array(list(words[rng.integers(0, len(words), n)]), mask=rng.random(n) < 0.03),
+                      **payload})
+        return t, ["k1", "k2"]
+    if kind == "float64_special":
+        vals = np.array([0.0, -0.0, np.nan, 1.5, -2.25, np.inf, -np.inf, 3.0])
+        t = pa.table({"k": pa.array(vals[rng.integers(0, len(vals), n)], mask=rng.random(n) < 0.05), **payload})
+        return t, ["k"]
+    if kind == "int8_all_columns":
+        t = pa.table({"k": pa.array(rng.integers(-128, 128, n).astype(np.int8)),
+                      "q": pa.array(rng.integers(0, 3, n).astype(np.int16))})
+        return t, ["k", "q"]
+    if kind == "all_distinct":
+        t = pa.table({"k": pa.array(rng.permutation(n).astype(np.int64)), **payload})
+        return t, ["k"]
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["int64_nulls", "int32_string", "float64_special", "int8_all_columns",
+                                  "all_distinct"])
+def test_distinct_matches_first_occurrence(gpu, kind):
+    n = 200_000
+    t, keys = _random_table(kind, n, seed=5)
+    got = gpu_distinct(t, keys)
+    exp = t.take(pa.array(first_rows(t, keys), pa.int64()))
+    assert_rows_equal(got, exp.cast(got.schema))
+
+
+def test_distinct_empty_and_single_row(gpu):
+    t = pa.table({"k": pa.array([], pa.int64()), "s": pa.array([], pa.large_utf8())})
+    got = gpu_distinct(t, ["k"])
+    assert got.num_rows == 0 and got.column_names == ["k", "s"]
+    t = pa.table({"k": pa.array([None], pa.int64()), "s": pa.array(["x"], pa.large_utf8())})
+    assert_rows_equal(gpu_distinct(t, ["k"]), t)
+
+
+def test_distinct_unknown_key_raises(gpu):
+    t = pa.table({"k": pa.array([1, 2], pa.int64())})
+    with pytest.raises(Exception):
+        gpu_distinct(t, ["nope"])
+
+
+def test_distinct_50M_rows_against_torch_unique(gpu):
+    """50M int64 keys over 10M values with a float payload, through the C ABI on device memory:
+    the kept rows are each key's first row (scatter-min of the row index over torch.unique's
+    inverse), in row order."""
+    import torch
+
+    from legate_dataframe import _capi as C
+
+    n = 50_000_000
+    g = torch.Generator(device="cuda").manual_seed(11)
+    k = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int64)
+    v = torch.rand(n, device="cuda", generator=g, dtype=torch.float64)
+    dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
+    out = C.distinct([dc(k, C.INT64), dc(v, C.DOUBLE)], [0])
+    m = out[0].length
+    uniq, inv = torch.unique(k, return_inverse=True)
+    first = torch.full((uniq.numel(),), n, device="cuda", dtype=torch.int64)
+    first.scatter_reduce_(0, inv, torch.arange(n, device="cuda", dtype=torch.int64), reduce="amin")
+    first, _ = torch.sort(first)
+    assert m == uniq.numel()
+    assert torch.equal(out[0].data.view(torch.int64)[:m], k[first])
+    assert torch.equal(out[1].data.view(torch.float64)[:m], v[first])
