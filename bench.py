@@ -15,7 +15,8 @@ use_threads=False) on the same sample is reported beside it.
 Other workloads (also reported as `secondary` entries of the default line): --workload join
 (1M-build x 10M-probe inner join), --workload join_left / join_full (50M-build x 500M-probe with
 string payloads), --workload partition (1B int64 keys -> 256 partitions), --copy-calibration (a
-plain copy of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE).
+plain copy of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE).  --workload distinct
+(100M rows by an int64 key over 10^7 values, with a float64 column) is measured on request only.
 """
 
 from __future__ import annotations
@@ -578,6 +579,36 @@ def run_partition(args, torch, C):
             "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
 
 
+def run_distinct(args, torch, C):
+    n = args.rows or 100_000_000
+    g = torch.Generator(device="cuda").manual_seed(6)
+    k = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int64)
+    v = torch.rand(n, device="cuda", generator=g, dtype=torch.float64)
+    cols = [C.DeviceColumn(C.INT64, n, k.view(torch.uint8)), C.DeviceColumn(C.DOUBLE, n, v.view(torch.uint8))]
+    for _ in range(args.warmup):
+        out = C.distinct(cols, [0])
+        del out
+    torch.cuda.synchronize()
+    C.profile_reset()
+    C.profile_enable(True)
+    m = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = C.distinct(cols, [0])
+        m = out[0].length
+        del out
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    C.profile_enable(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern = {k: C.profile_get(k) for k in ("distinct_insert",)}
+    alg = 8 * n + 32 * m  # the key column once; the kept rows' two columns read and written once
+    return {"workload": f"distinct {n} rows by an int64 key U[0, 1e7) with a float64 column (first row per key)",
+            "rows_per_s": n / (ms_step * 1e-3), "ms_per_step": ms_step, "out_rows": m, "alg_bytes": alg,
+            "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+
+
 def copy_calibration(torch, nbytes=4 << 30, reps=5):
     src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
     dst = torch.empty_like(src)
@@ -594,7 +625,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--workload",
-                    choices=["groupby", "join", "join_left", "join_full", "partition", "batched", "served"],
+                    choices=["groupby", "join", "join_left", "join_full", "partition", "batched", "served",
+                             "distinct"],
                     default="groupby")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--queries", type=int, default=200)
@@ -641,6 +673,8 @@ def main():
         res = run_batched(args, torch, C)
     elif args.workload == "served":
         res = run_served(args, torch, C)
+    elif args.workload == "distinct":
+        res = run_distinct(args, torch, C)
     elif args.workload in ("join_left", "join_full"):
         res = run_join_c3(args, torch, C, 1 if args.workload == "join_left" else 2)
     else:

@@ -14,8 +14,9 @@
 // Layout in HBM: an open-addressing set of 2^k >= 2n 64-bit slots, each (high 32 bits of the
 // row's 64-bit key hash << 32 | row index), installed with one atomicCAS; a row whose key is
 // already present lowers the slot's row with atomicMin (same hash bits, so the minimum is the
-// smaller row), so once every row is in, each key's slot holds its first row.  A row's slot index
-// (4 bytes per row) lets the marking pass test "slot holds me" without probing again.
+// smaller row), so once every row is in, each key's slot holds its first row.  Slot values only
+// decrease, so a row that reads its key's slot with a smaller row already there skips the atomic.
+// The kept rows are then flagged from a sequential scan of the slots.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -32,11 +33,10 @@ constexpr int kDThreads              = 256;
 constexpr int kDRowsPerThread        = 16;
 constexpr int kDBlockRows            = kDThreads * kDRowsPerThread;
 
-// Inserts every row into the set and records its slot; err = 1 when a probe sequence found no
-// room (cannot happen at load <= 1/2, kept as a guard).
+// Inserts every row into the set; err = 1 when a probe sequence found no room (cannot happen at
+// load <= 1/2, kept as a guard).
 __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t n, unsigned long long* __restrict__ set,
-                                                               uint64_t mask, uint32_t* __restrict__ slot_of_row,
-                                                               int32_t* __restrict__ err)
+                                                               uint64_t mask, int32_t* __restrict__ err)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t h              = row_hash64(k, r);
@@ -52,44 +52,40 @@ __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t 
           break;
         }
       }
-      // cur holds a row of the slot's key (its row may still be lowered, never its key)
+      // cur holds a row of the slot's key (its row may still be lowered, never its key); a stale
+      // read only shows a larger row than the slot holds, so cur's row < r means nothing to lower
       if ((cur >> 32) == (h >> 32) && rows_equal(k, (int64_t)(uint32_t)cur, k, r)) {
-        atomicMin(&set[i], mine);
+        if ((uint32_t)cur > (uint32_t)r) atomicMin(&set[i], mine);
         placed = true;
         break;
       }
       i = (i + 1) & mask;
     }
-    slot_of_row[r] = (uint32_t)i;
     if (!placed) *err = 1;
   }
 }
 
-// keep[r] = 1 when row r is its key's first row; per-block counts of kept rows.  Block b covers
-// rows [b * kDBlockRows, (b + 1) * kDBlockRows), thread t the kDRowsPerThread rows from
-// b * kDBlockRows + t * kDRowsPerThread, so a block scan of the per-thread counts orders the
-// kept rows by row index.
-__global__ __launch_bounds__(kDThreads) void k_distinct_mark(int64_t n, const unsigned long long* __restrict__ set,
-                                                             const uint32_t* __restrict__ slot_of_row,
-                                                             uint8_t* __restrict__ keep, uint32_t* __restrict__ blk_cnt)
+// keep[row] = 1 for the row each occupied slot holds (keep cleared before).
+__global__ __launch_bounds__(kDThreads) void k_distinct_mark(const unsigned long long* __restrict__ set, uint64_t cap,
+                                                             uint8_t* __restrict__ keep)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long v = set[i];
+    if (v != kDEmpty) keep[(uint32_t)v] = 1;
+  }
+}
+
+// Per-block counts of kept rows.  Block b covers rows [b * kDBlockRows, (b + 1) * kDBlockRows),
+// thread t the kDRowsPerThread rows from b * kDBlockRows + t * kDRowsPerThread, so a block scan
+// of the per-thread counts orders the kept rows by row index (as k_distinct_emit does).
+__global__ __launch_bounds__(kDThreads) void k_distinct_count(int64_t n, const uint8_t* __restrict__ keep,
+                                                              uint32_t* __restrict__ blk_cnt)
 {
   __shared__ uint64_t smem[17];
   const int64_t r0 = (int64_t)blockIdx.x * kDBlockRows + (int64_t)threadIdx.x * kDRowsPerThread;
-  uint32_t sl[kDRowsPerThread];
+  uint32_t cnt     = 0;
 #pragma unroll
-  for (int j = 0; j < kDRowsPerThread; ++j) sl[j] = r0 + j < n ? slot_of_row[r0 + j] : 0u;
-  unsigned long long v[kDRowsPerThread];
-#pragma unroll
-  for (int j = 0; j < kDRowsPerThread; ++j) v[j] = r0 + j < n ? set[sl[j]] : kDEmpty;
-  uint32_t cnt = 0;
-#pragma unroll
-  for (int j = 0; j < kDRowsPerThread; ++j) {
-    const int64_t r = r0 + j;
-    if (r >= n) continue;
-    const uint8_t kp = (uint32_t)v[j] == (uint32_t)r ? 1 : 0;
-    keep[r]          = kp;
-    cnt += kp;
-  }
+  for (int j = 0; j < kDRowsPerThread; ++j) cnt += r0 + j < n ? keep[r0 + j] : 0u;
   const uint64_t tot = block_reduce_sum(cnt, smem);
   if (threadIdx.x == 0) blk_cnt[blockIdx.x] = (uint32_t)tot;
 }
@@ -140,22 +136,25 @@ static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t n
     const uint64_t cap  = next_pow2((uint64_t)(n > 32 ? 2 * n : 64));
     const uint64_t mask = cap - 1;
     Buf<unsigned long long> set(a, cap);
-    Buf<uint32_t> slot_of_row(a, n);
     Buf<int32_t> err(a, 1);
     fill_u64(reinterpret_cast<uint64_t*>(set.get()), kDEmpty, (int64_t)cap, s);
     LDF_HIP_TRY(hipMemsetAsync(err.get(), 0, sizeof(int32_t), s));
     {
       ProfScope prof("distinct_insert", s);
       hipLaunchKernelGGL(k_distinct_insert, dim3(grid_for(n, kDThreads)), dim3(kDThreads), 0, s, k, n, set.get(), mask,
-                         slot_of_row.get(), err.get());
+                         err.get());
       LDF_HIP_TRY(hipGetLastError());
     }
     const int64_t nblk = (n + kDBlockRows - 1) / kDBlockRows;
     Buf<uint8_t> keep(a, n);
     Buf<uint32_t> blk_cnt(a, nblk);
     Buf<uint64_t> blk_off(a, nblk + 1);
-    hipLaunchKernelGGL(k_distinct_mark, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, set.get(), slot_of_row.get(),
-                       keep.get(), blk_cnt.get());
+    LDF_HIP_TRY(hipMemsetAsync(keep.get(), 0, (size_t)n, s));
+    hipLaunchKernelGGL(k_distinct_mark, dim3(grid_for((int64_t)cap, kDThreads)), dim3(kDThreads), 0, s,
+                       (const unsigned long long*)set.get(), cap, keep.get());
+    LDF_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_distinct_count, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),
+                       blk_cnt.get());
     LDF_HIP_TRY(hipGetLastError());
     exclusive_scan_u32_to_u64(blk_cnt.get(), blk_off.get(), nblk, a);
     uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
@@ -167,7 +166,6 @@ static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t n
     const bool failed = st[1] != 0;
     if (failed) throw Error(LDF_E_INTERNAL, "distinct: hash set overflow");
     set.reset();
-    slot_of_row.reset();
     map = Buf<int32_t>(a, n_out > 0 ? n_out : 1);
     if (n_out > 0) {
       hipLaunchKernelGGL(k_distinct_emit, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),
