@@ -198,7 +198,8 @@ int ldf_hash_partition(const ldf_table_view* table,
 /* Distinct rows by the key columns `keys` (cudf::distinct with KEEP_ANY, null_equality::EQUAL,
  * cpp/src/stream_compaction.cu:38-72): one row per distinct key tuple -- its first row, so the
  * output rows keep their input order -- with every column of `table` in order.  A null key equals
- * a null key; floating-point keys compare bitwise (as the groupby and the join do).
+ * a null key; every NaN key equals every NaN and -0.0 equals 0.0 (nan_equality::ALL_EQUAL; the
+ * groupby and the join compare floating-point keys bitwise, as Arrow does).
  * `out->cols` must have room for table->ncols columns; 1 to 8 key columns. */
 int ldf_distinct(const ldf_table_view* table,
                  const int32_t* keys,

@@ -9,7 +9,8 @@
 //
 // One row per distinct key tuple is kept: the first row of each key (KEEP_ANY allows any; the
 // first makes the output deterministic), and the output rows keep their input order.  Keys compare
-// as in the groupby and the join: a null equals a null, floating-point keys compare bitwise.
+// as cudf::distinct compares them there: a null equals a null, every NaN equals every NaN
+// (nan_equality::ALL_EQUAL) and -0.0 equals 0.0; other values by their bits.
 //
 // Layout in HBM: an open-addressing set of 2^k >= 2n 64-bit slots, each (high 32 bits of the
 // row's 64-bit key hash << 32 | row index), installed with one atomicCAS; a row whose key is
@@ -21,6 +22,7 @@
 
 #include <vector>
 
+#include "ldf_agg.hpp"
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
 
@@ -33,13 +35,57 @@ constexpr int kDThreads              = 256;
 constexpr int kDRowsPerThread        = 16;
 constexpr int kDBlockRows            = kDThreads * kDRowsPerThread;
 
+// Key bits with every NaN mapped to one NaN and -0.0 to 0.0 (other values unchanged).
+__device__ __forceinline__ uint64_t key_bits(const DCol& c, int64_t row)
+{
+  const uint64_t b = load_bits(c.data, c.width, row);
+  if (!is_float_type(c.type)) return b;
+  if (c.width == 4) {
+    const float f = __uint_as_float((uint32_t)b);
+    return f != f ? 0x7fc00000ull : (f == 0.0f ? 0ull : b);
+  }
+  const double d = __longlong_as_double((long long)b);
+  return d != d ? 0x7ff8000000000000ull : (d == 0.0 ? 0ull : b);
+}
+
+__device__ inline uint64_t key_hash(const DKeys& k, int64_t row)
+{
+  uint64_t h = 0;
+  for (int i = 0; i < k.n; ++i) {
+    const DCol& c = k.c[i];
+    uint64_t e;
+    if (!valid_at(c.valid, row)) e = 0x2545f4914f6cdd1dull;
+    else if (c.width == 0) e = string_hash64(static_cast<const uint8_t*>(c.data) + c.offsets[row],
+                                             c.offsets[row + 1] - c.offsets[row]);
+    else e = mix64(key_bits(c, row) + 0x9e3779b97f4a7c15ull);
+    h = mix64(h ^ (e + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
+  }
+  return h;
+}
+
+__device__ inline bool keys_equal(const DKeys& k, int64_t ra, int64_t rb)
+{
+  for (int i = 0; i < k.n; ++i) {
+    const DCol& c = k.c[i];
+    const bool va = valid_at(c.valid, ra), vb = valid_at(c.valid, rb);
+    if (va != vb) return false;
+    if (!va) continue;
+    if (c.width == 0) {
+      if (!strings_equal(c, ra, c, rb)) return false;
+    } else if (key_bits(c, ra) != key_bits(c, rb)) {
+      return false;
+    }
+  }
+  return true;
+}
+
 // Inserts every row into the set; err = 1 when a probe sequence found no room (cannot happen at
 // load <= 1/2, kept as a guard).
 __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t n, unsigned long long* __restrict__ set,
                                                                uint64_t mask, int32_t* __restrict__ err)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t h              = row_hash64(k, r);
+    const uint64_t h              = key_hash(k, r);
     const unsigned long long mine = ((h >> 32) << 32) | (uint64_t)(uint32_t)r;
     uint64_t i                    = h & mask;
     bool placed                   = false;
@@ -54,7 +100,7 @@ __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t 
       }
       // cur holds a row of the slot's key (its row may still be lowered, never its key); a stale
       // read only shows a larger row than the slot holds, so cur's row < r means nothing to lower
-      if ((cur >> 32) == (h >> 32) && rows_equal(k, (int64_t)(uint32_t)cur, k, r)) {
+      if ((cur >> 32) == (h >> 32) && keys_equal(k, (int64_t)(uint32_t)cur, r)) {
         if ((uint32_t)cur > (uint32_t)r) atomicMin(&set[i], mine);
         placed = true;
         break;

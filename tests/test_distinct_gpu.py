@@ -3,8 +3,8 @@ key tuple -- the key's first row, rows in input order -- with every column.  Bar
 the reference's own test case (python/tests/test_stream_compaction.py:97-121, compared sorted as
 the reference does and also in input order) and against a first-occurrence scan in Python (a dict
 over key tuples) on random tables with nulls, strings, composite keys and floating-point keys
-(bitwise: -0.0 and 0.0 are different keys, a NaN equals a NaN with the same bits); at 50M rows,
-against torch.unique's first index of each key."""
+(as cudf::distinct with nan_equality::ALL_EQUAL: every NaN one key, -0.0 equal to 0.0; NaNs with
+other payload bits included); at 50M rows, against torch.unique's first index of each key."""
 
 from __future__ import annotations
 
@@ -28,9 +28,16 @@ def _bits(v):
     return ("f", struct.pack("<d", v)) if isinstance(v, float) else v
 
 
+def _key(v):
+    """A key value as cudf::distinct compares it: one NaN, -0.0 == 0.0, otherwise by bits."""
+    if isinstance(v, float):
+        return ("nan",) if v != v else _bits(v + 0.0)  # -0.0 + 0.0 == +0.0
+    return v
+
+
 def first_rows(t: pa.Table, keys) -> list:
     """Row index of each key tuple's first occurrence, in row order."""
-    cols = [[_bits(v) for v in t.column(k).to_pylist()] for k in keys]
+    cols = [[_key(v) for v in t.column(k).to_pylist()] for k in keys]
     seen, out = set(), []
     for i, key in enumerate(zip(*cols)):
         if key not in seen:
@@ -72,9 +79,13 @@ def _random_table(kind: str, n: int, seed: int) -> tuple[pa.Table, list]:
                       "k2": pa.array(list(words[rng.integers(0, len(words), n)]), mask=rng.random(n) < 0.03),
                       **payload})
         return t, ["k1", "k2"]
-    if kind == "float64_special":
-        vals = np.array([0.0, -0.0, np.nan, 1.5, -2.25, np.inf, -np.inf, 3.0])
-        t = pa.table({"k": pa.array(vals[rng.integers(0, len(vals), n)], mask=rng.random(n) < 0.05), **payload})
+    if kind in ("float64_special", "float32_special"):
+        vals = np.array([0.0, -0.0, np.nan, -np.nan, 1.5, -2.25, np.inf, -np.inf, 3.0])
+        dt = np.float64 if kind == "float64_special" else np.float32
+        kv = vals[rng.integers(0, len(vals), n)].astype(dt)
+        if dt == np.float64:  # a NaN with other payload bits
+            kv[rng.integers(0, n, 50)] = np.array([0x7FF0000000000123], np.uint64).view(np.float64)[0]
+        t = pa.table({"k": pa.array(kv, mask=rng.random(n) < 0.05), **payload})
         return t, ["k"]
     if kind == "int8_all_columns":
         t = pa.table({"k": pa.array(rng.integers(-128, 128, n).astype(np.int8)),
@@ -86,8 +97,8 @@ def _random_table(kind: str, n: int, seed: int) -> tuple[pa.Table, list]:
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["int64_nulls", "int32_string", "float64_special", "int8_all_columns",
-                                  "all_distinct"])
+@pytest.mark.parametrize("kind", ["int64_nulls", "int32_string", "float64_special", "float32_special",
+                                  "int8_all_columns", "all_distinct"])
 def test_distinct_matches_first_occurrence(gpu, kind):
     n = 200_000
     t, keys = _random_table(kind, n, seed=5)
