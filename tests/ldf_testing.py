@@ -4,6 +4,7 @@ from __future__ import annotations
 
 import json
 import math
+import struct
 from functools import lru_cache
 from pathlib import Path
 
@@ -148,3 +149,27 @@ def groupby_via_oracle(t: pa.Table, keys, aggs) -> pa.Table:
                 arr = pa.array(raw.view(npt or vt.to_pandas_dtype()), type=vt, mask=~valid)
         cols.append(arr)
     return pa.Table.from_arrays(cols, names=list(keys) + [o for _, _, o in aggs])
+
+
+def bits_of(v):
+    """A value for exact comparison: floats by their bits (so NaN == NaN, -0.0 != 0.0)."""
+    return ("f", struct.pack("<d", v)) if isinstance(v, float) else v
+
+
+def distinct_key_of(v):
+    """A key value as cudf::distinct compares it (nan_equality::ALL_EQUAL): one NaN, -0.0 == 0.0,
+    other floats by their bits; nulls (None) equal."""
+    if isinstance(v, float):
+        return ("nan",) if v != v else bits_of(v + 0.0)  # -0.0 + 0.0 == +0.0
+    return v
+
+
+def first_rows(t: pa.Table, keys) -> list:
+    """Row index of each distinct key tuple's first occurrence, in row order."""
+    cols = [[distinct_key_of(v) for v in t.column(k).to_pylist()] for k in keys]
+    seen, out = set(), []
+    for i, key in enumerate(zip(*cols)):
+        if key not in seen:
+            seen.add(key)
+            out.append(i)
+    return out

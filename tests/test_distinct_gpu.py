@@ -8,11 +8,12 @@ other payload bits included); at 50M rows, against torch.unique's first index of
 
 from __future__ import annotations
 
-import struct
-
 import numpy as np
 import pyarrow as pa
 import pytest
+
+from ldf_testing import bits_of as _bits
+from ldf_testing import first_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -22,28 +23,6 @@ def gpu_distinct(t: pa.Table, keys, high_cardinality=False) -> pa.Table:
     from legate_dataframe.lib.stream_compaction import distinct
 
     return distinct(LogicalTable.from_arrow(t), keys, high_cardinality=high_cardinality).to_arrow()
-
-
-def _bits(v):
-    return ("f", struct.pack("<d", v)) if isinstance(v, float) else v
-
-
-def _key(v):
-    """A key value as cudf::distinct compares it: one NaN, -0.0 == 0.0, otherwise by bits."""
-    if isinstance(v, float):
-        return ("nan",) if v != v else _bits(v + 0.0)  # -0.0 + 0.0 == +0.0
-    return v
-
-
-def first_rows(t: pa.Table, keys) -> list:
-    """Row index of each key tuple's first occurrence, in row order."""
-    cols = [[_key(v) for v in t.column(k).to_pylist()] for k in keys]
-    seen, out = set(), []
-    for i, key in enumerate(zip(*cols)):
-        if key not in seen:
-            seen.add(key)
-            out.append(i)
-    return out
 
 
 def assert_rows_equal(got: pa.Table, exp: pa.Table):
