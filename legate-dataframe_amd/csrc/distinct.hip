@@ -12,7 +12,8 @@
 // as cudf::distinct compares them there: a null equals a null, every NaN equals every NaN
 // (nan_equality::ALL_EQUAL) and -0.0 equals 0.0; other values by their bits.
 //
-// Layout in HBM: an open-addressing set of 2^k >= 2n 64-bit slots, each (high 32 bits of the
+// Layout in HBM: an open-addressing set of 64-bit slots (2^k >= n/4 first, for inputs of 2^20
+// rows or more; 2^k >= 2n when that set overflows or for smaller inputs), each (high 32 bits of the
 // row's 64-bit key hash << 32 | row index), installed with one atomicCAS; a row whose key is
 // already present lowers the slot's row with atomicMin (same hash bits, so the minimum is the
 // smaller row), so once every row is in, each key's slot holds its first row.  Slot values only
@@ -34,6 +35,8 @@ constexpr unsigned long long kDEmpty = ~0ull;  // row index 0xFFFFFFFF never occ
 constexpr int kDThreads              = 256;
 constexpr int kDRowsPerThread        = 16;
 constexpr int kDBlockRows            = kDThreads * kDRowsPerThread;
+constexpr int64_t kDSmallFrom        = int64_t(1) << 20;  // inputs from this size try the n/4-slot set first
+constexpr int kDProbeLimit           = 128;               // probes per row in that set before it gives up
 
 // Key bits with every NaN mapped to one NaN and -0.0 to 0.0 (other values unchanged).
 __device__ __forceinline__ uint64_t key_bits(const DCol& c, int64_t row)
@@ -79,17 +82,17 @@ __device__ inline bool keys_equal(const DKeys& k, int64_t ra, int64_t rb)
   return true;
 }
 
-// Inserts every row into the set; err = 1 when a probe sequence found no room (cannot happen at
-// load <= 1/2, kept as a guard).
+// Inserts every row into the set; err = 1 when a row found no room within `plimit` probes (with
+// the full-size set, plimit = its capacity and load <= 1/2: cannot happen, kept as a guard).
 __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t n, unsigned long long* __restrict__ set,
-                                                               uint64_t mask, int32_t* __restrict__ err)
+                                                               uint64_t mask, uint64_t plimit, int32_t* __restrict__ err)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t h              = key_hash(k, r);
     const unsigned long long mine = ((h >> 32) << 32) | (uint64_t)(uint32_t)r;
     uint64_t i                    = h & mask;
     bool placed                   = false;
-    for (uint64_t probes = 0; probes <= mask; ++probes) {
+    for (uint64_t probes = 0; probes < plimit; ++probes) {
       unsigned long long cur = set[i];
       if (cur == kDEmpty) {
         cur = atomicCAS(&set[i], kDEmpty, mine);
@@ -179,39 +182,46 @@ static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t n
   int64_t n_out = 0;
   Buf<int32_t> map;
   if (n > 0) {
-    const uint64_t cap  = next_pow2((uint64_t)(n > 32 ? 2 * n : 64));
-    const uint64_t mask = cap - 1;
-    Buf<unsigned long long> set(a, cap);
-    Buf<int32_t> err(a, 1);
-    fill_u64(reinterpret_cast<uint64_t*>(set.get()), kDEmpty, (int64_t)cap, s);
-    LDF_HIP_TRY(hipMemsetAsync(err.get(), 0, sizeof(int32_t), s));
-    {
-      ProfScope prof("distinct_insert", s);
-      hipLaunchKernelGGL(k_distinct_insert, dim3(grid_for(n, kDThreads)), dim3(kDThreads), 0, s, k, n, set.get(), mask,
-                         err.get());
-      LDF_HIP_TRY(hipGetLastError());
-    }
-    const int64_t nblk = (n + kDBlockRows - 1) / kDBlockRows;
+    // The set first gets n/4 slots (for up to ~n/8 keys at load <= 1/2; a quarter of the memory
+    // traffic of the full size) with a probe limit per row; if any row hits the limit (more keys),
+    // the whole pass runs again with 2n slots, which always hold every key.
+    const uint64_t full_cap = next_pow2((uint64_t)(n > 32 ? 2 * n : 64));
+    uint64_t cap            = n >= kDSmallFrom ? next_pow2((uint64_t)(n / 4)) : full_cap;
+    const int64_t nblk      = (n + kDBlockRows - 1) / kDBlockRows;
     Buf<uint8_t> keep(a, n);
     Buf<uint32_t> blk_cnt(a, nblk);
     Buf<uint64_t> blk_off(a, nblk + 1);
-    LDF_HIP_TRY(hipMemsetAsync(keep.get(), 0, (size_t)n, s));
-    hipLaunchKernelGGL(k_distinct_mark, dim3(grid_for((int64_t)cap, kDThreads)), dim3(kDThreads), 0, s,
-                       (const unsigned long long*)set.get(), cap, keep.get());
-    LDF_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_distinct_count, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),
-                       blk_cnt.get());
-    LDF_HIP_TRY(hipGetLastError());
-    exclusive_scan_u32_to_u64(blk_cnt.get(), blk_off.get(), nblk, a);
-    uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
-    st[0] = st[1] = 0;
-    LDF_HIP_TRY(hipMemcpyAsync(st, blk_off.get() + nblk, 8, hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipMemcpyAsync(st + 1, err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    LDF_HIP_TRY(hipStreamSynchronize(s));
-    n_out             = (int64_t)st[0];
-    const bool failed = st[1] != 0;
-    if (failed) throw Error(LDF_E_INTERNAL, "distinct: hash set overflow");
-    set.reset();
+    Buf<int32_t> err(a, 1);
+    while (true) {
+      const bool last = cap >= full_cap;
+      Buf<unsigned long long> set(a, cap);
+      fill_u64(reinterpret_cast<uint64_t*>(set.get()), kDEmpty, (int64_t)cap, s);
+      LDF_HIP_TRY(hipMemsetAsync(err.get(), 0, sizeof(int32_t), s));
+      {
+        ProfScope prof("distinct_insert", s);
+        hipLaunchKernelGGL(k_distinct_insert, dim3(grid_for(n, kDThreads)), dim3(kDThreads), 0, s, k, n, set.get(),
+                           cap - 1, last ? cap : (uint64_t)kDProbeLimit, err.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      LDF_HIP_TRY(hipMemsetAsync(keep.get(), 0, (size_t)n, s));
+      hipLaunchKernelGGL(k_distinct_mark, dim3(grid_for((int64_t)cap, kDThreads)), dim3(kDThreads), 0, s,
+                         (const unsigned long long*)set.get(), cap, keep.get());
+      LDF_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_distinct_count, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),
+                         blk_cnt.get());
+      LDF_HIP_TRY(hipGetLastError());
+      exclusive_scan_u32_to_u64(blk_cnt.get(), blk_off.get(), nblk, a);
+      uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
+      st[0] = st[1] = 0;
+      LDF_HIP_TRY(hipMemcpyAsync(st, blk_off.get() + nblk, 8, hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipMemcpyAsync(st + 1, err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      n_out             = (int64_t)st[0];
+      const bool failed = st[1] != 0;
+      if (!failed) break;
+      if (last) throw Error(LDF_E_INTERNAL, "distinct: hash set overflow");
+      cap = full_cap;
+    }
     map = Buf<int32_t>(a, n_out > 0 ? n_out : 1);
     if (n_out > 0) {
       hipLaunchKernelGGL(k_distinct_emit, dim3((unsigned)nblk), dim3(kDThreads), 0, s, n, (const uint8_t*)keep.get(),

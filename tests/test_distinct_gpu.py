@@ -122,3 +122,18 @@ def test_distinct_50M_rows_against_torch_unique(gpu):
     assert m == uniq.numel()
     assert torch.equal(out[0].data.view(torch.int64)[:m], k[first])
     assert torch.equal(out[1].data.view(torch.float64)[:m], v[first])
+
+
+def test_distinct_small_set_overflow_falls_back(gpu):
+    """2^21 rows, every key distinct: the first set (n/4 slots) cannot hold them, rows hit the probe
+    limit and the call runs again with the full-size set; every row is kept, in order."""
+    import torch
+
+    from legate_dataframe import _capi as C
+
+    n = 1 << 21
+    g = torch.Generator(device="cuda").manual_seed(13)
+    k = torch.randperm(n, device="cuda", generator=g).to(torch.int64) * 7 - 3
+    out = C.distinct([C.DeviceColumn(C.INT64, n, k.view(torch.uint8))], [0])
+    assert out[0].length == n
+    assert torch.equal(out[0].data.view(torch.int64)[:n], k)
