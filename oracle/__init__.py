@@ -3,7 +3,9 @@
 ctypes wrapper of liboracle.so, the single-threaded C restatement of the reference's CPU task
 variants for the join / groupby / hash-partition path (see oracle.c for the file:line each
 function follows).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
-this package; the product never imports it.
+this package; the product never imports it.  `distinct_first_rows` (pure Python, small inputs) is
+the restatement for `distinct`; it is pinned by the reference's own test case and by
+pandas.DataFrame.duplicated(keep="first") in tests/test_distinct_checker.py.
 
 Columns are given as :class:`OCol` (numpy values or a list of str, an optional numpy bool
 validity array, and the Arrow type id).
@@ -12,6 +14,7 @@ validity array, and the Arrow type id).
 from __future__ import annotations
 
 import ctypes
+import struct
 import subprocess
 from pathlib import Path
 from typing import List, Optional, Sequence, Tuple
@@ -177,3 +180,26 @@ def partition(cols: Sequence[OCol], nparts: int, seed: int = 0) -> Tuple[np.ndar
     if load().orc_partition(_cols(cols), len(cols), n, nparts, seed, perm.ctypes.data, offs.ctypes.data):
         raise MemoryError("oracle partition failed")
     return perm[:n], offs
+
+
+def _distinct_key(v):
+    """A key value as cudf::distinct compares it with null_equality::EQUAL and
+    nan_equality::ALL_EQUAL (cpp/src/stream_compaction.cu:63-71 in the reference): nulls (None)
+    equal, every NaN one key, -0.0 equal to 0.0, other floats by their bits."""
+    if isinstance(v, float):
+        if v != v:
+            return ("nan",)
+        return ("f", struct.pack("<d", v + 0.0))  # -0.0 + 0.0 == +0.0
+    return v
+
+
+def distinct_first_rows(key_columns: Sequence[Sequence]) -> List[int]:
+    """Row index of each distinct key tuple's first occurrence, in row order: the rows
+    DistinctTask keeps (cpp/src/stream_compaction.cu:38-72; KEEP_ANY there, the first row of each
+    key here).  `key_columns` are Python lists (pyarrow to_pylist())."""
+    seen, out = set(), []
+    for i, key in enumerate(zip(*[[_distinct_key(v) for v in c] for c in key_columns])):
+        if key not in seen:
+            seen.add(key)
+            out.append(i)
+    return out

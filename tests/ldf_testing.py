@@ -156,20 +156,8 @@ def bits_of(v):
     return ("f", struct.pack("<d", v)) if isinstance(v, float) else v
 
 
-def distinct_key_of(v):
-    """A key value as cudf::distinct compares it (nan_equality::ALL_EQUAL): one NaN, -0.0 == 0.0,
-    other floats by their bits; nulls (None) equal."""
-    if isinstance(v, float):
-        return ("nan",) if v != v else bits_of(v + 0.0)  # -0.0 + 0.0 == +0.0
-    return v
-
-
 def first_rows(t: pa.Table, keys) -> list:
-    """Row index of each distinct key tuple's first occurrence, in row order."""
-    cols = [[distinct_key_of(v) for v in t.column(k).to_pylist()] for k in keys]
-    seen, out = set(), []
-    for i, key in enumerate(zip(*cols)):
-        if key not in seen:
-            seen.add(key)
-            out.append(i)
-    return out
+    """Row index of each distinct key tuple's first occurrence, in row order (the oracle's)."""
+    from oracle import distinct_first_rows
+
+    return distinct_first_rows([t.column(k).to_pylist() for k in keys])
