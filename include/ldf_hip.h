@@ -219,7 +219,13 @@ int ldf_murmur3_hash(const ldf_table_view* table,
 
 /* Legate-layout adapters (cpp/src/core/null_mask.cu:38-114, cpp/src/core/ranges.cu:33-138):
  * one byte per row validity <-> LSB bitmap, Rect<1>{lo,hi} (inclusive, global) string ranges
- * <-> local int64 offsets. */
+ * <-> local int64 offsets.
+ * ldf_bools_to_bitmask writes the (n+7)/8 bitmap bytes (bits past n in the last byte are 0) and,
+ * when null_count_dev (device memory) is not NULL, stores the count of zero bools there.
+ * ldf_ranges_to_offsets: offsets[i] = ranges[i].lo - ranges[0].lo, offsets[n] = ranges[n-1].hi + 1
+ * - ranges[0].lo (an empty string has hi = lo - 1); n + 1 offsets are written (offsets[0] = 0
+ * when n = 0).  ldf_offsets_to_ranges: ranges[i] = {base + offsets[i], base + offsets[i+1] - 1}
+ * (base 0 gives the reference's local ranges, cudf_offsets_to_local_ranges). */
 int ldf_bools_to_bitmask(const uint8_t* bools, int64_t n, uint8_t* bitmask, int64_t* null_count_dev,
                          void* stream);
 int ldf_bitmask_to_bools(const uint8_t* bitmask, int64_t n, uint8_t* bools, void* stream);
@@ -243,6 +249,17 @@ void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream
  * blocks (blocks in use stay allocated).  No counterpart in the reference, whose task memory
  * comes from Legate's pool (cpp/src/core/allocator.cu:27-81). */
 void ldf_pool_release(void);
+
+/* The default allocator's own entry points, for a host layer that owns buffers the library
+ * returned with a NULL allocator (the single-owner hand-off of cpp/src/core/column.cu:64-70 and
+ * cpp/src/core/allocator.cu:60-81: the buffer is released to the allocator that made it).
+ * ldf_pool_alloc returns a block of at least `bytes` (256-byte aligned) usable on `stream`, or
+ * NULL when the device is out of memory; ldf_pool_free returns a block of the default allocator
+ * to its cache for reuse in `stream`'s order (other pointers are ignored).  ldf_pool_stats
+ * reports the blocks in use and the cached blocks with their bytes (any pointer may be NULL). */
+void* ldf_pool_alloc(size_t bytes, void* stream);
+void ldf_pool_free(void* ptr, void* stream);
+void ldf_pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_blocks, int64_t* cached_bytes);
 
 /* ---- batched form -------------------------------------------------------------------------
  * Many independent joins or groupbys of one schema (the per-partition point tasks a Legate

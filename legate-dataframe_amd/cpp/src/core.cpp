@@ -35,16 +35,20 @@ void check_status(int rc)
   throw HipError(rc, msg);
 }
 
+// Column buffers come from the library's default (caching) allocator, the same one that makes
+// the buffers of the operators' outputs (called with a NULL ldf_allocator), and go back to it:
+// one owner per buffer, as the reference's allocator hands its buffers to the column it binds
+// (cpp/src/core/column.cu:64-70, cpp/src/core/allocator.cu:60-81).
 void* device_alloc(size_t bytes)
 {
-  void* p = nullptr;
-  hip_check(hipMallocAsync(&p, bytes ? bytes : 1, (hipStream_t)g_stream), "hipMallocAsync");
+  void* p = ldf_pool_alloc(bytes ? bytes : 1, g_stream);
+  if (p == nullptr) throw HipError(LDF_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
   return p;
 }
 
 void device_free(void* p)
 {
-  if (p) (void)hipFreeAsync(p, (hipStream_t)g_stream);
+  if (p) ldf_pool_free(p, g_stream);
 }
 
 void copy_to_device(void* dst, const void* src, size_t bytes)

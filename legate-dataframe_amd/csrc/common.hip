@@ -110,6 +110,26 @@ void pool_deallocate(void* p, hipStream_t s)
   P.live.erase(it);
 }
 
+void pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_blocks, int64_t* cached_bytes)
+{
+  DevicePool& P = device_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  int64_t lb = 0, lB = 0, cb = 0, cB = 0;
+  for (auto& kv : P.live) {
+    ++lb;
+    lB += (int64_t)kv.second.second;
+  }
+  for (auto& per_stream : P.free_blocks)
+    for (auto& cls : per_stream.second) {
+      cb += (int64_t)cls.second.size();
+      cB += (int64_t)(cls.first * cls.second.size());
+    }
+  if (live_blocks) *live_blocks = lb;
+  if (live_bytes) *live_bytes = lB;
+  if (cached_blocks) *cached_blocks = cb;
+  if (cached_bytes) *cached_bytes = cB;
+}
+
 void pool_release_all()
 {
   DevicePool& P = device_pool();
@@ -881,23 +901,24 @@ void gather_column(const ldf_column_view& src, const int32_t* map, int64_t n_out
 // ------------------------------------------------------------------------------------------
 // Legate-layout adapters
 // ------------------------------------------------------------------------------------------
+// One wave per 64 rows: a ballot of the row bytes is the bitmap word; lanes 0..7 store its bytes
+// (only the (n+7)/8 bytes of the bitmap are written) and lane 0 counts the word's nulls.
 __global__ void k_bools_to_bitmask(const uint8_t* __restrict__ bools, int64_t n, uint8_t* __restrict__ mask,
                                    unsigned long long* null_count)
 {
-  uint64_t nulls = 0;
-  int64_t nbytes = (n + 7) / 8;
-  for (int64_t byte = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; byte < nbytes; byte += (int64_t)gridDim.x * blockDim.x) {
-    uint8_t m = 0;
-    for (int b = 0; b < 8; ++b) {
-      int64_t i = byte * 8 + b;
-      if (i < n) {
-        if (bools[i]) m |= (uint8_t)(1u << b);
-        else ++nulls;
-      }
-    }
-    mask[byte] = m;
+  const int lane         = threadIdx.x & (kWave - 1);
+  const int64_t nwords   = (n + 63) / 64;
+  const int64_t nwaves   = (int64_t)gridDim.x * (blockDim.x / kWave);
+  unsigned long long nul = 0;
+  for (int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave; w < nwords; w += nwaves) {
+    const int64_t i     = w * 64 + lane;
+    const uint64_t bits = __ballot(i < n && bools[i] != 0);
+    const int64_t rem   = n - w * 64;  // rows in this word (1..64)
+    const int nbytes    = rem >= 64 ? 8 : (int)((rem + 7) / 8);
+    if (lane < nbytes) mask[w * 8 + lane] = (uint8_t)(bits >> (8 * lane));
+    if (lane == 0) nul += (unsigned long long)((rem >= 64 ? 64 : rem) - __popcll(bits));
   }
-  if (null_count && nulls) atomicAdd(null_count, (unsigned long long)nulls);
+  if (null_count && lane == 0 && nul) atomicAdd(null_count, nul);
 }
 
 __global__ void k_bitmask_to_bools(const uint8_t* __restrict__ mask, int64_t n, uint8_t* __restrict__ bools)
@@ -968,13 +989,33 @@ void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream
 
 void ldf_pool_release(void) { pool_release_all(); }
 
+void* ldf_pool_alloc(size_t bytes, void* stream)
+{
+  try {
+    return pool_allocate(bytes ? bytes : 1, (hipStream_t)stream);
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void ldf_pool_free(void* ptr, void* stream)
+{
+  if (ptr) pool_deallocate(ptr, (hipStream_t)stream);
+}
+
+void ldf_pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_blocks, int64_t* cached_bytes)
+{
+  pool_stats(live_blocks, live_bytes, cached_blocks, cached_bytes);
+}
+
 int ldf_bools_to_bitmask(const uint8_t* bools, int64_t n, uint8_t* bitmask, int64_t* null_count_dev, void* stream)
 {
   return guarded([&] {
     LDF_EXPECTS(n >= 0, "negative length");
+    if (null_count_dev) fill_u64(reinterpret_cast<uint64_t*>(null_count_dev), 0, 1, (hipStream_t)stream);
     if (n == 0) return;
-    hipLaunchKernelGGL(k_bools_to_bitmask, dim3(grid_for((n + 7) / 8, 256)), dim3(256), 0, (hipStream_t)stream, bools, n,
-                       bitmask, (unsigned long long*)null_count_dev);
+    hipLaunchKernelGGL(k_bools_to_bitmask, dim3(grid_for((n + 63) / 64 * 64, 256)), dim3(256), 0, (hipStream_t)stream,
+                       bools, n, bitmask, (unsigned long long*)null_count_dev);
     LDF_HIP_TRY(hipGetLastError());
   });
 }

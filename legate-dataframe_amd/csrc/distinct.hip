@@ -42,13 +42,7 @@ constexpr int kDProbeLimit           = 128;               // probes per row in t
 __device__ __forceinline__ uint64_t key_bits(const DCol& c, int64_t row)
 {
   const uint64_t b = load_bits(c.data, c.width, row);
-  if (!is_float_type(c.type)) return b;
-  if (c.width == 4) {
-    const float f = __uint_as_float((uint32_t)b);
-    return f != f ? 0x7fc00000ull : (f == 0.0f ? 0ull : b);
-  }
-  const double d = __longlong_as_double((long long)b);
-  return d != d ? 0x7ff8000000000000ull : (d == 0.0 ? 0ull : b);
+  return is_float_type(c.type) ? murmur_input_bits(b, c.width) : b;
 }
 
 __device__ inline uint64_t key_hash(const DKeys& k, int64_t row)

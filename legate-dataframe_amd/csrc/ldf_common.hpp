@@ -89,6 +89,7 @@ class ProfScope {
 void* pool_allocate(size_t bytes, hipStream_t s);
 void pool_deallocate(void* p, hipStream_t s);
 void pool_release_all();
+void pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_blocks, int64_t* cached_bytes);
 
 class Alloc {
  public:

@@ -79,6 +79,23 @@ __device__ __forceinline__ uint32_t murmur3_fixed(uint64_t bits, int width, uint
   return fmix32(h);
 }
 
+// Value bits as libcudf's MurmurHash3_x86_32 hashes a floating-point element: every NaN becomes
+// the canonical quiet NaN and -0.0 becomes 0.0 first (its normalize_nans_and_zeros), so values
+// that compare equal land in the same partition.  `fwidth` is 4 (float), 8 (double) or 0 (not a
+// floating-point column: bits unchanged).
+__device__ __forceinline__ uint64_t murmur_input_bits(uint64_t bits, int fwidth)
+{
+  if (fwidth == 4) {
+    const float f = __uint_as_float((uint32_t)bits);
+    return f != f ? 0x7fc00000ull : (f == 0.0f ? 0ull : bits);
+  }
+  if (fwidth == 8) {
+    const double d = __longlong_as_double((long long)bits);
+    return d != d ? 0x7ff8000000000000ull : (d == 0.0 ? 0ull : bits);
+  }
+  return bits;
+}
+
 // MurmurHash3_x86_32 over an arbitrary byte string.
 __device__ inline uint32_t murmur3_bytes(const uint8_t* p, int64_t len, uint32_t seed)
 {

@@ -4,7 +4,8 @@
 //   cudf::hash_partition(table, cols, nparts, HASH_MURMUR3, DEFAULT_HASH_SEED)
 // and splits the result at the returned offsets.  The row hash restated here is libcudf's:
 // h = seed; for each hashed column h = hash_combine(h, MurmurHash3_x86_32(value, seed)),
-// a null element hashing to 0xFFFFFFFF; the partition is h mod nparts.
+// a null element hashing to 0xFFFFFFFF, a float/double element hashed after its NaNs are made
+// the canonical quiet NaN and -0.0 is made 0.0; the partition is h mod nparts.
 //
 // Kernels (all HBM-streaming, no MFMA):
 //   k_hp_hist    block b counts the partition ids of its contiguous row range in LDS and
@@ -47,7 +48,8 @@ __device__ __forceinline__ uint32_t partition_row_hash(const HashCols& hc, int64
       int64_t b = c.offsets[row], en = c.offsets[row + 1];
       e = murmur3_bytes(static_cast<const uint8_t*>(c.data) + b, en - b, hc.seed);
     } else {
-      e = murmur3_fixed(load_bits(c.data, c.width, row), c.width, hc.seed);
+      const int fw = c.type == LDF_TYPE_FLOAT ? 4 : (c.type == LDF_TYPE_DOUBLE ? 8 : 0);
+      e = murmur3_fixed(murmur_input_bits(load_bits(c.data, c.width, row), fw), c.width, hc.seed);
     }
     h = hash_combine32(h, e);
   }
@@ -77,8 +79,9 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_hist(HashCols hc, int64_t n, 
 }
 
 // The histogram of one fixed-width non-null column hashed on itself: eight rows per thread
-// have their loads in flight together.
-template <typename T>
+// have their loads in flight together.  FW = 4 / 8 for a float / double column (its bits are
+// normalised before hashing), else 0.
+template <typename T, int FW>
 __global__ __launch_bounds__(kHpThreads) void k_hp_hist_v(const T* __restrict__ src, int64_t n, uint32_t seed,
                                                           int64_t rows_per_block, uint32_t nparts, FastMod fm,
                                                           uint32_t* __restrict__ counts)
@@ -99,7 +102,8 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_hist_v(const T* __restrict__ 
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if (r0 + (int64_t)j * blockDim.x >= end) break;
-      const uint32_t p = fastmod(hash_combine32(seed, murmur3_fixed((uint64_t)v[j], (int)sizeof(T), seed)), fm);
+      const uint32_t p =
+        fastmod(hash_combine32(seed, murmur3_fixed(murmur_input_bits((uint64_t)v[j], FW), (int)sizeof(T), seed)), fm);
       atomicAdd(&hist[p], 1u);
     }
   }
@@ -248,7 +252,7 @@ constexpr int kVSub     = 4096;
 constexpr int kVPerWave = kVSub / kVWaves;  // 512 contiguous rows per wave
 constexpr int kVGroups  = kVPerWave / kWave;
 
-template <typename T>
+template <typename T, int FW>
 __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restrict__ src, int64_t n, uint32_t seed,
                                                             int64_t rows_per_block, uint32_t nparts, FastMod fm, int pbits,
                                                             const uint64_t* __restrict__ offsets, T* __restrict__ dst)
@@ -299,7 +303,8 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
       const bool active = local < sub_n;
       uint32_t p        = 0;
       if (active)
-        p = fastmod(hash_combine32(seed, murmur3_fixed((uint64_t)my_val[g], (int)sizeof(T), seed)), fm);
+        p = fastmod(hash_combine32(seed, murmur3_fixed(murmur_input_bits((uint64_t)my_val[g], FW), (int)sizeof(T), seed)),
+                    fm);
       uint64_t same = __ballot(active);
       for (int b = 0; b < pbits; ++b) {
         uint64_t bb = __ballot(active && ((p >> b) & 1));
@@ -407,17 +412,17 @@ static size_t scatter_v_lds_bytes(uint32_t nparts, int width)
 }
 constexpr size_t kHpVLdsMax = 150 * 1024;
 
-template <typename T>
+template <typename T, int FW>
 static void launch_scatter_v(const ldf_column_view& c, int64_t n, uint32_t seed, int64_t rpb, int64_t nblk,
                              uint32_t nparts, int pbits, const uint64_t* scan, void* dst, hipStream_t s)
 {
   static bool attr = false;
   if (!attr) {
-    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T, FW>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)kHpVLdsMax));
     attr = true;
   }
-  hipLaunchKernelGGL(k_hp_scatter_v<T>, dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)), s,
+  hipLaunchKernelGGL((k_hp_scatter_v<T, FW>), dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)), s,
                      static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
                      static_cast<T*>(dst));
   LDF_HIP_TRY(hipGetLastError());
@@ -492,6 +497,7 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
     Buf<uint64_t> scan(a, (size_t)nparts * nblk + 1);
     // one fixed-width non-null column hashed on itself: the value-staged kernels
     const int w0      = type_width(table->cols[0].type_id);
+    const int fw0     = table->cols[0].type_id == LDF_TYPE_FLOAT ? 4 : (table->cols[0].type_id == LDF_TYPE_DOUBLE ? 8 : 0);
     const bool single = direct && table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
                         scatter_v_lds_bytes((uint32_t)nparts, w0) <= kHpVLdsMax;
     const FastMod fm  = fastmod_make((uint32_t)nparts);
@@ -502,17 +508,23 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       if (!single)
         hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), hl, s, hc, n, rpb, (uint32_t)nparts, fm, counts.get());
       else if (w0 == 1)
-        hipLaunchKernelGGL(k_hp_hist_v<uint8_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint8_t*)d0, n, seed, rpb,
-                           (uint32_t)nparts, fm, counts.get());
+        hipLaunchKernelGGL((k_hp_hist_v<uint8_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint8_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
       else if (w0 == 2)
-        hipLaunchKernelGGL(k_hp_hist_v<uint16_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint16_t*)d0, n, seed, rpb,
-                           (uint32_t)nparts, fm, counts.get());
+        hipLaunchKernelGGL((k_hp_hist_v<uint16_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint16_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
+      else if (w0 == 4 && fw0 == 4)
+        hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 4>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
       else if (w0 == 4)
-        hipLaunchKernelGGL(k_hp_hist_v<uint32_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed, rpb,
-                           (uint32_t)nparts, fm, counts.get());
+        hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
+      else if (fw0 == 8)
+        hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 8>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
       else
-        hipLaunchKernelGGL(k_hp_hist_v<uint64_t>, dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed, rpb,
-                           (uint32_t)nparts, fm, counts.get());
+        hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
+                           rpb, (uint32_t)nparts, fm, counts.get());
       LDF_HIP_TRY(hipGetLastError());
     }
     exclusive_scan_u32_to_u64(counts.get(), scan.get(), (int64_t)nparts * nblk, a);
@@ -542,12 +554,14 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
     ProfScope prof("hp_scatter", s);
     if (single) {
       const uint64_t* sp = (const uint64_t*)scan.get();
-      switch (sc.width[0]) {
-        case 1: launch_scatter_v<uint8_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
-        case 2: launch_scatter_v<uint16_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
-        case 4: launch_scatter_v<uint32_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
-        default: launch_scatter_v<uint64_t>(table->cols[0], n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s); break;
-      }
+      const ldf_column_view& c0 = table->cols[0];
+      if (sc.width[0] == 1) launch_scatter_v<uint8_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
+      else if (sc.width[0] == 2) launch_scatter_v<uint16_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
+      else if (sc.width[0] == 4 && fw0 == 4)
+        launch_scatter_v<uint32_t, 4>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
+      else if (sc.width[0] == 4) launch_scatter_v<uint32_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
+      else if (fw0 == 8) launch_scatter_v<uint64_t, 8>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
+      else launch_scatter_v<uint64_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
     } else {
       hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, fm, pbits,
                          (const uint64_t*)scan.get(), sc, perm.get());

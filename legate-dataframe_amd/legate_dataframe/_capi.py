@@ -30,7 +30,8 @@ EXPORTED = [
     "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_distinct",
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
-    "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release",
+    "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release", "ldf_pool_alloc",
+    "ldf_pool_free", "ldf_pool_stats",
     "ldf_batch_join_create", "ldf_batch_groupby_create", "ldf_batch_run", "ldf_batch_result",
     "ldf_batch_config_used", "ldf_batch_destroy", "ldf_server_create", "ldf_server_add_table",
     "ldf_server_submit", "ldf_server_wait", "ldf_server_release", "ldf_server_destroy",
@@ -119,6 +120,12 @@ def load(require: bool = True):
         L.ldf_offsets_to_ranges.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p]
         L.ldf_free_table.argtypes = [P(TableOut), P(Allocator), ctypes.c_void_p]
+        L.ldf_pool_alloc.argtypes = [ctypes.c_size_t, ctypes.c_void_p]
+        L.ldf_pool_alloc.restype = ctypes.c_void_p
+        L.ldf_pool_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ldf_pool_free.restype = None
+        L.ldf_pool_stats.argtypes = [P(ctypes.c_int64)] * 4
+        L.ldf_pool_stats.restype = None
         L.ldf_profile_enable.argtypes = [ctypes.c_int]
         L.ldf_profile_get.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_double)]
         L.ldf_profile_get.restype = ctypes.c_int
@@ -438,6 +445,13 @@ def hash_partition_raw(table, cols, nparts: int, seed: int = 0):
 def pool_release() -> None:
     """Returns the default allocator's cached device blocks to the device."""
     load().ldf_pool_release()
+
+
+def pool_stats() -> dict:
+    """Blocks in use and cached blocks (with bytes) of the default allocator."""
+    v = [ctypes.c_int64() for _ in range(4)]
+    load().ldf_pool_stats(*[ctypes.byref(x) for x in v])
+    return dict(live_blocks=v[0].value, live_bytes=v[1].value, cached_blocks=v[2].value, cached_bytes=v[3].value)
 
 
 def profile_enable(on: bool = True) -> None:

@@ -24,8 +24,12 @@
  *   orc_partition   cudf::hash_partition(HASH_MURMUR3, seed) as called by
  *                   cpp/src/core/repartition_by_hash.cu:296-302: row hash = fold of
  *                   hash_combine(h, murmur3(value, seed)) from h = seed over the hashed
- *                   columns (a null element hashes to 0xFFFFFFFF); partition = hash mod P;
- *                   rows keep input order within a partition.
+ *                   columns (a null element hashes to 0xFFFFFFFF; a float/double element is
+ *                   hashed after NaN -> canonical quiet NaN and -0.0 -> 0.0); partition =
+ *                   hash mod P; rows keep input order within a partition.  The murmur3 and
+ *                   float normalisation are pinned by published vectors; the hash_combine
+ *                   fold, its seed start and the null sentinel are parity unpinned (libcudf,
+ *                   where they live, is absent and the reference's tests hold no vectors).
  */
 #include <math.h>
 #include <stdint.h>
@@ -559,6 +563,20 @@ uint32_t orc_row_hash(const orc_col* cols, int nc, int64_t r, uint32_t seed)
       e = orc_murmur3_32((const uint8_t*)c->data + c->offsets[r], c->offsets[r + 1] - c->offsets[r], seed);
     else {
       uint64_t b = bits_at(c, r);
+      /* libcudf's MurmurHash3_x86_32<float/double> hashes normalize_nans_and_zeros(value):
+       * any NaN -> the canonical quiet NaN, -0.0 -> 0.0 */
+      if (c->type == T_FLOAT) {
+        float f;
+        uint32_t w = (uint32_t)b;
+        memcpy(&f, &w, 4);
+        if (f != f) b = 0x7fc00000u;
+        else if (f == 0.0f) b = 0;
+      } else if (c->type == T_DOUBLE) {
+        double d;
+        memcpy(&d, &b, 8);
+        if (d != d) b = 0x7ff8000000000000ull;
+        else if (d == 0.0) b = 0;
+      }
       uint8_t buf[8];
       memcpy(buf, &b, 8); /* little endian: the value's own bytes */
       e = orc_murmur3_32(buf, c->width, seed);

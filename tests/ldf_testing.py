@@ -161,3 +161,89 @@ def first_rows(t: pa.Table, keys) -> list:
     from oracle import distinct_first_rows
 
     return distinct_first_rows([t.column(k).to_pylist() for k in keys])
+
+
+# ---------------------------------------------------------------------------------------------
+# order-independent fingerprints of device tables (for results too large to sort on the host)
+# ---------------------------------------------------------------------------------------------
+_M1 = 0xBF58476D1CE4E5B9 - (1 << 64)
+_M2 = 0x94D049BB133111EB - (1 << 64)
+_G = 0x9E3779B97F4A7C15 - (1 << 64)
+
+
+def _s64(x):
+    """A Python int wrapped to a signed 64-bit value."""
+    x &= (1 << 64) - 1
+    return x - (1 << 64) if x >= 1 << 63 else x
+
+
+def _lsr(torch, x, s):
+    return (x >> s) & ((1 << (64 - s)) - 1)
+
+
+def _mix(torch, x):
+    """splitmix64's finaliser on int64 tensors (wrap-around arithmetic)."""
+    x = x ^ _lsr(torch, x, 30)
+    x = x * _M1
+    x = x ^ _lsr(torch, x, 27)
+    x = x * _M2
+    return x ^ _lsr(torch, x, 31)
+
+
+def column_values_i64(torch, col, lo, hi):
+    """Rows [lo, hi) of a fixed-width DeviceColumn as int64 (raw bits, zero-extended) and a bool
+    validity tensor."""
+    w = {1: torch.uint8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[_width(col)]
+    v = col.data[lo * _width(col): hi * _width(col)].view(w).to(torch.int64)
+    if _width(col) == 4:
+        v = v & 0xFFFFFFFF
+    elif _width(col) == 2:
+        v = v & 0xFFFF
+    if col.validity is None:
+        ok = torch.ones(hi - lo, dtype=torch.bool, device=v.device)
+    else:
+        idx = torch.arange(lo, hi, device=v.device)
+        ok = ((col.validity[idx >> 3].to(torch.int64) >> (idx & 7)) & 1).bool()
+    return v, ok
+
+
+def _width(col):
+    from legate_dataframe import _capi as C
+
+    return C.WIDTH[col.type_id]
+
+
+def table_fingerprint(cols, offsets=None, chunk=1 << 26):
+    """Per segment of rows (one segment, or offsets[i]:offsets[i+1]): the row count and the
+    wrap-around sums of two independent 64-bit hashes of each row's values (null-aware).  Equal
+    fingerprints mean equal row multisets up to a hash collision.  Returns an int64 array
+    [nseg, 3]."""
+    import torch
+
+    n = cols[0].length
+    if offsets is None:
+        offsets = np.array([0, n], np.int64)
+    nseg = len(offsets) - 1
+    dev = cols[0].data.device
+    out = torch.zeros((nseg, 3), dtype=torch.int64, device=dev)
+    offs_t = torch.from_numpy(np.asarray(offsets, np.int64)).to(dev)
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk)
+        h = torch.zeros(hi - lo, dtype=torch.int64, device=dev)
+        for j, c in enumerate(cols):
+            v, ok = column_values_i64(torch, c, lo, hi)
+            v = torch.where(ok, v, torch.full_like(v, 0x5BD1E995 + j))
+            h = _mix(torch, h * 31 + v + _s64(_G * (j + 1)) + ok.to(torch.int64))
+        h2 = _mix(torch, h ^ 0x2545F4914F6CDD1D)
+        seg = torch.searchsorted(offs_t, torch.arange(lo, hi, device=dev), right=True) - 1
+        out[:, 0].index_add_(0, seg, torch.ones_like(h))
+        out[:, 1].index_add_(0, seg, h)
+        out[:, 2].index_add_(0, seg, h2)
+    return out.cpu().numpy()
+
+
+def device_columns_from_numpy(arrays):
+    """[(values, type_id, valid or None)] -> DeviceColumns (for fingerprints of host tables)."""
+    from legate_dataframe import _capi as C
+
+    return [C.column_from_numpy(v, t, ok) for v, t, ok in arrays]

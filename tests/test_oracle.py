@@ -67,6 +67,29 @@ def test_row_hash_composition():
     assert [int(v) for v in got] == [_combine(0, oracle.murmur3_32(t.encode(), 0)) for t in s]
 
 
+def test_row_hash_float_normalisation_published_vectors():
+    """libcudf's MurmurHash3_x86_32<float/double> hashes normalize_nans_and_zeros(value): -0.0
+    hashes as the bytes of 0.0, whose murmur3 is a published vector (4 zero bytes, seed 0:
+    0x2362F9DE), and every NaN hashes as the canonical quiet NaN's bytes."""
+    from oracle import T_FLOAT
+    f32 = np.array([0.0, -0.0], np.float32)
+    h = oracle.row_hashes([OCol(T_FLOAT, f32)], 0)
+    assert [int(v) for v in h] == [_combine(0, 0x2362F9DE)] * 2
+    nans32 = np.array([0x7FC00000, 0xFFC00000, 0x7F800001, 0x7FC12345, 0xFFFFFFFF], np.uint32).view(np.float32)
+    h = oracle.row_hashes([OCol(T_FLOAT, nans32)], 0)
+    assert len({int(v) for v in h}) == 1
+    assert int(h[0]) == _combine(0, oracle.murmur3_32(struct.pack("<I", 0x7FC00000), 0))
+    nans64 = np.array([0x7FF8000000000000, 0xFFF8000000000000, 0x7FF0000000000001, 0x7FF8DEADBEEF0001],
+                      np.uint64).view(np.float64)
+    d = np.concatenate([nans64, np.array([0.0, -0.0, 1.5, -1.5])])
+    h = oracle.row_hashes([OCol(T_DOUBLE, d)], 0)
+    assert len({int(v) for v in h[:4]}) == 1 and h[4] == h[5] and h[6] != h[7]
+    assert int(h[0]) == _combine(0, oracle.murmur3_32(struct.pack("<Q", 0x7FF8000000000000), 0))
+    assert int(h[4]) == _combine(0, oracle.murmur3_32(b"\0" * 8, 0))
+    # other values hash their own bytes
+    assert int(h[6]) == _combine(0, oracle.murmur3_32(struct.pack("<d", 1.5), 0))
+
+
 def test_partition_contract():
     rng = np.random.default_rng(1)
     x = rng.integers(-(2**63), 2**63 - 1, 5000, dtype=np.int64)

@@ -54,6 +54,44 @@ def test_partition_nullable_multicolumn_strings(gpu):
         assert got.equals(exp), cols
 
 
+@pytest.mark.parametrize("nullable", [False, True])
+def test_partition_float_keys_nan_and_signed_zero(gpu, nullable):
+    """Float keys as libcudf hashes them: NaNs of any payload share a partition, -0.0 shares 0.0's
+    (both the single-column value-staged path and the generic path), exact against the oracle."""
+    rng = np.random.default_rng(9)
+    n = 120_000
+    nan_bits = np.array([0x7FF8000000000000, 0xFFF8000000000000, 0x7FF0000000000001, 0x7FF8DEADBEEF0001],
+                        np.uint64)
+    d = rng.normal(size=n)
+    pick = rng.integers(0, 6, n)
+    d[pick == 0] = nan_bits[rng.integers(0, 4, n)].view(np.float64)[pick == 0]
+    d[pick == 1] = 0.0
+    d[pick == 2] = -0.0
+    f = d.astype(np.float32)
+    f32_nans = np.array([0x7FC00000, 0xFFC00000, 0x7F800001, 0x7FC12345], np.uint32).view(np.float32)
+    f[pick == 0] = f32_nans[rng.integers(0, 4, n)][pick == 0]
+    mask = (rng.random(n) < 0.05) if nullable else None
+    t = pa.table({"d": pa.array(d, mask=mask), "f": pa.array(f, mask=mask)})
+    for cols, tab in ((["d"], t.select(["d"])), (["f"], t.select(["f"])), (["d"], t), (["f", "d"], t)):
+        got, offs = gpu_partition(tab, cols, 64, 0)
+        exp, eoffs = _expected(tab, cols, 64, 0)
+        assert list(offs) == list(eoffs), cols
+        for c in tab.column_names:  # compare bits: NaN payloads must travel unchanged
+            gb = got[c].combine_chunks()
+            eb = exp[c].combine_chunks()
+            assert gb.is_valid().equals(eb.is_valid())
+            w = np.uint64 if c == "d" else np.uint32
+            assert np.array_equal(gb.fill_null(0).to_numpy().view(w), eb.fill_null(0).to_numpy().view(w)), (cols, c)
+        # rows equal under libcudf's comparison share a partition
+        part = np.repeat(np.arange(64), np.diff(offs))
+        key = got[cols[0]].combine_chunks()
+        vals = key.fill_null(0).to_numpy()
+        valid = np.asarray(key.is_valid())
+        if len(cols) == 1:
+            for sel in (np.isnan(vals) & valid, (vals == 0) & valid):
+                assert len(set(part[sel].tolist())) == 1
+
+
 def test_row_hash_matches_oracle(gpu):
     import torch
 
@@ -84,20 +122,26 @@ def test_partition_empty_and_tiny(gpu):
     assert list(offs) == list(eoffs) and got.equals(exp)
 
 
-def test_partition_200m_keys_invariants(gpu):
-    """200M int64 keys into 256 partitions (a fifth of one GPU's 1B-key share): offsets are
-    monotone and sum to n, every key lands in its murmur3 partition, the multiset of keys is
-    preserved (sum and xor of keys), and one partition is compared against the oracle exactly."""
+@pytest.mark.parametrize("n", [200_000_000, 1_000_000_000])
+def test_partition_keys_invariants_at_scale(gpu, n):
+    """200M and 1B int64 keys (one GPU's whole share of the shuffle) into 256 partitions: offsets
+    are monotone and sum to n, every key lands in its murmur3 partition (the device row hash is
+    checked bit for bit against the oracle's on a 1M-key slice first), the multiset of keys is
+    preserved (wrap-around sum and a sum of a mixed form of every key), and partition 17 equals
+    the input rows of that partition in input order."""
     import torch
 
     from legate_dataframe import _capi as C
 
-    n, P = 200_000_000, 256
+    P = 256
     g = torch.Generator(device="cuda").manual_seed(3)
     x = torch.randint(-(2**63), 2**63 - 1, (n,), device="cuda", generator=g, dtype=torch.int64)
     col = C.DeviceColumn(C.INT64, n, x.view(torch.uint8))
     out, offs = C.hash_partition([col], [0], P)
     y = out[0].data.view(torch.int64)
+    xs = x[:1_000_000].cpu().numpy()
+    hs = C.murmur3_hash([C.DeviceColumn(C.INT64, len(xs), x[:1_000_000].view(torch.uint8))], [0])
+    assert np.array_equal(hs.cpu().numpy().view(np.uint32), oracle.row_hashes([oracle.OCol(oracle.T_INT64, xs)], 0))
     assert offs[0] == 0 and offs[-1] == n and all(offs[i] <= offs[i + 1] for i in range(P))
     h = C.murmur3_hash([C.DeviceColumn(C.INT64, n, y.view(torch.uint8))], [0]).to(torch.int64) & 0xFFFFFFFF
     part = torch.repeat_interleave(torch.arange(P, device="cuda"), torch.tensor(np.diff(offs), device="cuda"))
