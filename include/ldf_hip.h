@@ -195,6 +195,27 @@ int ldf_hash_partition(const ldf_table_view* table,
                        ldf_table_out* out,
                        int64_t* offsets);
 
+/* Allocation-free hash partition.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same result as ldf_hash_partition for a table whose every column is
+ * fixed width and non-nullable (1 to 8 columns): the caller passes the outputs -- out->cols[i].data
+ * with out->cols[i].data_bytes >= rows * width of column i -- and the call fills in each
+ * out->cols[i] (type, length, no validity) and the host `offsets` (nparts + 1 entries).  The
+ * counts and scans live in a fixed 16 MiB device array that each GPU holds until the process
+ * ends; the call never allocates (more partitions use fewer, larger row blocks so that they fit).
+ * Calls that overlap on one GPU race on that array: serialise them (one stream, or one call at a
+ * time per device).  Returns LDF_E_INVALID for nullable, string or too many columns or too small
+ * output buffers, LDF_E_UNSUPPORTED above 2048 partitions.  Synchronises `stream` to return the
+ * offsets. */
+int ldf_hash_partition_noalloc(const ldf_table_view* table,
+                               const int32_t* cols,
+                               int32_t ncols,
+                               int32_t nparts,
+                               uint32_t seed,
+                               void* stream,
+                               ldf_table_out* out,
+                               int64_t* offsets);
+
 /* Distinct rows by the key columns `keys` (cudf::distinct with KEEP_ANY, null_equality::EQUAL,
  * cpp/src/stream_compaction.cu:38-72): one row per distinct key tuple -- its first row, so the
  * output rows keep their input order -- with every column of `table` in order.  A null key equals

@@ -180,7 +180,7 @@ ProfState& prof()
 
 ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), s_(s)
 {
-  if (!prof().on) return;
+  if (name == nullptr || !prof().on) return;
   if (hipEventCreate(&a_) != hipSuccess || hipEventRecord(a_, s_) != hipSuccess) a_ = nullptr;
 }
 

@@ -18,6 +18,9 @@
 //                contiguous runs, so rows keep their input order inside a partition.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "ldf_common.hpp"
 #include "ldf_device.hpp"
 
@@ -434,6 +437,16 @@ static size_t scatter_lds_bytes(uint32_t nparts)
   return (b + 15) & ~size_t(15);
 }
 
+static void set_scatter_lds_attr()
+{
+  static bool done = false;
+  if (!done) {
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)scatter_lds_bytes(kHpMaxParts)));
+    done = true;
+  }
+}
+
 }  // namespace ldf
 
 using namespace ldf;
@@ -451,6 +464,122 @@ extern "C" int ldf_murmur3_hash(const ldf_table_view* table, const int32_t* cols
     LDF_HIP_TRY(hipGetLastError());
   });
 }
+
+namespace ldf {
+
+// Blocks of the partition kernels for n rows: rows_per_block is a multiple of the staging step.
+static void partition_blocks(int64_t n, int64_t max_blocks, int64_t* nblk, int64_t* rpb)
+{
+  int64_t b = (n + kHpSub - 1) / kHpSub;
+  if (b > max_blocks) b = max_blocks;
+  if (b < 1) b = 1;
+  int64_t r = (n + b - 1) / b;
+  r         = (r + kHpSub - 1) / kHpSub * kHpSub;
+  *rpb      = r;
+  *nblk     = (n + r - 1) / r;
+}
+
+// Histogram, scan and scatter of a table whose every column is fixed width and non-nullable
+// (`dst[i]` receives column i, rows grouped by partition), with the scratch given: counts
+// u32[nparts * nblk], scan u64[nparts * nblk + 1], ws u64[scan_workspace_elems(nparts * nblk)],
+// offs u64[nparts + 1] (the partition start rows, on the device).
+static void partition_fixed(const ldf_table_view* table, const HashCols& hc, const int32_t* cols, int32_t ncols,
+                            int64_t n, int32_t nparts, uint32_t seed, int64_t nblk, int64_t rpb, void* const* dst,
+                            uint32_t* counts, uint64_t* scan, uint64_t* ws, uint64_t* offs, bool profile, hipStream_t s)
+{
+  int pbits = 0;
+  while ((1u << pbits) < (uint32_t)nparts) ++pbits;
+  const int w0      = type_width(table->cols[0].type_id);
+  const int fw0     = table->cols[0].type_id == LDF_TYPE_FLOAT ? 4 : (table->cols[0].type_id == LDF_TYPE_DOUBLE ? 8 : 0);
+  // one fixed-width column hashed on itself: the value-staged kernels
+  const bool single = table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
+                      scatter_v_lds_bytes((uint32_t)nparts, w0) <= kHpVLdsMax;
+  const FastMod fm  = fastmod_make((uint32_t)nparts);
+  {
+    ProfScope prof(profile ? "hp_hist" : nullptr, s);
+    const size_t hl = nparts * sizeof(uint32_t);
+    const void* d0  = table->cols[0].data;
+    if (!single)
+      hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), hl, s, hc, n, rpb, (uint32_t)nparts, fm, counts);
+    else if (w0 == 1)
+      hipLaunchKernelGGL((k_hp_hist_v<uint8_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint8_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    else if (w0 == 2)
+      hipLaunchKernelGGL((k_hp_hist_v<uint16_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint16_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    else if (w0 == 4 && fw0 == 4)
+      hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 4>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    else if (w0 == 4)
+      hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    else if (fw0 == 8)
+      hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 8>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    else
+      hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
+                         rpb, (uint32_t)nparts, fm, counts);
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  exclusive_scan_u32_to_u64_ws(counts, scan, (int64_t)nparts * nblk, ws, s);
+
+  ProfScope prof(profile ? "hp_scatter" : nullptr, s);
+  if (single) {
+    const ldf_column_view& c0 = table->cols[0];
+    if (w0 == 1) launch_scatter_v<uint8_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+    else if (w0 == 2) launch_scatter_v<uint16_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+    else if (w0 == 4 && fw0 == 4) launch_scatter_v<uint32_t, 4>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+    else if (w0 == 4) launch_scatter_v<uint32_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+    else if (fw0 == 8) launch_scatter_v<uint64_t, 8>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+    else launch_scatter_v<uint64_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, scan, dst[0], s);
+  } else {
+    ScatterCols sc;
+    sc.n = table->ncols;
+    for (int i = 0; i < table->ncols; ++i) {
+      sc.src[i]   = table->cols[i].data;
+      sc.dst[i]   = dst[i];
+      sc.width[i] = type_width(table->cols[i].type_id);
+    }
+    set_scatter_lds_attr();
+    hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes((uint32_t)nparts), s, hc, n, rpb,
+                       (uint32_t)nparts, fm, pbits, (const uint64_t*)scan, sc, (int64_t*)nullptr);
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_hp_pick_offsets, dim3(grid_for(nparts + 1, 256)), dim3(256), 0, s, (const uint64_t*)scan,
+                     (uint32_t)nparts, (uint32_t)nblk, offs);
+  LDF_HIP_TRY(hipGetLastError());
+}
+
+// ---- allocation-free variant: scratch in a fixed device array --------------------------------
+// Each GPU the library runs on holds its own copy of `scratch` until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static unsigned char* scratch_for_current_device()
+{
+  int dev = 0;
+  LDF_HIP_TRY(hipGetDevice(&dev));
+  LDF_EXPECTS(dev >= 0 && dev < kMaxDevices, "hash partition (no alloc): device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr     = nullptr;
+    hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      throw Error(LDF_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+}  // namespace ldf
+
+using namespace ldf;
 
 extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* cols, int32_t ncols, int32_t nparts,
                                   uint32_t seed, const ldf_allocator* alloc, void* stream, ldf_table_out* out,
@@ -487,102 +616,123 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       return;
     }
 
-    int64_t nblk = (n + kHpSub - 1) / kHpSub;
-    if (nblk > 1536) nblk = 1536;
-    int64_t rpb = (n + nblk - 1) / nblk;
-    rpb         = (rpb + kHpSub - 1) / kHpSub * kHpSub;
-    nblk        = (n + rpb - 1) / rpb;
-
-    Buf<uint32_t> counts(a, (size_t)nparts * nblk);
-    Buf<uint64_t> scan(a, (size_t)nparts * nblk + 1);
-    // one fixed-width non-null column hashed on itself: the value-staged kernels
-    const int w0      = type_width(table->cols[0].type_id);
-    const int fw0     = table->cols[0].type_id == LDF_TYPE_FLOAT ? 4 : (table->cols[0].type_id == LDF_TYPE_DOUBLE ? 8 : 0);
-    const bool single = direct && table->ncols == 1 && ncols == 1 && cols[0] == 0 &&
-                        scatter_v_lds_bytes((uint32_t)nparts, w0) <= kHpVLdsMax;
-    const FastMod fm  = fastmod_make((uint32_t)nparts);
-    {
-      ProfScope prof("hp_hist", s);
-      const size_t hl = nparts * sizeof(uint32_t);
-      const void* d0  = table->cols[0].data;
-      if (!single)
-        hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), hl, s, hc, n, rpb, (uint32_t)nparts, fm, counts.get());
-      else if (w0 == 1)
-        hipLaunchKernelGGL((k_hp_hist_v<uint8_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint8_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      else if (w0 == 2)
-        hipLaunchKernelGGL((k_hp_hist_v<uint16_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint16_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      else if (w0 == 4 && fw0 == 4)
-        hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 4>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      else if (w0 == 4)
-        hipLaunchKernelGGL((k_hp_hist_v<uint32_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint32_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      else if (fw0 == 8)
-        hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 8>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      else
-        hipLaunchKernelGGL((k_hp_hist_v<uint64_t, 0>), dim3(nblk), dim3(kHpThreads), hl, s, (const uint64_t*)d0, n, seed,
-                           rpb, (uint32_t)nparts, fm, counts.get());
-      LDF_HIP_TRY(hipGetLastError());
-    }
-    exclusive_scan_u32_to_u64(counts.get(), scan.get(), (int64_t)nparts * nblk, a);
-
-    ScatterCols sc;
-    sc.n = 0;
-    Buf<int64_t> perm;
+    int64_t nblk, rpb;
+    partition_blocks(n, 1536, &nblk, &rpb);
+    const int64_t ncounts = (int64_t)nparts * nblk;
+    Buf<uint32_t> counts(a, (size_t)ncounts);
+    Buf<uint64_t> scan(a, (size_t)ncounts + 1);
+    Buf<uint64_t> ws(a, (size_t)scan_workspace_elems(ncounts));
+    Buf<uint64_t> offs(a, (size_t)nparts + 1);
     if (direct) {
+      void* dst[kMaxKeyCols];
       for (int i = 0; i < table->ncols; ++i) {
         alloc_column(table->cols[i].type_id, n, false, 0, a, &out->cols[i]);
-        sc.src[i]   = table->cols[i].data;
-        sc.dst[i]   = out->cols[i].data;
-        sc.width[i] = type_width(table->cols[i].type_id);
+        dst[i] = out->cols[i].data;
       }
-      sc.n = table->ncols;
+      partition_fixed(table, hc, cols, ncols, n, nparts, seed, nblk, rpb, dst, counts.get(), scan.get(), ws.get(),
+                      offs.get(), true, s);
     } else {
       LDF_EXPECTS(n < (int64_t(1) << 31), "hash partition: nullable or string columns limited to 2^31-1 rows");
-      perm = Buf<int64_t>(a, n);
-    }
-    size_t lds = scatter_lds_bytes((uint32_t)nparts);
-    static bool lds_attr = false;
-    if (!lds_attr) {
-      LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)scatter_lds_bytes(kHpMaxParts)));
-      lds_attr = true;
-    }
-    ProfScope prof("hp_scatter", s);
-    if (single) {
-      const uint64_t* sp = (const uint64_t*)scan.get();
-      const ldf_column_view& c0 = table->cols[0];
-      if (sc.width[0] == 1) launch_scatter_v<uint8_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-      else if (sc.width[0] == 2) launch_scatter_v<uint16_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-      else if (sc.width[0] == 4 && fw0 == 4)
-        launch_scatter_v<uint32_t, 4>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-      else if (sc.width[0] == 4) launch_scatter_v<uint32_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-      else if (fw0 == 8) launch_scatter_v<uint64_t, 8>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-      else launch_scatter_v<uint64_t, 0>(c0, n, seed, rpb, nblk, nparts, pbits, sp, sc.dst[0], s);
-    } else {
-      hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), lds, s, hc, n, rpb, (uint32_t)nparts, fm, pbits,
-                         (const uint64_t*)scan.get(), sc, perm.get());
-      LDF_HIP_TRY(hipGetLastError());
-    }
-
-    if (!direct) {
+      const FastMod fm = fastmod_make((uint32_t)nparts);
+      {
+        ProfScope prof("hp_hist", s);
+        hipLaunchKernelGGL(k_hp_hist, dim3(nblk), dim3(kHpThreads), nparts * sizeof(uint32_t), s, hc, n, rpb,
+                           (uint32_t)nparts, fm, counts.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
+      exclusive_scan_u32_to_u64_ws(counts.get(), scan.get(), ncounts, ws.get(), s);
+      Buf<int64_t> perm(a, n);
+      ScatterCols sc;
+      sc.n = 0;
+      set_scatter_lds_attr();
+      {
+        ProfScope prof("hp_scatter", s);
+        hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes((uint32_t)nparts), s, hc, n, rpb,
+                           (uint32_t)nparts, fm, pbits, (const uint64_t*)scan.get(), sc, perm.get());
+        LDF_HIP_TRY(hipGetLastError());
+      }
       Buf<int32_t> map(a, n);
       hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, perm.get(), n, map.get());
       LDF_HIP_TRY(hipGetLastError());
       std::vector<const ldf_column_view*> src(table->ncols);
       for (int i = 0; i < table->ncols; ++i) src[i] = &table->cols[i];
       gather_columns(src.data(), table->ncols, map.get(), n, false, a, out->cols, /*random_map=*/true);
+      hipLaunchKernelGGL(k_hp_pick_offsets, dim3(grid_for(nparts + 1, 256)), dim3(256), 0, s,
+                         (const uint64_t*)scan.get(), (uint32_t)nparts, (uint32_t)nblk, offs.get());
+      LDF_HIP_TRY(hipGetLastError());
     }
-
-    Buf<uint64_t> offs(a, (size_t)nparts + 1);
-    hipLaunchKernelGGL(k_hp_pick_offsets, dim3(grid_for(nparts + 1, 256)), dim3(256), 0, s, (const uint64_t*)scan.get(),
-                       (uint32_t)nparts, (uint32_t)nblk, offs.get());
-    LDF_HIP_TRY(hipGetLastError());
     std::vector<uint64_t> h(nparts + 1);
     copy_to_host(h.data(), offs.get(), (nparts + 1) * sizeof(uint64_t), s);
     for (int p = 0; p <= nparts; ++p) offsets[p] = (int64_t)h[p];
+  });
+}
+
+extern "C" int ldf_hash_partition_noalloc(const ldf_table_view* table, const int32_t* cols, int32_t ncols,
+                                          int32_t nparts, uint32_t seed, void* stream, ldf_table_out* out,
+                                          int64_t* offsets)
+{
+  return guarded([&] {
+    LDF_EXPECTS(table != nullptr && out != nullptr && offsets != nullptr && out->cols != nullptr, "NULL argument");
+    LDF_EXPECTS(nparts >= 1, "hash partition (no alloc): nparts must be >= 1");
+    if (nparts > kHpMaxParts)
+      throw Error(LDF_E_UNSUPPORTED, "hash partition (no alloc): at most 2048 partitions are supported");
+    LDF_EXPECTS(table->ncols >= 1 && table->ncols <= kMaxKeyCols,
+                "hash partition (no alloc): between 1 and 8 columns");
+    LDF_EXPECTS(out->ncols >= table->ncols, "hash partition (no alloc): output table has too few column slots");
+    hipStream_t s   = (hipStream_t)stream;
+    const int64_t n = table->cols[0].length;
+    for (int i = 0; i < table->ncols; ++i) {
+      validate_view(table->cols[i], n, "hash_partition_noalloc");
+      LDF_EXPECTS(type_width(table->cols[i].type_id) > 0 && table->cols[i].validity == nullptr,
+                  "hash partition (no alloc): every column must be fixed width and non-nullable");
+      LDF_EXPECTS(out->cols[i].data != nullptr || n == 0,
+                  "hash partition (no alloc): out->cols[i].data must hold the partitioned column");
+      LDF_EXPECTS(out->cols[i].data_bytes >= n * type_width(table->cols[i].type_id),
+                  "hash partition (no alloc): out->cols[i].data_bytes is smaller than the column");
+    }
+    HashCols hc = make_hash_cols(table, cols, ncols, seed);
+    LDF_EXPECTS(n < (int64_t(1) << 40), "hash partition (no alloc): too many rows");
+    unsigned char* base = scratch_for_current_device();  // once per device: the array's device address
+    for (int i = 0; i < table->ncols; ++i) {
+      out->cols[i].type_id    = table->cols[i].type_id;
+      out->cols[i].length     = n;
+      out->cols[i].validity   = nullptr;
+      out->cols[i].offsets    = nullptr;
+      out->cols[i].null_count = 0;
+      out->cols[i].validity_bytes = 0;
+    }
+    out->ncols = table->ncols;
+    if (n == 0) {
+      for (int p = 0; p <= nparts; ++p) offsets[p] = 0;
+      return;
+    }
+    // the scratch holds counts u32[nparts * nblk], scan u64[nparts * nblk + 1], the scan's
+    // workspace and offs u64[nparts + 1]: fewer blocks for more partitions
+    int64_t max_blocks = 1536;
+    auto need = [&](int64_t nb) {
+      const int64_t c = (int64_t)nparts * nb;
+      return (size_t)(((c * 4 + 255) / 256) * 256 + ((c + 1) * 8 + 255) / 256 * 256 +
+                      (scan_workspace_elems(c) * 8 + 255) / 256 * 256 + (nparts + 1) * 8);
+    };
+    while (max_blocks > 1 && need(max_blocks) > kNoAllocScratchBytes) max_blocks /= 2;
+    int64_t nblk, rpb;
+    partition_blocks(n, max_blocks, &nblk, &rpb);
+    const int64_t c = (int64_t)nparts * nblk;
+    if (need(nblk) > kNoAllocScratchBytes)
+      throw Error(LDF_E_INVALID, "hash partition (no alloc): the partition counts exceed the " +
+                                   std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch");
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base);
+    uint64_t* scan   = reinterpret_cast<uint64_t*>(base + (c * 4 + 255) / 256 * 256);
+    uint64_t* ws     = reinterpret_cast<uint64_t*>(reinterpret_cast<unsigned char*>(scan) + ((c + 1) * 8 + 255) / 256 * 256);
+    uint64_t* offs   = reinterpret_cast<uint64_t*>(reinterpret_cast<unsigned char*>(ws) +
+                                                 (scan_workspace_elems(c) * 8 + 255) / 256 * 256);
+    void* dst[kMaxKeyCols];
+    for (int i = 0; i < table->ncols; ++i) dst[i] = out->cols[i].data;
+    // every scratch word the kernels read is written first in this call (the histogram stores
+    // every count, the scan every prefix), so nothing needs clearing between calls
+    partition_fixed(table, hc, cols, ncols, n, nparts, seed, nblk, rpb, dst, counts, scan, ws, offs, false, s);
+    static_assert(sizeof(int64_t) == sizeof(uint64_t), "offsets are copied as 64-bit words");
+    LDF_HIP_TRY(hipMemcpyAsync(offsets, offs, (size_t)(nparts + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LDF_HIP_TRY(hipStreamSynchronize(s));
   });
 }

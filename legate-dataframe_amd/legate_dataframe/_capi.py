@@ -27,7 +27,7 @@ AGG = {"sum": 0, "product": 1, "min": 2, "max": 3, "count": 4, "mean": 5, "varia
 COUNT_MODE = {"only_valid": 0, "only_null": 1, "all": 2}
 
 EXPORTED = [
-    "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_distinct",
+    "ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_hash_partition_noalloc", "ldf_murmur3_hash", "ldf_distinct",
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
     "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release", "ldf_pool_alloc",
@@ -111,6 +111,8 @@ def load(require: bool = True):
                                          P(ctypes.c_int64)]
         L.ldf_distinct.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, P(Allocator), ctypes.c_void_p,
                                    P(TableOut)]
+        L.ldf_hash_partition_noalloc.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_uint32, ctypes.c_void_p, P(TableOut), P(ctypes.c_int64)]
         L.ldf_murmur3_hash.argtypes = [P(TableView), P(ctypes.c_int32), ctypes.c_int32, ctypes.c_uint32,
                                        ctypes.c_void_p, ctypes.c_void_p]
         L.ldf_bools_to_bitmask.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
@@ -146,7 +148,8 @@ def load(require: bool = True):
             getattr(L, f).restype = ctypes.c_int
         L.ldf_last_error.restype = ctypes.c_char_p
         L.ldf_version.restype = ctypes.c_char_p
-        for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_murmur3_hash", "ldf_distinct",
+        for f in ("ldf_join", "ldf_groupby", "ldf_hash_partition", "ldf_hash_partition_noalloc", "ldf_murmur3_hash",
+                  "ldf_distinct",
                   "ldf_bools_to_bitmask",
                   "ldf_bitmask_to_bools", "ldf_ranges_to_offsets", "ldf_offsets_to_ranges"):
             getattr(L, f).restype = ctypes.c_int
@@ -440,6 +443,25 @@ def hash_partition_raw(table, cols, nparts: int, seed: int = 0):
     check(L.ldf_hash_partition(ctypes.byref(tv), int32_array(cols), len(cols), nparts, seed, None, stream_ptr(),
                                ctypes.byref(rt.out), offs))
     return rt, list(offs)
+
+
+def hash_partition_noalloc(table, cols, nparts: int, out_data, seed: int = 0):
+    """ldf_hash_partition_noalloc: `out_data` holds one caller-allocated uint8 tensor per column
+    (at least rows * width bytes); returns the partitioned columns (views of those tensors) and the
+    nparts + 1 offsets."""
+    L = load()
+    tv, _ta = make_table_view([c.view() for c in table])
+    cols_out = (Column * max(1, len(table)))()
+    for i, t in enumerate(out_data):
+        cols_out[i].data = t.data_ptr()
+        cols_out[i].data_bytes = t.numel()
+    out = TableOut(len(table), cols_out)
+    offs = (ctypes.c_int64 * (nparts + 1))()
+    check(L.ldf_hash_partition_noalloc(ctypes.byref(tv), int32_array(cols), len(cols), nparts, seed, stream_ptr(),
+                                       ctypes.byref(out), offs))
+    res = [DeviceColumn(cols_out[i].type_id, cols_out[i].length, out_data[i][: cols_out[i].length * WIDTH[cols_out[i].type_id]])
+           for i in range(out.ncols)]
+    return res, list(offs)
 
 
 def pool_release() -> None:
