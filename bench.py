@@ -8,9 +8,10 @@ resident in HBM, outputs allocated inside the timed region.  `value` = rows/s.
 
 roofline: the dominant kernel (gb_aggregate) timed with HIP events on its own stream during the
 timed steps; achieved = its algorithmic bytes (input columns read once) / its mean duration.
-cpu_baseline: the CPU oracle (single-threaded C restatement of the reference CPU variant) on a
-10M-row sample of the same workload; Arrow Acero (the reference CPU variant's engine, pyarrow,
-use_threads=False) on the same sample is reported beside it.
+cpu_baseline: Arrow Acero (pyarrow, use_threads=False: the engine and options of the reference
+CPU variant) on the whole 100M-row workload on 1 core (`value`), the same on N single-threaded
+processes over hash partitions of the rows (`n_cores`), and the oracle's C restatement on 1 core
+(`port`); the 1M x 10M inner join carries the same three figures.
 
 Other workloads (also reported as `secondary` entries of the default line): --workload join
 (1M-build x 10M-probe inner join), --workload join_left / join_full (50M-build x 500M-probe with
@@ -160,33 +161,198 @@ def groupby_alg_bytes(n, ngroups):
 
 
 # ---------------------------------------------------------------------------------------------
-# CPU baseline on a sample
+# CPU baseline: the reference CPU variant's engine (Arrow Acero, single-threaded) on the host
 # ---------------------------------------------------------------------------------------------
-def cpu_baseline_groupby(torch, host, sample):
+# GroupByAggregationTask::cpu_variant and JoinTask::cpu_variant run Acero with use_threads=false
+# (cpp/src/groupby_aggregation.cpp:87-91, cpp/src/join.cpp:214-227), one task per Legate CPU;
+# with N CPUs the rows are first hash-partitioned so each task sees whole groups / key matches.
+# The same Acero calls run here through pyarrow: once on 1 core over the whole workload, and as N
+# single-threaded worker processes, each on its hash partition of the rows (N = the host's cores,
+# at most 16, the GPU box's CPU share).  Workers are spawned before the GPU is initialised and
+# read the rows from memory-mapped files; only the Acero call itself is timed.
+CPU_WORKERS_MAX = 16
+CPU_POOL = None  # (multiprocessing pool, workers) when the CPU baseline runs
+
+
+def cpu_worker_count():
+    return max(1, min(CPU_WORKERS_MAX, os.cpu_count() or 1))
+
+
+def start_cpu_pool():
+    import multiprocessing as mp
+
+    n = cpu_worker_count()
+    pool = mp.get_context("spawn").Pool(n)
+    pool.map(_w_ready, range(n))  # every worker is up before the parent touches the GPU
+    return pool, n
+
+
+def _w_ready(_):
+    return os.getpid()
+
+
+def _mm(path, lo, hi):
     import numpy as np
+
+    return np.ascontiguousarray(np.load(path, mmap_mode="r")[lo:hi])
+
+
+def _w_acero_groupby(job):
+    """One GroupByAggregationTask::cpu_variant on rows [lo, hi): the 7 aggregations."""
     import pyarrow as pa
 
+    paths, lo, hi = job
+    m = [_mm(paths[f"m{j}"], lo, hi) for j in range(3)]
+    tab = pa.table({"k": _mm(paths["k"], lo, hi), "f": pa.array(_mm(paths["f"], lo, hi), mask=~m[0]),
+                    "i": pa.array(_mm(paths["i"], lo, hi), mask=~m[1]), "u": pa.array(_mm(paths["u"], lo, hi), mask=~m[2])})
+    t0 = time.perf_counter()
+    out = pa.TableGroupBy(tab, ["k"], use_threads=False).aggregate(
+        [("f", "sum"), ("f", "mean"), ("i", "sum"), ("i", "min"), ("i", "max"), ("i", "count"), ("u", "count_distinct")])
+    return time.perf_counter() - t0, out.num_rows
+
+
+def _w_oracle_groupby(job):
+    """The oracle's single-threaded C restatement of the same groupby (a side figure)."""
     import oracle
 
-    k = host["k"][:sample].cpu().numpy()
-    f = host["f"][:sample].cpu().numpy()
-    i = host["i"][:sample].cpu().numpy()
-    u = host["u"][:sample].cpu().numpy()
-    m = [x[:sample].cpu().numpy() for x in host["masks"]]
-    keys = [oracle.OCol(oracle.T_INT32, k)]
-    aggs = [(oracle.OCol(oracle.T_DOUBLE, f, m[0]), "sum", None), (oracle.OCol(oracle.T_DOUBLE, f, m[0]), "mean", None),
-            (oracle.OCol(oracle.T_INT64, i, m[1]), "sum", None), (oracle.OCol(oracle.T_INT64, i, m[1]), "min", None),
-            (oracle.OCol(oracle.T_INT64, i, m[1]), "max", None), (oracle.OCol(oracle.T_INT64, i, m[1]), "count", None),
-            (oracle.OCol(oracle.T_INT32, u, m[2]), "count_distinct", None)]
+    paths, lo, hi = job
+    m = [_mm(paths[f"m{j}"], lo, hi) for j in range(3)]
+    f, i, u = _mm(paths["f"], lo, hi), _mm(paths["i"], lo, hi), _mm(paths["u"], lo, hi)
+    fc, ic = oracle.OCol(oracle.T_DOUBLE, f, m[0]), oracle.OCol(oracle.T_INT64, i, m[1])
+    aggs = [(fc, "sum", None), (fc, "mean", None), (ic, "sum", None), (ic, "min", None), (ic, "max", None),
+            (ic, "count", None), (oracle.OCol(oracle.T_INT32, u, m[2]), "count_distinct", None)]
+    keys = [oracle.OCol(oracle.T_INT32, _mm(paths["k"], lo, hi))]
     t0 = time.perf_counter()
-    oracle.groupby(keys, aggs)
-    t_oracle = time.perf_counter() - t0
-    tab = pa.table({"k": k, "f": pa.array(f, mask=~m[0]), "i": pa.array(i, mask=~m[1]), "u": pa.array(u, mask=~m[2])})
+    reps, _ = oracle.groupby(keys, aggs)
+    return time.perf_counter() - t0, len(reps)
+
+
+def _w_acero_join(job):
+    """One JoinTask::cpu_variant: INNER hashjoin of lhs rows [llo, lhi) (probe) with rhs rows
+    [rlo, rhi) (build); no nulls, so the nulls_equal emulation adds no mask columns."""
+    import pyarrow as pa
+
+    paths, llo, lhi, rlo, rhi = job
+    lhs = pa.table({"k": _mm(paths["pk"], llo, lhi), "x": _mm(paths["pp"], llo, lhi)})
+    rhs = pa.table({"kr": _mm(paths["bk"], rlo, rhi), "y": _mm(paths["bp"], rlo, rhi)})
     t0 = time.perf_counter()
-    pa.TableGroupBy(tab, ["k"], use_threads=False).aggregate(
-        [("f", "sum"), ("f", "mean"), ("i", "sum"), ("i", "min"), ("i", "max"), ("i", "count"), ("u", "count_distinct")])
-    t_acero = time.perf_counter() - t0
-    return sample / t_oracle, sample / t_acero, t_oracle
+    out = lhs.join(rhs, keys="k", right_keys="kr", join_type="inner", use_threads=False)
+    return time.perf_counter() - t0, out.num_rows
+
+
+def _w_oracle_join(job):
+    import oracle
+
+    paths, llo, lhi, rlo, rhi = job
+    pk, bk = _mm(paths["pk"], llo, lhi), _mm(paths["bk"], rlo, rhi)
+    t0 = time.perf_counter()
+    lm, _ = oracle.join([oracle.OCol(oracle.T_INT64, pk)], [oracle.OCol(oracle.T_INT64, bk)], "inner")
+    return time.perf_counter() - t0, len(lm)
+
+
+class HostFiles:
+    """Host copies of device columns as .npy files in a temporary directory (workers map them)."""
+
+    def __init__(self):
+        import tempfile
+
+        self.dir = tempfile.mkdtemp(prefix="ldf_cpu_baseline_")
+        self.paths = {}
+
+    def put(self, name, tensor_or_array):
+        import numpy as np
+
+        a = tensor_or_array.cpu().numpy() if hasattr(tensor_or_array, "cpu") else np.asarray(tensor_or_array)
+        path = os.path.join(self.dir, name + ".npy")
+        np.save(path, a)
+        self.paths[name] = path
+
+    def close(self):
+        import shutil
+
+        shutil.rmtree(self.dir, ignore_errors=True)
+
+
+def hash_slices(torch, key, nparts):
+    """Stable hash partition of the rows by `key` (a device tensor): (row order, offsets)."""
+    import numpy as np
+
+    h = key.to(torch.int64) * -7046029254386353131  # 0x9E3779B97F4A7C15 as int64
+    part = ((h >> 32) & 0x7FFFFFFF) % nparts
+    order = torch.argsort(part, stable=True)
+    counts = torch.bincount(part, minlength=nparts).cpu().numpy()
+    return order, np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+
+def cpu_baseline_groupby(torch, pool, nw, host, n):
+    files = HostFiles()
+    try:
+        for name in ("k", "f", "i", "u"):
+            files.put(name, host[name])
+        for j, mk in enumerate(host["masks"]):
+            files.put(f"m{j}", mk)
+        whole = dict(files.paths)
+        order, offs = hash_slices(torch, host["k"], nw)
+        for name in ("k", "f", "i", "u"):
+            files.put(name + "_p", host[name][order])
+        for j, mk in enumerate(host["masks"]):
+            files.put(f"m{j}_p", mk[order])
+        del order
+        parts = {key: files.paths[key + "_p"] for key in whole}
+        log(f"[cpu] groupby: 1 core, Acero on {n} rows")
+        t1, g1 = pool.apply(_w_acero_groupby, ((whole, 0, n),))
+        log(f"[cpu] groupby: {nw} workers, Acero on hash slices")
+        res = pool.map(_w_acero_groupby, [(parts, int(offs[j]), int(offs[j + 1])) for j in range(nw)])
+        tn = max(r[0] for r in res)
+        gn = sum(r[1] for r in res)
+        log("[cpu] groupby: 1 core, the oracle (C restatement)")
+        to, go = pool.apply(_w_oracle_groupby, ((whole, 0, n),))
+    finally:
+        files.close()
+    assert g1 == gn == go, (g1, gn, go)
+    return {"value": n / t1, "unit": "rows/s", "cores": 1, "kind": "reference",
+            "engine": "pyarrow 25 Acero TableGroupBy(use_threads=False): the engine and options of "
+                      "GroupByAggregationTask::cpu_variant (cpp/src/groupby_aggregation.cpp:87-91); the Legate task "
+                      "wrapper around it is not built here",
+            "sample": f"the whole workload: all {n} rows, 7 aggregations, {g1} groups, {t1:.1f} s on 1 core",
+            "n_cores": {"value": n / tn, "cores": nw,
+                        "note": f"{nw} single-threaded processes, each Acero on one hash partition of the rows "
+                                "(whole groups per process, as Legate's CPU tasks after the shuffle; the shuffle "
+                                "itself is not timed); value = rows / slowest process"},
+            "port": {"value": n / to, "cores": 1, "kind": "port",
+                     "note": "the oracle's single-threaded C restatement, same rows"}}
+
+
+def cpu_baseline_join(torch, pool, nw, cols):
+    pk, pp, bk, bp = cols
+    n, m = pk.numel(), bk.numel()
+    files = HostFiles()
+    try:
+        for name, t in (("pk", pk), ("pp", pp), ("bk", bk), ("bp", bp)):
+            files.put(name, t)
+        whole = dict(files.paths)
+        lo_, loffs = hash_slices(torch, pk, nw)
+        ro_, roffs = hash_slices(torch, bk, nw)
+        files.put("pk_p", pk[lo_])
+        files.put("pp_p", pp[lo_])
+        files.put("bk_p", bk[ro_])
+        files.put("bp_p", bp[ro_])
+        parts = {key: files.paths[key + "_p"] for key in whole}
+        t1, r1 = pool.apply(_w_acero_join, ((whole, 0, n, 0, m),))
+        res = pool.map(_w_acero_join, [(parts, int(loffs[j]), int(loffs[j + 1]), int(roffs[j]), int(roffs[j + 1]))
+                                       for j in range(nw)])
+        to, ro = pool.apply(_w_oracle_join, ((whole, 0, n, 0, m),))
+    finally:
+        files.close()
+    tn = max(r[0] for r in res)
+    assert r1 == sum(r[1] for r in res) == ro, (r1, ro)
+    return {"value": (n + m) / t1, "unit": "rows/s", "cores": 1, "kind": "reference",
+            "engine": "pyarrow 25 Acero hashjoin (Table.join, use_threads=False; build = rhs): the engine and options "
+                      "of JoinTask::cpu_variant (cpp/src/join.cpp:214-227)",
+            "sample": f"the whole workload: {n} probe x {m} build rows, {r1} output rows, {t1:.2f} s on 1 core",
+            "n_cores": {"value": (n + m) / tn, "cores": nw,
+                        "note": f"{nw} single-threaded processes, each joining one hash partition of both sides"},
+            "port": {"value": (n + m) / to, "cores": 1, "kind": "port", "note": "the oracle's C restatement"}}
 
 
 def profiles_traffic(name):
@@ -242,13 +408,8 @@ def run_groupby(args, torch, C):
                      "kernel_ms": k_avg_ms, "launches": launches, "alg_bytes_per_launch": inp,
                      "whole_step_frac": (inp + outb) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
-    if not args.no_cpu:
-        sample = min(n, 10_000_000)
-        o_rps, a_rps, t_o = cpu_baseline_groupby(torch, host, sample)
-        res["cpu_baseline"] = {"value": o_rps, "unit": "rows/s", "cores": 1, "kind": "port",
-                               "sample": f"first {sample} rows of the same workload, all 7 aggregations, "
-                                         f"{t_o:.1f} s single-threaded",
-                               "acero_value": a_rps, "acero_note": "pyarrow TableGroupBy use_threads=False"}
+    if CPU_POOL is not None:
+        res["cpu_baseline"] = cpu_baseline_groupby(torch, CPU_POOL[0], CPU_POOL[1], host, n)
     return res
 
 
@@ -279,10 +440,14 @@ def run_join(args, torch, C):
     ms_step = (t1 - t0) * 1e3 / args.steps
     alg = (n + m) * 16 + nout * 32
     kern = {k: C.profile_get(k) for k in JOIN_KERNELS}
-    return {"workload": "inner join build 1M (permutation keys) x probe 10M (uniform [0,2M)), i64 keys, "
-                        "i64/f64 payloads", "rows_per_s": (n + m) / (ms_step * 1e-3), "ms_per_step": ms_step,
-            "out_rows": nout, "alg_bytes": alg, "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+    res = {"workload": "inner join build 1M (permutation keys) x probe 10M (uniform [0,2M)), i64 keys, "
+                       "i64/f64 payloads", "rows_per_s": (n + m) / (ms_step * 1e-3), "ms_per_step": ms_step,
+           "out_rows": nout, "alg_bytes": alg, "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
+    if CPU_POOL is not None:
+        log("[cpu] inner join 1M x 10M: Acero on 1 core and on hash slices")
+        res["cpu_baseline"] = cpu_baseline_join(torch, CPU_POOL[0], CPU_POOL[1], (pkeys, ppay, bkeys, bpay))
+    return res
 
 
 JOIN_KERNELS = ("join_build", "join_count", "join_emit", "gather", "gather_str")
@@ -530,6 +695,16 @@ def run_served(args, torch, C):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nq))
     poisson = serve(arrivals)
     log(f"[served] poisson done: {poisson}")
+    # one query at a time through the same server: persistent join indexes, no batching or overlap
+    seq = []
+    for kind, t, arg in queries:
+        t0 = time.perf_counter()
+        qid = srv.submit_join(t, arg, 0) if kind == "join" else srv.submit_groupby(t, arg[0], arg[1], SERVED_AGGS)
+        srv.wait(qid)
+        srv.release(qid)
+        seq.append(time.perf_counter() - t0)
+    seq = np.array(seq)
+    log(f"[served] sequential with persistent indexes: {nq / seq.sum():.1f} q/s")
     srv.close()
     # the plain path under the same arrivals: one query at a time, first come first served
     t_free, lat_plain = 0.0, []
@@ -548,7 +723,12 @@ def run_served(args, torch, C):
             "plain_sequential": {"queries_per_s": nq / float(svc.sum()), "mean_service_ms": float(svc.mean() * 1e3),
                                  "p50_ms_at_same_arrivals": float(np.percentile(lat_plain, 50)),
                                  "p99_ms_at_same_arrivals": float(np.percentile(lat_plain, 99))},
+            "sequential_with_indexes": {"queries_per_s": nq / float(seq.sum()),
+                                        "mean_service_ms": float(seq.mean() * 1e3),
+                                        "note": "each query alone through the server (its persistent join "
+                                                "indexes), no batching or overlap"},
             "throughput_vs_sequential": burst["queries_per_s"] / (nq / float(svc.sum())),
+            "throughput_vs_sequential_with_indexes": burst["queries_per_s"] / (nq / float(seq.sum())),
             "index_build_ms": index_ms}
 
 
@@ -639,6 +819,18 @@ def main():
               "multi-GPU runs are not supported.", file=sys.stderr)
         sys.exit(2)
 
+    global CPU_POOL
+    if not args.no_cpu and args.workload in ("groupby", "join") and not args.copy_calibration:
+        CPU_POOL = start_cpu_pool()  # before the GPU is initialised
+    try:
+        _main(args)
+    finally:
+        if CPU_POOL is not None:
+            CPU_POOL[0].close()
+            CPU_POOL[0].join()
+
+
+def _main(args):
     import torch
 
     from legate_dataframe import _capi as C
