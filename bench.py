@@ -405,6 +405,8 @@ def run_groupby(args, torch, C):
                    "rows": n, "groups": ngroups, "form": "plain, one call per step"},
         "roofline": {"bound": "hbm", "kernel": "gb_aggregate", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiles_traffic("gb_aggregate_groupby100M"),
+                     "traffic_source": "profiles/gb_aggregate_groupby100M_traffic.json (rocprofv3 --pmc FETCH_SIZE and "
+                                       "WRITE_SIZE runs of this workload, copy-calibrated; not measured in this run)",
                      "kernel_ms": k_avg_ms, "launches": launches, "alg_bytes_per_launch": inp,
                      "whole_step_frac": (inp + outb) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
