@@ -38,7 +38,7 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics",
 ]
 
-HIP_SOURCES = ["common.hip", "partition.hip", "join.hip", "groupby.hip", "batched.hip", "distinct.hip"]
+HIP_SOURCES = ["common.hip", "partition.hip", "join.hip", "groupby.hip", "median.hip", "batched.hip", "distinct.hip"]
 
 
 def _run(cmd: list[str]) -> None:

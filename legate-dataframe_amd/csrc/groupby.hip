@@ -2278,7 +2278,7 @@ static ColDev to_coldev(const ldf_column_view& c)
   return d;
 }
 
-static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys, const ldf_agg* aggs, int32_t naggs,
+void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys, const ldf_agg* aggs, int32_t naggs,
                         Alloc& a, ldf_table_out* out)
 {
   hipStream_t s = a.stream();
@@ -2300,7 +2300,7 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
   for (int i = 0; i < naggs; ++i) {
     const ldf_agg& g = aggs[i];
     LDF_EXPECTS(g.kind >= LDF_AGG_SUM && g.kind <= LDF_AGG_COUNT_ALL, "Unsupported groupby aggregation");
-    if (g.kind == LDF_AGG_APPROX_MEDIAN) throw Error(LDF_E_UNSUPPORTED, "approximate_median is not implemented");
+    LDF_EXPECTS(g.kind != LDF_AGG_APPROX_MEDIAN, "groupby: approximate_median goes through run_groupby_with_medians");
     if (g.kind == LDF_AGG_COUNT_ALL) continue;
     LDF_EXPECTS(g.in_col >= 0 && g.in_col < tv->ncols, "groupby: aggregation input column out of range");
     if (g.kind == LDF_AGG_COUNT || g.kind == LDF_AGG_COUNT_DISTINCT) {
@@ -2999,11 +2999,19 @@ static void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_
 
 using namespace ldf;
 
+namespace ldf {
+void run_groupby_with_medians(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys, const ldf_agg* aggs,
+                              int32_t naggs, Alloc& a, ldf_table_out* out);  // median.hip
+}
+
 extern "C" int ldf_groupby(const ldf_table_view* table, const int32_t* keys, int32_t nkeys, const ldf_agg* aggs,
                            int32_t naggs, const ldf_allocator* alloc, void* stream, ldf_table_out* out)
 {
   return guarded([&] {
     Alloc a(alloc, (hipStream_t)stream);
-    run_groupby(table, keys, nkeys, aggs, naggs, a, out);
+    bool median = false;
+    for (int i = 0; i < naggs && aggs != nullptr; ++i) median |= aggs[i].kind == LDF_AGG_APPROX_MEDIAN;
+    if (median) run_groupby_with_medians(table, keys, nkeys, aggs, naggs, a, out);
+    else run_groupby(table, keys, nkeys, aggs, naggs, a, out);
   });
 }

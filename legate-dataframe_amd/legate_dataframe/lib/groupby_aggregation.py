@@ -30,7 +30,8 @@ def groupby_aggregation(
     """Perform a groupby and aggregation in a single operation.
 
     Available aggregations: "sum", "product", "min", "max", "count", "mean", "variance",
-    "stddev", "count_distinct", "count_all".  Each column aggregation is
+    "stddev", "approximate_median", "count_distinct", "count_all".  "approximate_median" is the
+    exact median (float64), as the reference's GPU variant computes it.  Each column aggregation is
     ``(<input-column-name>, <aggregation-kind>, <output-column-name>)``; the kind may be
     ``(kind, pyarrow.compute.CountOptions(...))`` for "count" and "count_distinct".
     "count_all" takes ``None`` as its input column.

@@ -17,7 +17,9 @@
  *                   Acero "aggregate" with hash_<kind>; null keys form one group; SUM/MEAN/
  *                   MIN/MAX/PRODUCT/VARIANCE/STDDEV of a group without valid values is null;
  *                   MIN/MAX skip NaN unless all valid values are NaN; integer SUM/PRODUCT wrap;
- *                   COUNT and COUNT_DISTINCT follow CountOptions modes.  Floating sums use
+ *                   COUNT and COUNT_DISTINCT follow CountOptions modes; APPROX_MEDIAN is the
+ *                   exact median the GPU variant computes (cpp/src/groupby_aggregation.cu:75-77;
+ *                   Arrow's own CPU kind is a t-digest estimate).  Floating sums use
  *                   Neumaier compensated summation (the FP64 oracle for SUM/MEAN).
  *                   Groups are returned in order of first appearance.
  *   orc_murmur3_32  MurmurHash3_x86_32 (Austin Appleby, public domain).
@@ -317,6 +319,17 @@ static inline uint64_t ordered_key(const orc_col* c, int64_t r)
   return (uint64_t)signed_at(c, r) ^ 0x8000000000000000ull;
 }
 
+/* median: a valid value and its sort key (ordered_key, NaN after every number) */
+typedef struct {
+  uint64_t key;
+  double value;
+} med_item;
+static int med_cmp(const void* a, const void* b)
+{
+  uint64_t x = ((const med_item*)a)->key, y = ((const med_item*)b)->key;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
 /* distinct (group, value) pairs */
 typedef struct {
   int64_t* g;
@@ -487,6 +500,41 @@ int64_t orc_groupby(const orc_col* keys, int nk, int64_t n, const orc_agg* aggs,
         free(s.row);
         free(s.h);
         free(hasnull);
+        break;
+      }
+      case A_MEDIAN: {
+        /* the reference's GPU variant maps approximate_median to cudf's exact median
+         * (cpp/src/groupby_aggregation.cu:75-77): the valid values of a group sorted (any NaN
+         * after every number), the middle one, or 0.5 a + 0.5 b of the two middle ones for an
+         * even count (quantile 0.5, linear interpolation); null for a group without valid
+         * values.  Values are ordered by their typed value; the result is their double. */
+        int64_t* start = (int64_t*)calloc(G + 1, sizeof(int64_t));
+        med_item* it   = (med_item*)malloc((size_t)(n > 0 ? n : 1) * sizeof(med_item));
+        if (!start || !it) return -1;
+        for (int64_t r = 0; r < n; ++r)
+          if (is_valid(c, r)) start[gid[r] + 1]++;
+        for (size_t g = 0; g < G; ++g) start[g + 1] += start[g];
+        int64_t* fill = (int64_t*)malloc(G * sizeof(int64_t));
+        if (!fill) return -1;
+        memcpy(fill, start, G * sizeof(int64_t));
+        for (int64_t r = 0; r < n; ++r) {
+          if (!is_valid(c, r)) continue;
+          med_item m;
+          m.value = double_at(c, r);
+          m.key   = (is_float_t(c->type) && isnan(m.value)) ? ~0ull : ordered_key(c, r);
+          it[fill[gid[r]]++] = m;
+        }
+        for (size_t g = 0; g < G; ++g) {
+          int64_t k = start[g + 1] - start[g];
+          o->valid[g] = k > 0;
+          if (k == 0) continue;
+          qsort(it + start[g], (size_t)k, sizeof(med_item), med_cmp);
+          double a = it[start[g] + (k - 1) / 2].value, b = it[start[g] + k / 2].value;
+          o->f64[g] = (k & 1) ? a : 0.5 * a + 0.5 * b;
+        }
+        free(start);
+        free(it);
+        free(fill);
         break;
       }
       default: return -1;

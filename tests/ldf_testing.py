@@ -134,7 +134,8 @@ def groupby_via_oracle(t: pa.Table, keys, aggs) -> pa.Table:
         vt = t.column(col).type if col is not None else pa.int64()
         if kind in ("count", "count_all", "count_distinct"):
             arr = pa.array(bits.view(np.int64))
-        elif kind in ("mean", "variance", "stddev") or (kind in ("sum", "product") and pa.types.is_floating(vt)):
+        elif kind in ("mean", "variance", "stddev", "approximate_median") or \
+                (kind in ("sum", "product") and pa.types.is_floating(vt)):
             arr = pa.array(f64, mask=~valid)
         elif kind in ("sum", "product"):
             arr = pa.array(bits.view(np.uint64 if pa.types.is_unsigned_integer(vt) or vt == pa.bool_() else np.int64),

@@ -273,6 +273,36 @@ def test_groupby_c2_full_size_count_distinct(gpu, keys_kind):
     assert int(got.sum()) == int(ref.sum())
 
 
+@pytest.mark.parametrize("n", [5_000, 1_500_000])
+def test_groupby_approximate_median_vs_oracle(gpu, n):
+    """approximate_median is the exact median (cudf's, which the reference's GPU variant uses):
+    against the oracle's sort-based median (pinned by numpy.median on the CPU), bit for bit, with
+    nulls, NaN, all-null groups, integer and float inputs, null and string keys, and beside other
+    aggregations (1.5M rows: the groups come from the partitioned path)."""
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, n)
+    x[rng.random(n) < 0.01] = np.nan
+    k = rng.integers(0, max(10, n // 50), n)
+    xm = rng.random(n) < 0.1
+    xm[k == 3] = True  # a group without valid values
+    t = pa.table({"k": pa.array(k, mask=rng.random(n) < 0.01), "s": [f"g{v}" for v in rng.integers(0, 5, n)],
+                  "x": pa.array(x, mask=xm), "i": rng.integers(-(2**40), 2**40, n).astype(np.int64),
+                  "u": rng.integers(0, 1000, n).astype(np.uint16), "f": rng.random(n).astype(np.float32)})
+    aggs = [("x", "approximate_median", "xm"), ("i", "sum", "is"), ("i", "approximate_median", "im"),
+            ("u", "approximate_median", "um"), ("f", "approximate_median", "fm"), ("x", "count", "xc")]
+    for keys in (["k"], ["k", "s"]):
+        got = _norm(gpu_groupby(t, keys, aggs))
+        assert got.schema.field("xm").type == pa.float64()
+        exp = _norm(groupby_via_oracle(t, keys, aggs))
+        assert_groupby_equal(got, exp, keys, rel=0.0)
+
+
+def test_groupby_median_only(gpu):
+    t = pa.table({"k": [1, 1, 1, 2, 2, 3], "v": [5, 1, 3, 2, 8, 7]})
+    got = gpu_groupby(t, ["k"], [("v", "approximate_median", "m")])
+    assert dict(zip(got["k"].to_pylist(), got["m"].to_pylist())) == {1: 3.0, 2: 5.0, 3: 7.0}
+
+
 def test_groupby_errors(gpu):
     from legate_dataframe import LogicalTable
     from legate_dataframe.lib.groupby_aggregation import groupby_aggregation

@@ -243,3 +243,32 @@ def test_groupby_oracle_brute_force():
         assert int(res[3][0][g]) == len(vals)
         assert int(res[4][0][g]) == len({int(v[i]) % 7 for i in rows if vv[i]})
         assert int(res[5][0][g]) == len(rows)
+
+
+def test_groupby_oracle_median_vs_numpy():
+    """approximate_median as the GPU variant computes it (cudf's exact median): per group the
+    median of the valid values, numpy.median of the group's values; no valid value -> null;
+    NaN sorts after every number; integer inputs give the double of the middle value(s)."""
+    rng = np.random.default_rng(13)
+    n = 6000
+    k = rng.integers(0, 40, n)
+    x = rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, n)
+    xv = rng.random(n) > 0.15
+    xv[k == 7] = False  # a group without valid values
+    i = rng.integers(-(2**40), 2**40, n)
+    reps, res = oracle.groupby([OCol(T_INT64, k)], [(OCol(T_DOUBLE, x, xv), "approximate_median", None),
+                                                    (OCol(T_INT64, i), "approximate_median", None)])
+    for g, r in enumerate(reps):
+        sel = k == k[r]
+        vals = x[sel & xv]
+        assert res[0][2][g] == (len(vals) > 0)
+        if len(vals):
+            assert res[0][1][g] == np.median(vals)
+        assert res[1][1][g] == np.median(i[sel].astype(np.float64)) or len(set(i[sel])) < 2
+        srt = sorted(int(v) for v in i[sel])
+        m = len(srt)
+        assert res[1][1][g] == (float(srt[m // 2]) if m % 2 else 0.5 * float(srt[m // 2 - 1]) + 0.5 * float(srt[m // 2]))
+    # NaN after every number: a group {1, 2, nan} has median 2, {1, nan} has 0.5 + 0.5 nan = nan
+    reps, res = oracle.groupby([OCol(T_INT64, np.array([0, 0, 0, 1, 1]))],
+                               [(OCol(T_DOUBLE, np.array([np.nan, 1.0, 2.0, np.nan, 1.0])), "approximate_median", None)])
+    assert res[0][1][0] == 2.0 and np.isnan(res[0][1][1])
