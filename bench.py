@@ -764,7 +764,11 @@ def run_partition(args, torch, C):
 def run_distinct(args, torch, C):
     n = args.rows or 100_000_000
     g = torch.Generator(device="cuda").manual_seed(6)
-    k = torch.randint(0, 10_000_000, (n,), device="cuda", generator=g, dtype=torch.int64)
+    nk = args.distinct_keys or 10_000_000
+    if nk >= n:  # every key distinct: a permutation of the rows
+        k = torch.randperm(n, device="cuda", generator=g).to(torch.int64)
+    else:
+        k = torch.randint(0, nk, (n,), device="cuda", generator=g, dtype=torch.int64)
     v = torch.rand(n, device="cuda", generator=g, dtype=torch.float64)
     cols = [C.DeviceColumn(C.INT64, n, k.view(torch.uint8)), C.DeviceColumn(C.DOUBLE, n, v.view(torch.uint8))]
     for _ in range(args.warmup):
@@ -785,7 +789,8 @@ def run_distinct(args, torch, C):
     ms_step = (t1 - t0) * 1e3 / args.steps
     kern = {k: C.profile_get(k) for k in ("distinct_insert",)}
     alg = 8 * n + 32 * m  # the key column once; the kept rows' two columns read and written once
-    return {"workload": f"distinct {n} rows by an int64 key U[0, 1e7) with a float64 column (first row per key)",
+    keys = "a permutation of the rows (every key distinct)" if nk >= n else f"U[0, {nk})"
+    return {"workload": f"distinct {n} rows by an int64 key {keys} with a float64 column (first row per key)",
             "rows_per_s": n / (ms_step * 1e-3), "ms_per_step": ms_step, "out_rows": m, "alg_bytes": alg,
             "whole_step_frac": alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()}}
@@ -812,6 +817,8 @@ def main():
                     default="groupby")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--queries", type=int, default=200)
+    ap.add_argument("--distinct-keys", type=int, default=0,
+                    help="distinct workload: key values U[0, K) (default 1e7); K >= rows gives every key distinct")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")

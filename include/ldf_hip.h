@@ -221,7 +221,8 @@ int ldf_hash_partition_noalloc(const ldf_table_view* table,
  * output rows keep their input order -- with every column of `table` in order.  A null key equals
  * a null key; every NaN key equals every NaN and -0.0 equals 0.0 (nan_equality::ALL_EQUAL; the
  * groupby and the join compare floating-point keys bitwise, as Arrow does).
- * `out->cols` must have room for table->ncols columns; 1 to 8 key columns. */
+ * `out->cols` must have room for table->ncols columns; one or more key columns (a column named
+ * twice counts once). */
 int ldf_distinct(const ldf_table_view* table,
                  const int32_t* keys,
                  int32_t nkeys,

@@ -13,7 +13,8 @@
 // (nan_equality::ALL_EQUAL) and -0.0 equals 0.0; other values by their bits.
 //
 // Layout in HBM: an open-addressing set of 64-bit slots (2^k >= n/4 first, for inputs of 2^20
-// rows or more; 2^k >= 2n when that set overflows or for smaller inputs), each (high 32 bits of the
+// rows or more whose sampled key count is at most about n/5; 2^k >= 2n when that set overflows,
+// for more keys or for smaller inputs), each (high 32 bits of the
 // row's 64-bit key hash << 32 | row index), installed with one atomicCAS; a row whose key is
 // already present lowers the slot's row with atomicMin (same hash bits, so the minimum is the
 // smaller row), so once every row is in, each key's slot holds its first row.  Slot values only
@@ -21,6 +22,7 @@
 // The kept rows are then flagged from a sequential scan of the slots.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ldf_agg.hpp"
@@ -45,7 +47,25 @@ __device__ __forceinline__ uint64_t key_bits(const DCol& c, int64_t row)
   return is_float_type(c.type) ? murmur_input_bits(b, c.width) : b;
 }
 
-__device__ inline uint64_t key_hash(const DKeys& k, int64_t row)
+// The key columns' descriptors, in device memory (any number of key columns).
+struct DKeysP {
+  const DCol* c;
+  int32_t n;
+};
+
+// Up to 32 descriptors per launch, passed by value and stored into the device array.
+constexpr int kDColChunk = 32;
+struct DColChunk {
+  DCol c[kDColChunk];
+  int32_t n;
+  int32_t offset;
+};
+__global__ void k_put_dcols(DColChunk ch, DCol* __restrict__ dst)
+{
+  if ((int)threadIdx.x < ch.n) dst[ch.offset + threadIdx.x] = ch.c[threadIdx.x];
+}
+
+__device__ inline uint64_t key_hash(const DKeysP& k, int64_t row)
 {
   uint64_t h = 0;
   for (int i = 0; i < k.n; ++i) {
@@ -60,7 +80,7 @@ __device__ inline uint64_t key_hash(const DKeys& k, int64_t row)
   return h;
 }
 
-__device__ inline bool keys_equal(const DKeys& k, int64_t ra, int64_t rb)
+__device__ inline bool keys_equal(const DKeysP& k, int64_t ra, int64_t rb)
 {
   for (int i = 0; i < k.n; ++i) {
     const DCol& c = k.c[i];
@@ -78,10 +98,13 @@ __device__ inline bool keys_equal(const DKeys& k, int64_t ra, int64_t rb)
 
 // Inserts every row into the set; err = 1 when a row found no room within `plimit` probes (with
 // the full-size set, plimit = its capacity and load <= 1/2: cannot happen, kept as a guard).
-__global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t n, unsigned long long* __restrict__ set,
+// Once err is set the pass is void (it reruns with the full-size set), so rows stop probing.
+__global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeysP k, int64_t n, unsigned long long* __restrict__ set,
                                                                uint64_t mask, uint64_t plimit, int32_t* __restrict__ err)
 {
+  const volatile int32_t* failed = err;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (*failed) return;
     const uint64_t h              = key_hash(k, r);
     const unsigned long long mine = ((h >> 32) << 32) | (uint64_t)(uint32_t)r;
     uint64_t i                    = h & mask;
@@ -106,6 +129,33 @@ __global__ __launch_bounds__(kDThreads) void k_distinct_insert(DKeys k, int64_t 
     }
     if (!placed) *err = 1;
   }
+}
+
+// Cardinality sample: S rows at a stride, their key hashes into a set of 2^20 slots; dups
+// counts the sampled rows whose hash was already there.  Sampling a fraction f of rows whose D keys
+// repeat evenly gives about S^2 / (2 D) repeats (the birthday count), so few repeats mean many
+// keys -- more than the n/4-slot set holds.
+constexpr int kDSample     = 1 << 18;
+constexpr int kDSampleBits = 20;
+__global__ __launch_bounds__(kDThreads) void k_distinct_sample(DKeysP k, int64_t S, int64_t stride,
+                                                               unsigned long long* __restrict__ set,
+                                                               unsigned long long* __restrict__ dups)
+{
+  unsigned long long mine = 0;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < S; j += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h = key_hash(k, j * stride) | 1ull;  // never the empty marker 0
+    uint64_t i       = h & ((1ull << kDSampleBits) - 1);
+    while (true) {
+      const unsigned long long old = atomicCAS(&set[i], 0ull, (unsigned long long)h);
+      if (old == 0ull) break;
+      if (old == h) {
+        ++mine;
+        break;
+      }
+      i = (i + 1) & ((1ull << kDSampleBits) - 1);
+    }
+  }
+  if (mine) atomicAdd(dups, mine);
 }
 
 // keep[row] = 1 for the row each occupied slot holds (keep cleared before).
@@ -160,18 +210,31 @@ static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t n
 {
   hipStream_t s = a.stream();
   LDF_EXPECTS(t != nullptr && out != nullptr && (out->cols != nullptr || t->ncols == 0), "distinct: NULL argument");
-  LDF_EXPECTS(nkeys >= 1 && nkeys <= kMaxKeyCols, "distinct: between 1 and 8 key columns");
+  LDF_EXPECTS(nkeys >= 1, "distinct: at least one key column");
   LDF_EXPECTS(keys != nullptr, "distinct: NULL key list");
   LDF_EXPECTS(out->ncols >= t->ncols, "distinct: output table has too few column slots");
   const int64_t n = t->ncols > 0 ? t->cols[0].length : 0;
   for (int i = 0; i < t->ncols; ++i) validate_view(t->cols[i], n, "distinct");
   LDF_EXPECTS(n < INT32_MAX, "distinct: limited to 2^31-1 rows per call");
-  DKeys k;
-  k.n = nkeys;
+  // the key columns, each once (a repeated key column changes nothing)
+  std::vector<DCol> kcols;
+  std::vector<int32_t> seen;
   for (int i = 0; i < nkeys; ++i) {
     if (keys[i] < 0 || keys[i] >= t->ncols) throw Error(LDF_E_INVALID, "distinct: key index out of range");
-    k.c[i] = to_dcol(t->cols[keys[i]]);
+    if (std::find(seen.begin(), seen.end(), keys[i]) != seen.end()) continue;
+    seen.push_back(keys[i]);
+    kcols.push_back(to_dcol(t->cols[keys[i]]));
   }
+  Buf<DCol> kdev(a, kcols.size());
+  for (size_t off = 0; off < kcols.size(); off += kDColChunk) {
+    DColChunk ch;
+    ch.n      = (int32_t)std::min<size_t>(kDColChunk, kcols.size() - off);
+    ch.offset = (int32_t)off;
+    for (int i = 0; i < ch.n; ++i) ch.c[i] = kcols[off + i];
+    hipLaunchKernelGGL(k_put_dcols, dim3(1), dim3(kDColChunk), 0, s, ch, kdev.get());
+    LDF_HIP_TRY(hipGetLastError());
+  }
+  const DKeysP k{kdev.get(), (int32_t)kcols.size()};
 
   int64_t n_out = 0;
   Buf<int32_t> map;
@@ -181,6 +244,21 @@ static void run_distinct(const ldf_table_view* t, const int32_t* keys, int32_t n
     // the whole pass runs again with 2n slots, which always hold every key.
     const uint64_t full_cap = next_pow2((uint64_t)(n > 32 ? 2 * n : 64));
     uint64_t cap            = n >= kDSmallFrom ? next_pow2((uint64_t)(n / 4)) : full_cap;
+    if (cap < full_cap) {
+      // estimated keys above n/5 (fewer than 2.5 S^2 / n repeats in the sample): the n/4-slot set
+      // would overflow, so go to the full-size set at once
+      const int64_t S = std::min<int64_t>(n, kDSample);
+      Buf<unsigned long long> sset(a, (size_t)1 << kDSampleBits);
+      Buf<unsigned long long> dups(a, 1);
+      fill_u64(reinterpret_cast<uint64_t*>(sset.get()), 0, (int64_t)1 << kDSampleBits, s);
+      fill_u64(reinterpret_cast<uint64_t*>(dups.get()), 0, 1, s);
+      hipLaunchKernelGGL(k_distinct_sample, dim3(grid_for(S, kDThreads)), dim3(kDThreads), 0, s, k, S, n / S, sset.get(),
+                         dups.get());
+      LDF_HIP_TRY(hipGetLastError());
+      unsigned long long d = 0;
+      copy_to_host(&d, dups.get(), sizeof(d), s);
+      if ((double)d < 2.5 * (double)S * (double)S / (double)n) cap = full_cap;
+    }
     const int64_t nblk      = (n + kDBlockRows - 1) / kDBlockRows;
     Buf<uint8_t> keep(a, n);
     Buf<uint32_t> blk_cnt(a, nblk);

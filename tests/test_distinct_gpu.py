@@ -124,9 +124,13 @@ def test_distinct_50M_rows_against_torch_unique(gpu):
     assert torch.equal(out[1].data.view(torch.float64)[:m], v[first])
 
 
-def test_distinct_small_set_overflow_falls_back(gpu):
-    """2^21 rows, every key distinct: the first set (n/4 slots) cannot hold them, rows hit the probe
-    limit and the call runs again with the full-size set; every row is kept, in order."""
+@pytest.mark.parametrize("fooled", [False, True])
+def test_distinct_small_set_overflow_falls_back(gpu, fooled):
+    """2^21 rows, every key distinct: the cardinality sample sees no repeat and the call uses the
+    full-size set at once.  fooled: the sampled rows (every 8th) all hold one key, so the sample
+    predicts few keys; the first set (n/4 slots) cannot hold the ~1.8M keys, rows hit the probe limit
+    and the call runs again with the full-size set.  Either way each key's first row is kept, in
+    order."""
     import torch
 
     from legate_dataframe import _capi as C
@@ -134,6 +138,29 @@ def test_distinct_small_set_overflow_falls_back(gpu):
     n = 1 << 21
     g = torch.Generator(device="cuda").manual_seed(13)
     k = torch.randperm(n, device="cuda", generator=g).to(torch.int64) * 7 - 3
+    if fooled:
+        k[::8] = -1
     out = C.distinct([C.DeviceColumn(C.INT64, n, k.view(torch.uint8))], [0])
-    assert out[0].length == n
-    assert torch.equal(out[0].data.view(torch.int64)[:n], k)
+    keep = torch.ones(n, dtype=torch.bool, device="cuda")
+    if fooled:
+        keep[8::8] = False  # only the first -1 (row 0) is kept
+    exp = k[keep]
+    assert out[0].length == exp.numel()
+    assert torch.equal(out[0].data.view(torch.int64)[:exp.numel()], exp)
+
+
+def test_distinct_many_key_columns_and_repeated_keys(gpu):
+    """keys=None on a 12-column table (every column a key: more than fit in a kernel's argument
+    block, so the key descriptors live in device memory) and a key list naming a column twice."""
+    rng = np.random.default_rng(21)
+    n = 30_000
+    cols = {f"c{i}": rng.integers(0, 3 if i < 6 else 2, n).astype(np.int8 if i % 2 else np.int32) for i in range(12)}
+    cols["c3"] = pa.array(cols["c3"], mask=rng.random(n) < 0.1)
+    cols["s"] = [f"s{v}" for v in rng.integers(0, 2, n)]
+    t = pa.table(cols)
+    got = gpu_distinct(t, None)
+    exp = t.take(pa.array(first_rows(t, t.column_names), pa.int64()))
+    assert_rows_equal(got, exp.cast(got.schema))
+    got = gpu_distinct(t, ["c1", "c0", "c1", "s"])
+    exp = t.take(pa.array(first_rows(t, ["c1", "c0", "s"]), pa.int64()))
+    assert_rows_equal(got, exp.cast(got.schema))
