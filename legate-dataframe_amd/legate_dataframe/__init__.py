@@ -2,8 +2,8 @@
 
 Same entry points as the reference package (python/legate_dataframe/__init__.py):
 ``LogicalTable``, ``LogicalColumn``, ``lib.join.join``, ``lib.groupby_aggregation.groupby_aggregation``;
-``lib.stream_compaction.distinct``; plus ``lib.hash_partition.hash_partition`` for the local step of
-the hash repartition.
+``lib.stream_compaction.distinct``, ``lib.parquet.parquet_read`` / ``parquet_write``; plus
+``lib.hash_partition.hash_partition`` for the local step of the hash repartition.
 The columns live in HBM; the operators run hand-written HIP kernels through the C ABI of
 libldf_hip.so (include/ldf_hip.h) and raise if that library is missing.
 """

@@ -289,6 +289,38 @@ def column_from_strings(strs, valid=None, device=None) -> DeviceColumn:
     return col
 
 
+def column_from_arrow(arr, device=None) -> DeviceColumn:
+    """A pyarrow array (fixed width, timestamp or string) as a DeviceColumn in HBM."""
+    import numpy as np
+    import pyarrow as pa
+
+    if isinstance(arr, pa.ChunkedArray):
+        arr = arr.combine_chunks()
+    valid = np.asarray(arr.is_valid()) if arr.null_count else None
+    if pa.types.is_string(arr.type) or pa.types.is_large_string(arr.type):
+        return column_from_strings(arr.to_pylist(), valid, device)
+    t = arr.type
+    if pa.types.is_timestamp(t):
+        tid, vals = TIMESTAMP, arr.cast(pa.int64()).fill_null(0).to_numpy()
+    elif t == pa.bool_():
+        tid, vals = BOOL, np.asarray(arr.fill_null(False)).astype(np.uint8)
+    else:
+        tid = {pa.uint8(): UINT8, pa.int8(): INT8, pa.uint16(): UINT16, pa.int16(): INT16, pa.uint32(): UINT32,
+               pa.int32(): INT32, pa.uint64(): UINT64, pa.int64(): INT64, pa.float32(): FLOAT, pa.float64(): DOUBLE,
+               pa.date32(): DATE32}[t]
+        vals = arr.fill_null(0).to_numpy(zero_copy_only=False) if arr.null_count else arr.to_numpy(zero_copy_only=False)
+    return column_from_numpy(vals, tid, valid, device)
+
+
+def columns_from_parquet(files, columns=None, device=None) -> List[DeviceColumn]:
+    """Parquet files (read on the host with pyarrow) as DeviceColumns in HBM, e.g. the tables a
+    served-form Server registers with add_table."""
+    from legate_dataframe.lib.parquet import read_arrow
+
+    t = read_arrow(files, columns=columns)
+    return [column_from_arrow(t.column(i), device) for i in range(t.num_columns)]
+
+
 def column_to_numpy(c: DeviceColumn):
     """(values, valid) on the host; strings come back as a list of str (None for null)."""
     import numpy as np
