@@ -296,7 +296,8 @@ void ldf_pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_b
  * instances' results concatenated, instance i owning rows [row_offsets[i], row_offsets[i+1]);
  * the buffers belong to the batch (valid until the next run or destroy).  Results are those
  * of ldf_join / ldf_groupby on each instance (rows in another order).
- * Limits: one fixed-width key column; join types INNER, LEFT, SEMI, ANTI; fixed-width output
+ * Limits: one fixed-width key column; join types INNER, LEFT, FULL, SEMI, ANTI (FULL: the
+ * unmatched build rows of an instance follow its probe rows); fixed-width output
  * columns; groupby kinds SUM, MEAN, COUNT, COUNT_ALL, MIN, MAX over at most 4 value columns
  * (an FP64 SUM / MEAN input outside the exact range |x| < 2^62 makes ldf_batch_result return
  * LDF_E_UNSUPPORTED).  A run over inputs whose sizes changed after create may make
@@ -307,7 +308,9 @@ typedef struct ldf_batch_config {
   int32_t tile_rows;  /* rows per work item: 1024, 2048 or 4096; 0 = autotuned */
   int32_t lds_slots;  /* groupby: slots of the LDS hash table that aggregates one partition
                          (256, 512, 1024 or 2048; the partition fan-out follows from it, so it
-                         sets how much of each instance's table lives in LDS); 0 = autotuned */
+                         sets how much of each instance's table lives in LDS); join: instance
+                         tables of at most this many slots are probed from an LDS copy (1 = every
+                         table in HBM; 256 .. 2048); 0 = autotuned */
   int32_t use_graph;  /* 1 = capture the run's launch sequence in a hipGraph and replay it */
 } ldf_batch_config;
 

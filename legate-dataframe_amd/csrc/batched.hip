@@ -97,6 +97,7 @@ struct JInst {
   int64_t brow;   // first build row in the concatenated build arrays
   int64_t prow;   // first probe row in the concatenated probe arrays
   int64_t ptile;  // first probe tile in the concatenated tile arrays
+  int64_t utile;  // FULL: first of the tiles that hold its unmatched build rows (one per build tile)
   BSlot* tabp;            // the instance's table (cap + 2 slots)
   const uint32_t* rowsp;  // build rows grouped by slot; entries are brow + local row
 };
@@ -128,13 +129,13 @@ __device__ __forceinline__ int64_t bj_insert(const JInst& I, int w, int64_t r, B
   }
 }
 
-// slot of the probe row's key in the instance's table (local index), -1 = no build row has it
-__device__ __forceinline__ int64_t bj_find(const JInst& I, int w, int64_t r, bool ne)
+// slot of the probe row's key in the instance's table `t` (its HBM table, or a copy in LDS;
+// local index), -1 = no build row has it
+__device__ __forceinline__ int64_t bj_find(const JInst& I, const BSlot* t, int w, int64_t r, bool ne)
 {
   if (!valid_at(I.pvalid, r)) return ne ? (int64_t)I.cap + 1 : -1;
   const unsigned long long k = load_bits(I.pkey, w, r);
   if (k == kBE) return (int64_t)I.cap;
-  const BSlot* t = I.tabp;
   uint64_t i     = mix64(k) & I.mask;
   while (true) {
     unsigned long long cur = t[i].key;
@@ -148,7 +149,8 @@ __device__ __forceinline__ uint64_t bj_out_rows(uint32_t cnt, int jt)
 {
   switch (jt) {
     case LDF_JOIN_INNER: return cnt;
-    case LDF_JOIN_LEFT: return cnt > 0 ? cnt : 1;
+    case LDF_JOIN_LEFT:
+    case LDF_JOIN_FULL: return cnt > 0 ? cnt : 1;
     case LDF_JOIN_SEMI: return cnt > 0 ? 1 : 0;
     default: return cnt > 0 ? 0 : 1;  // ANTI
   }
@@ -198,10 +200,16 @@ __global__ void k_bj_group(const int64_t* __restrict__ slot_of_row, int64_t nbt,
 }
 
 // info[prow + r] = start | cnt << 32; btot[ptile + tile] = output rows of the tile
+// FULL: hit[slot] = 1 for every slot a probe row matches (plain stores of the same value).
+// With lds_max > 0 an instance whose table has at most lds_max slots is probed in an LDS copy of
+// it (the block loads the table once per work item: placement chosen by the autotuner).
 __global__ __launch_bounds__(kQT) void k_bj_count(const JInst* __restrict__ inst, const Item* __restrict__ items,
                                                   int nitems, unsigned int* cursor, int tile, int w, bool ne, int jt,
-                                                  uint64_t* __restrict__ info, uint64_t* __restrict__ btot)
+                                                  uint64_t* __restrict__ info, uint64_t* __restrict__ btot,
+                                                  uint8_t* __restrict__ hit, int lds_max)
 {
+  extern __shared__ __align__(16) unsigned char dyn[];
+  BSlot* ltab = reinterpret_cast<BSlot*>(dyn);
   __shared__ int sh;
   __shared__ uint64_t smem[17];
   for (;;) {
@@ -211,13 +219,20 @@ __global__ __launch_bounds__(kQT) void k_bj_count(const JInst* __restrict__ inst
     const JInst I    = inst[m.inst];
     const int64_t r0 = (int64_t)m.tile * tile;
     const int64_t r1 = r0 + tile < I.np ? r0 + tile : I.np;
-    uint64_t sum     = 0;
+    const bool in_lds = lds_max > 0 && I.cap <= (uint64_t)lds_max;  // the same for the whole block
+    if (in_lds) {
+      for (int64_t i = threadIdx.x; i < (int64_t)I.cap + 2; i += kQT) ltab[i] = I.tabp[i];
+      __syncthreads();
+    }
+    const BSlot* T = in_lds ? ltab : I.tabp;
+    uint64_t sum   = 0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kQT) {
-      const int64_t s = bj_find(I, w, r, ne);
+      const int64_t s = bj_find(I, T, w, r, ne);
       uint32_t cnt = 0, st = 0;
       if (s >= 0) {
-        cnt = I.tabp[s].cnt;
-        st  = I.tabp[s].start;
+        cnt = T[s].cnt;
+        st  = T[s].start;
+        if (hit != nullptr && cnt > 0 && hit[I.tab + s] == 0) hit[I.tab + s] = 1;
       }
       info[I.prow + r] = (uint64_t)st | ((uint64_t)cnt << 32);
       sum += bj_out_rows(cnt, jt);
@@ -288,6 +303,53 @@ __global__ __launch_bounds__(kQT) void k_bj_emit(const JInst* __restrict__ inst,
       lmap[gq]          = (int32_t)(base + lo);
       if (rmap != nullptr)
         rmap[gq] = mc[lo] == 0 ? -1 : (int32_t)((int64_t)I.rowsp[st[lo] + (q - off[lo])] - I.brow);
+    }
+  }
+}
+
+// FULL: the build rows of each (instance, build tile) whose slot no probe row matched --
+// btot[utile + tile] = their number (count pass); the emit pass writes them, in build-row order,
+// as output rows with no lhs row (lmap = -1) after the instance's probe rows.
+__global__ __launch_bounds__(kQT) void k_bj_unmatched(const JInst* __restrict__ inst, const Item* __restrict__ items,
+                                                      int nitems, unsigned int* cursor, int btile,
+                                                      const int64_t* __restrict__ slot_row,
+                                                      const uint8_t* __restrict__ hit, uint64_t* __restrict__ btot,
+                                                      const uint64_t* __restrict__ boff, int64_t out_cap,
+                                                      int32_t* __restrict__ lmap, int32_t* __restrict__ rmap,
+                                                      int32_t* __restrict__ flags)
+{
+  __shared__ int sh;
+  __shared__ uint64_t smem[17];
+  const int per = btile / kQT;
+  for (;;) {
+    const int it = pop_item(cursor, &sh);
+    if (it >= nitems) break;
+    const Item m     = items[it];
+    const JInst I    = inst[m.inst];
+    const int64_t r0 = (int64_t)m.tile * btile + (int64_t)threadIdx.x * per;  // this thread's rows, in order
+    uint32_t c       = 0;
+    for (int j = 0; j < per; ++j) {
+      const int64_t r = r0 + j;
+      if (r < I.nb && hit[slot_row[I.brow + r]] == 0) ++c;
+    }
+    uint64_t total;
+    uint64_t pos = block_exclusive_scan(c, smem, &total);
+    if (boff == nullptr) {  // count pass
+      if (threadIdx.x == 0) btot[I.utile + m.tile] = total;
+      continue;
+    }
+    const uint64_t gbase = boff[I.utile + m.tile];
+    if (gbase + total > (uint64_t)out_cap) {
+      if (threadIdx.x == 0) flags[0] = 1;
+      continue;
+    }
+    for (int j = 0; j < per; ++j) {
+      const int64_t r = r0 + j;
+      if (r < I.nb && hit[slot_row[I.brow + r]] == 0) {
+        lmap[gbase + pos] = -1;
+        rmap[gbase + pos] = (int32_t)r;
+        ++pos;
+      }
     }
   }
 }
@@ -945,8 +1007,17 @@ struct BatchImpl {
   }
 };
 
-void check_config(const ldf_batch_config& c)
+void check_config(const ldf_batch_config& c, bool join = false)
 {
+  if (join) {  // joins: lds_slots = 1 probes every table in HBM
+    LDF_EXPECTS(c.load_factor > 0.0 && c.load_factor <= 0.9, "batch: load_factor must be in (0, 0.9]");
+    LDF_EXPECTS(c.tile_rows == 1024 || c.tile_rows == 2048 || c.tile_rows == 4096,
+                "batch: tile_rows must be 1024, 2048 or 4096");
+    LDF_EXPECTS(c.lds_slots == 1 || c.lds_slots == 256 || c.lds_slots == 512 || c.lds_slots == 1024 ||
+                  c.lds_slots == 2048,
+                "batch join: lds_slots must be 1 (tables in HBM), 256, 512, 1024 or 2048");
+    return;
+  }
   LDF_EXPECTS(c.load_factor > 0.0 && c.load_factor <= 0.9, "batch: load_factor must be in (0, 0.9]");
   LDF_EXPECTS(c.tile_rows == 1024 || c.tile_rows == 2048 || c.tile_rows == 4096,
               "batch: tile_rows must be 1024, 2048 or 4096");
@@ -966,6 +1037,7 @@ std::vector<int> by_size(const std::vector<int64_t>& sz)
 // ---- join batch ---------------------------------------------------------------------------
 struct JoinBatch : BatchImpl {
   int ninst = 0, w = 0, jt = 0, tile = 0, btile = 0, nout = 0, grid_q = 0;
+  int lds_max = 0;  // tables of at most this many slots are probed from LDS (0: all in HBM)
   bool ne       = true;
   bool prebuilt = false;  // probes prebuilt indexes: no build phases
   int64_t nslot = 0, nbt = 0, npt = 0, ntiles = 0, out_cap = 0;
@@ -988,6 +1060,7 @@ struct JoinBatch : BatchImpl {
   int32_t* flags    = nullptr;
   int32_t* lmap     = nullptr;
   int32_t* rmap     = nullptr;
+  uint8_t* hit      = nullptr;  // FULL: build slots some probe row matched
   const void** d_src         = nullptr;
   const uint8_t** d_sval     = nullptr;
   unsigned long long* nulls  = nullptr;
@@ -996,12 +1069,21 @@ struct JoinBatch : BatchImpl {
 
   void enqueue_count(hipStream_t s)
   {
-    fill_u32(cursors, 0, 4, s);
+    fill_u32(cursors, 0, 6, s);
     fill_u32(reinterpret_cast<uint32_t*>(flags), 0, 2, s);
     if (!prebuilt) build(s);
+    if (hit != nullptr) fill_u8(hit, 0, nslot, s);
     if (npitems > 0) {
-      hipLaunchKernelGGL(k_bj_count, dim3(std::min(grid_q, npitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
-                         (const Item*)d_pitems, npitems, cursors + 1, tile, w, ne, jt, info, btot);
+      hipLaunchKernelGGL(k_bj_count, dim3(std::min(grid_q, npitems)), dim3(kQT),
+                         lds_max > 0 ? (size_t)(lds_max + 2) * sizeof(BSlot) : 0, s, (const JInst*)d_inst,
+                         (const Item*)d_pitems, npitems, cursors + 1, tile, w, ne, jt, info, btot, hit, lds_max);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+    if (hit != nullptr && nbitems > 0) {
+      hipLaunchKernelGGL(k_bj_unmatched, dim3(std::min(grid_q, nbitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
+                         (const Item*)d_bitems, nbitems, cursors + 4, btile, (const int64_t*)slot_row,
+                         (const uint8_t*)hit, btot, (const uint64_t*)nullptr, (int64_t)0, (int32_t*)nullptr,
+                         (int32_t*)nullptr, flags);
       LDF_HIP_TRY(hipGetLastError());
     }
     exclusive_scan_u64_ws(btot, boff, ntiles, ws, s);
@@ -1016,8 +1098,11 @@ struct JoinBatch : BatchImpl {
     hipLaunchKernelGGL(k_bj_fill, dim3(grid_for(nslot, 256)), dim3(256), 0, s, tab, nslot);
     LDF_HIP_TRY(hipGetLastError());
     if (nbitems > 0) {
+      // FULL places null build keys in the null slot even when nulls do not match (no probe then
+      // finds them, and they come out as unmatched build rows)
       hipLaunchKernelGGL(k_bj_build, dim3(std::min(grid_q, nbitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
-                         (const Item*)d_bitems, nbitems, cursors + 0, btile, w, ne, tab, slot_row);
+                         (const Item*)d_bitems, nbitems, cursors + 0, btile, w, ne || jt == LDF_JOIN_FULL, tab,
+                         slot_row);
       LDF_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(k_bj_counts, dim3(grid_for(nslot, 256)), dim3(256), 0, s, (const BSlot*)tab, nslot, cnt32);
@@ -1048,6 +1133,12 @@ struct JoinBatch : BatchImpl {
       hipLaunchKernelGGL(k_bj_emit, dim3(std::min(grid_q, npitems)), dim3(kQT), lds, s, (const JInst*)d_inst,
                          (const Item*)d_pitems, npitems, cursors + 2, tile, jt, (const uint64_t*)info,
                          (const uint64_t*)boff, out_cap, lmap, rmap, flags);
+      LDF_HIP_TRY(hipGetLastError());
+    }
+    if (hit != nullptr && nbitems > 0 && out_cap > 0) {
+      hipLaunchKernelGGL(k_bj_unmatched, dim3(std::min(grid_q, nbitems)), dim3(kQT), 0, s, (const JInst*)d_inst,
+                         (const Item*)d_bitems, nbitems, cursors + 5, btile, (const int64_t*)slot_row,
+                         (const uint8_t*)hit, btot, (const uint64_t*)boff, out_cap, lmap, rmap, flags);
       LDF_HIP_TRY(hipGetLastError());
     }
     if (out_cap > 0 && nout > 0) {
@@ -1096,15 +1187,16 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
 {
   LDF_EXPECTS(ninst >= 1 && lhs != nullptr && rhs != nullptr, "batch join: no instances");
   LDF_EXPECTS(nkeys == 1 && lhs_keys != nullptr && rhs_keys != nullptr, "batch join: exactly one key column");
-  if (jt == LDF_JOIN_FULL) throw Error(LDF_E_UNSUPPORTED, "batch join: FULL joins run through ldf_join");
-  LDF_EXPECTS(jt == LDF_JOIN_INNER || jt == LDF_JOIN_LEFT || jt == LDF_JOIN_SEMI || jt == LDF_JOIN_ANTI,
-              "Unknown JoinType");
+  LDF_EXPECTS(jt >= LDF_JOIN_INNER && jt <= LDF_JOIN_ANTI, "Unknown JoinType");
+  if (jt == LDF_JOIN_FULL && prebuilt != nullptr)
+    throw Error(LDF_E_UNSUPPORTED, "batch join: FULL joins against prebuilt indexes are not supported");
   if ((jt == LDF_JOIN_SEMI || jt == LDF_JOIN_ANTI) && n_rhs_out > 0)
     throw Error(LDF_E_INVALID, "rhs must have no output columns for SEMI and ANTI join");
   LDF_EXPECTS(n_lhs_out >= 0 && n_rhs_out >= 0 && n_lhs_out + n_rhs_out <= kMaxBOut,
               "batch join: at most 16 output columns");
-  check_config(cfg);
+  check_config(cfg, true);
   std::unique_ptr<JoinBatch> b(new JoinBatch());
+  b->lds_max = cfg.lds_slots > 1 && prebuilt == nullptr ? cfg.lds_slots : 0;
   b->cfg   = cfg;
   b->cs    = s;
   b->ninst = ninst;
@@ -1138,7 +1230,7 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
   for (int c = 0; c < b->nout; ++c) {
     const int32_t t = (oside[c] ? R0 : L0).cols[ocol[c]].type_id;
     if (type_width(t) <= 0) throw Error(LDF_E_UNSUPPORTED, "batch join: output columns must be fixed-width");
-    nullable[c] = oside[c] == 1 && jt == LDF_JOIN_LEFT;
+    nullable[c] = jt == LDF_JOIN_FULL || (oside[c] == 1 && jt == LDF_JOIN_LEFT);
   }
   std::vector<int64_t> np(ninst), nb(ninst);
   for (int i = 0; i < ninst; ++i) {
@@ -1184,6 +1276,8 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
     }
     prow += np[i];
     ptile += (np[i] + b->tile - 1) / b->tile;
+    I.utile = ptile;  // FULL: the instance's unmatched build rows come after its probe rows
+    if (jt == LDF_JOIN_FULL) ptile += (nb[i] + b->btile - 1) / b->btile;
   }
   b->nslot  = tabo;
   b->nbt    = brow;
@@ -1213,7 +1307,8 @@ JoinBatch* create_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table
   b->boff     = b->dnew<uint64_t>(b->ntiles + 1);
   b->ooff     = b->dnew<int64_t>(ninst + 1);
   b->ws       = b->dnew<uint64_t>(std::max(scan_workspace_elems(b->nslot), scan_workspace_elems(b->ntiles)));
-  b->cursors  = b->dnew<uint32_t>(4);
+  b->cursors  = b->dnew<uint32_t>(6);
+  if (jt == LDF_JOIN_FULL) b->hit = b->dnew<uint8_t>(b->nslot);
   b->flags    = b->dnew<int32_t>(2);
   b->nulls    = b->dnew<unsigned long long>(kMaxBOut);
   for (int i = 0; i < ninst; ++i) {
@@ -1747,8 +1842,7 @@ ldf_batch_config resolve_join_config(const ldf_batch_config* in, int32_t ninst, 
 {
   ldf_batch_config c{};
   if (in) c = *in;
-  if (c.lds_slots == 0) c.lds_slots = 1024;  // unused by joins
-  if (c.load_factor != 0.0 && c.tile_rows != 0) return c;
+  if (c.load_factor != 0.0 && c.tile_rows != 0 && c.lds_slots != 0) return c;
   std::vector<int64_t> rows(ninst);
   for (int i = 0; i < ninst; ++i) rows[i] = lhs[i].ncols > 0 ? lhs[i].cols[0].length : 0;
   const std::string key = tune_key("join", ninst, rows);
@@ -1760,9 +1854,12 @@ ldf_batch_config resolve_join_config(const ldf_batch_config* in, int32_t ninst, 
     if (it != T.m.end()) {
       if (c.load_factor == 0.0) c.load_factor = it->second.load_factor;
       if (c.tile_rows == 0) c.tile_rows = it->second.tile_rows;
+      if (c.lds_slots == 0) c.lds_slots = it->second.lds_slots == 1 || it->second.lds_slots == 2048 ? it->second.lds_slots : 1;
       return c;
     }
   }
+  // load factor x tile rows with the tables in HBM, then the best of those with the tables of up
+  // to 2048 slots probed from LDS
   std::vector<ldf_batch_config> cands;
   for (double lf : {0.5, 0.7})
     for (int t : {1024, 2048, 4096}) {
@@ -1771,6 +1868,7 @@ ldf_batch_config resolve_join_config(const ldf_batch_config* in, int32_t ninst, 
       else if (lf != 0.5) continue;
       if (c.tile_rows == 0) k.tile_rows = t;
       else if (t != 1024) continue;
+      if (c.lds_slots == 0) k.lds_slots = 1;
       cands.push_back(k);
     }
   const std::vector<int> smp = tune_sample(rows);
@@ -1779,9 +1877,15 @@ ldf_batch_config resolve_join_config(const ldf_batch_config* in, int32_t ninst, 
     sl.push_back(lhs[i]);
     sr.push_back(rhs[i]);
   }
-  ldf_batch_config best = tune(c, cands, s, [&](const ldf_batch_config& k) -> BatchImpl* {
+  auto make = [&](const ldf_batch_config& k) -> BatchImpl* {
     return create_join((int32_t)smp.size(), sl.data(), sr.data(), lk, rk, 1, jt, ne, lo, nlo, ro, nro, k, s);
-  });
+  };
+  ldf_batch_config best = tune(c, cands, s, make);
+  if (c.lds_slots == 0) {
+    ldf_batch_config lds = best;
+    lds.lds_slots        = 2048;
+    best                 = tune(c, {best, lds}, s, make);
+  }
   std::lock_guard<std::mutex> lk2(T.mu);
   T.store(key, best);
   return best;

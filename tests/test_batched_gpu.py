@@ -55,7 +55,7 @@ def _join_instances(rng, sizes, nullable):
 
 
 @pytest.mark.parametrize("cfg", range(len(CFGS)))
-@pytest.mark.parametrize("jt", [0, 1, 3, 4], ids=["inner", "left", "semi", "anti"])
+@pytest.mark.parametrize("jt", [0, 1, 2, 3, 4], ids=["inner", "left", "full", "semi", "anti"])
 def test_batch_join_matches_plain(gpu, jt, cfg):
     rng = np.random.default_rng(10 + jt)
     sizes = [(0, 0), (1, 0), (0, 5), (3000, 300), (1, 1), (17, 900), (50000, 5000), (1234, 123), (256, 2048),
@@ -84,6 +84,16 @@ def test_batch_join_nulls_equal_false_and_selected_columns(gpu):
         assert _rows(cols, offs[i], offs[i + 1]) == _rows(plain)
     # output nullability follows the plain path: LEFT makes the rhs columns nullable
     assert cols[0].validity is None and cols[1].validity is not None and cols[2].validity is not None
+    # FULL with nulls not equal: null build keys match nothing and come out as unmatched rhs rows
+    b = C.batch_join(L, R, [0], [0], 2, nulls_equal=False, config=CFGS[1])
+    for rep in range(2):
+        b.run()
+        cols, offs = b.result()
+        for i in range(3):
+            plain = C.join(L[i], R[i], [0], [0], 2, nulls_equal=False)
+            assert _rows(cols, offs[i], offs[i + 1]) == _rows(plain), (rep, i)
+    assert all(c.validity is not None for c in cols)  # FULL: every column nullable
+    b.close()
 
 
 def _gb_instances(rng, specs, knull=False, vnull=False):
@@ -165,8 +175,11 @@ def test_batch_autotune_and_identical_results(gpu):
 def test_batch_errors(gpu):
     rng = np.random.default_rng(1)
     L, R = _join_instances(rng, [(10, 10)], nullable=False)
-    with pytest.raises(C.LdfError):  # FULL runs through ldf_join
-        C.batch_join(L, R, [0], [0], 2, config=CFGS[0])
+    with pytest.raises(ValueError):  # not a JoinType
+        C.batch_join(L, R, [0], [0], 7, config=CFGS[0])
+    S = [[C.column_from_strings(["a", "b"])]]
+    with pytest.raises(C.LdfError):  # the key must be fixed width
+        C.batch_join(S, S, [0], [0], 0, config=CFGS[0])
     with pytest.raises(ValueError):
         C.batch_join(L, R, [0], [0], 0, config={"load_factor": 1.5, "tile_rows": 1024, "lds_slots": 512})
     with pytest.raises(ValueError):  # SEMI with rhs columns
