@@ -361,7 +361,8 @@ typedef struct ldf_query {
   int32_t probe_key;
   int32_t naggs;        /* GROUPBY */
   const ldf_agg* aggs;  /* GROUPBY (copied at submit) */
-  int64_t row_begin;    /* GROUPBY: a multiple of 64 */
+  int64_t row_begin;    /* GROUPBY: any row (a range not starting on a 64-row word reads a
+                           bit-shifted copy of the validity) */
   int64_t row_end;
 } ldf_query;
 

@@ -1964,6 +1964,30 @@ __global__ void k_copy_bits(const uint64_t* __restrict__ src, int64_t src_words,
   }
 }
 
+// Bits [off, off + n) of a validity bitmap of src_bytes bytes (no padding assumed: only those
+// bytes are read) into dst starting at bit 0, one 64-bit word per thread.
+__global__ void k_slice_bits(const uint8_t* __restrict__ src, int64_t src_bytes, int64_t off, int64_t n,
+                             uint64_t* __restrict__ dst)
+{
+  const int64_t nw = (n + 63) / 64;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nw; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b  = off + 64 * j;
+    const int64_t by = b >> 3;
+    const int sh     = (int)(b & 7);
+    uint64_t v       = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (by + k < src_bytes) v |= (uint64_t)src[by + k] << (8 * k);
+    if (sh != 0) {
+      v >>= sh;
+      if (by + 8 < src_bytes) v |= (uint64_t)src[by + 8] << (64 - sh);
+    }
+    const int64_t rem = n - 64 * j;
+    if (rem < 64) v &= (1ull << rem) - 1;
+    dst[j] = v;
+  }
+}
+
 int64_t now_ns()
 {
   return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
@@ -2158,10 +2182,11 @@ struct Server {
   void execute(std::vector<Query*>& batch, hipStream_t s)
   {
     Alloc a(nullptr, s);
+    std::vector<Buf<uint64_t>> keep;  // validity slices of unaligned row ranges
     Query& h = *batch.front();
     if (h.plain) {  // large groupby: the plain entry point on this stream
       const ServedTable& t = *tables[h.q.table];
-      std::vector<ldf_column_view> v = slice(t, h.q.row_begin, h.q.row_end);
+      std::vector<ldf_column_view> v = slice(t, h.q.row_begin, h.q.row_end, a, keep);
       ldf_table_view tv{(int32_t)v.size(), v.data()};
       h.cols.resize(1 + h.aggs.size());
       ldf_table_out out{(int32_t)h.cols.size(), h.cols.data()};
@@ -2199,7 +2224,7 @@ struct Server {
     } else {
       for (int k = 0; k < n; ++k) {
         const Query& q = *batch[k];
-        views[k]       = slice(*tables[q.q.table], q.q.row_begin, q.q.row_end);
+        views[k]       = slice(*tables[q.q.table], q.q.row_begin, q.q.row_end, a, keep);
         tv[k]          = {(int32_t)views[k].size(), views[k].data()};
       }
       const int32_t key = tables[h.q.table]->key_col;
@@ -2236,14 +2261,27 @@ struct Server {
     }
   }
 
-  // table rows [lo, hi) as views (validity sliced at a byte boundary: lo is a multiple of 64)
-  std::vector<ldf_column_view> slice(const ServedTable& t, int64_t lo, int64_t hi) const
+  // table rows [lo, hi) as views: values by pointer; validity by pointer when lo is a multiple
+  // of 64, else as a bit-shifted copy in `keep` (released in the stream's order)
+  std::vector<ldf_column_view> slice(const ServedTable& t, int64_t lo, int64_t hi, Alloc& a,
+                                     std::vector<Buf<uint64_t>>& keep) const
   {
     std::vector<ldf_column_view> v = t.cols;
     for (auto& c : v) {
       const int w = type_width(c.type_id);
       c.data      = static_cast<const uint8_t*>(c.data) + lo * w;
-      if (c.validity) c.validity += lo / 8;
+      if (c.validity && (lo & 63) == 0) {
+        c.validity += lo / 8;
+      } else if (c.validity) {
+        const int64_t n = hi - lo;
+        keep.emplace_back(a, (size_t)std::max<int64_t>(1, (n + 63) / 64));
+        if (n > 0) {
+          hipLaunchKernelGGL(k_slice_bits, dim3(grid_for((n + 63) / 64, 256)), dim3(256), 0, a.stream(), c.validity,
+                             (t.rows + 7) / 8, lo, n, keep.back().get());
+          LDF_HIP_TRY(hipGetLastError());
+        }
+        c.validity = reinterpret_cast<const uint8_t*>(keep.back().get());
+      }
       c.length     = hi - lo;
       c.null_count = -1;
     }
@@ -2368,7 +2406,6 @@ extern "C" int ldf_server_submit(ldf_server* server, const ldf_query* query, int
     } else if (query->kind == LDF_QUERY_GROUPBY) {
       LDF_EXPECTS(query->row_begin >= 0 && query->row_begin <= query->row_end && query->row_end <= t.rows,
                   "server: groupby rows out of range");
-      LDF_EXPECTS(query->row_begin % 64 == 0, "server: row_begin must be a multiple of 64");
       LDF_EXPECTS(query->naggs >= 0 && query->naggs <= kMaxBAgg && (query->naggs == 0 || query->aggs != nullptr),
                   "server: 0..16 aggregations");
       q->aggs.assign(query->aggs, query->aggs + query->naggs);

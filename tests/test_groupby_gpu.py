@@ -201,9 +201,11 @@ def test_groupby_nulls_and_float_keys(gpu):
 
 
 def test_groupby_c2_full_size_properties(gpu):
-    """The 100M-row, 1M-key Zipf(1.1) groupby at full size, checked through size-independent
-    properties: group count = distinct keys, per-key counts = bincount, integer sums / min /
-    max against a torch reduction by sorted key, and the global sum of float sums."""
+    """The 100M-row, 1M-key Zipf(1.1) groupby at full size (integer columns only; the bench's own
+    100M-row input with nulls, float sums and NUNIQUE is checked against the oracle in
+    test_configs_full_gpu.py), through size-independent properties: group count = distinct keys,
+    per-key counts = torch.unique's counts, integer sums / min / max against a torch reduction by
+    sorted key."""
     import torch
 
     from legate_dataframe import _capi as C
