@@ -128,10 +128,12 @@ __device__ inline uint32_t murmur3_bytes(const uint8_t* p, int64_t len, uint32_t
 struct FastMod {
   uint64_t m;
   uint32_t d;
+  uint32_t pow2;  // d is a power of two: a mod d = a & (d - 1)
 };
-inline FastMod fastmod_make(uint32_t d) { return FastMod{~0ull / d + 1, d}; }
+inline FastMod fastmod_make(uint32_t d) { return FastMod{~0ull / d + 1, d, (d & (d - 1)) == 0 ? 1u : 0u}; }
 __device__ __forceinline__ uint32_t fastmod(uint32_t a, const FastMod& f)
 {
+  if (f.pow2) return a & (f.d - 1);
   const uint64_t low = f.m * (uint64_t)a;
   return (uint32_t)__umul64hi(low, (uint64_t)f.d);
 }

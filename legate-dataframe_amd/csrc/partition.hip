@@ -12,10 +12,14 @@
 //                writes counts[p * nblk + b] (partition-major, so one exclusive scan of the
 //                whole array yields every (partition, block) write cursor);
 //   k_hp_scatter the same rows again, 8192 at a time: each wave ranks its 512 rows by
-//                partition with wave64 ballots (stable: rank = rows of the same partition
-//                earlier in the wave's chunk), the ranks are combined across the 16 waves in
-//                LDS, rows are staged in partition order in LDS and then written out as
-//                contiguous runs, so rows keep their input order inside a partition.
+//                partition (stable: rank = rows of the same partition earlier in the wave's
+//                chunk), the ranks are combined across the 16 waves in LDS, rows are staged
+//                in partition order in LDS and then written out as contiguous runs, so rows
+//                keep their input order inside a partition.  A group of 64 rows finds the
+//                lanes of each partition by having every lane OR its bit into a 64-bit LDS
+//                word per (wave, partition) and read it back: three LDS operations instead
+//                of one ballot and its mask arithmetic per partition bit (the ballots are kept
+//                for partition counts whose words do not fit the idle staging area).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -123,6 +127,7 @@ struct ScatterCols {
 
 // LDS layout (dynamic): cursor u64[P] | wtab u16[W][P] | total u32[P] | start u32[P] |
 //                       stage_row u16[kHpSub] | stage_part u16[kHpSub]
+template <bool ORM>
 __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t n, int64_t rows_per_block, uint32_t nparts,
                                                            FastMod fm, int pbits, const uint64_t* __restrict__ offsets,
                                                            ScatterCols sc, int64_t* __restrict__ perm)
@@ -147,7 +152,11 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t 
 
   for (int64_t sub = begin; sub < end; sub += kHpSub) {
     const int sub_n = (int)((end - sub) < kHpSub ? (end - sub) : kHpSub);
-    for (uint32_t i = threadIdx.x; i < kHpWaves * nparts; i += blockDim.x) wtab[i] = 0;
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(stage_r);  // ORM: lane masks, in the idle staging area
+    for (uint32_t i = threadIdx.x; i < kHpWaves * nparts; i += blockDim.x) {
+      wtab[i] = 0;
+      if (ORM) wmask[i] = 0;
+    }
     __syncthreads();
 
     // 1. per-wave stable ranks over the wave's contiguous 512-row chunk
@@ -159,6 +168,22 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t 
       const bool active = local < sub_n;
       uint32_t p = 0;
       if (active) p = fastmod(partition_row_hash(hc, sub + local), fm);
+      if (ORM) {  // as in k_hp_scatter_v
+        uint32_t rank_in_group = 0, old = 0;
+        if (active) {
+          uint64_t* w = wmask + wave * nparts + p;
+          __hip_atomic_fetch_or(w, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint64_t same  = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          old                  = wtab[wave * nparts + p];
+          const uint64_t below = same & lanes_below;
+          rank_in_group        = (uint32_t)__popcll(below);
+          if (below == 0) wtab[wave * nparts + p] = (uint16_t)(old + (uint32_t)__popcll(same));
+          __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        my_part[g] = (uint16_t)p;
+        my_rank[g] = (uint16_t)(old + rank_in_group);
+        continue;
+      }
       uint64_t same = __ballot(active);
       for (int b = 0; b < pbits; ++b) {
         uint64_t bb = __ballot(active && ((p >> b) & 1));
@@ -255,7 +280,7 @@ constexpr int kVSub     = 4096;
 constexpr int kVPerWave = kVSub / kVWaves;  // 512 contiguous rows per wave
 constexpr int kVGroups  = kVPerWave / kWave;
 
-template <typename T, int FW>
+template <typename T, int FW, bool ORM>
 __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restrict__ src, int64_t n, uint32_t seed,
                                                             int64_t rows_per_block, uint32_t nparts, FastMod fm, int pbits,
                                                             const uint64_t* __restrict__ offsets, T* __restrict__ dst)
@@ -294,7 +319,11 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
       const int64_t r = sub + kVSub + wave * kVPerWave + g * kWave + lane;
       next_val[g]     = r < end ? src[r] : T(0);
     }
-    for (uint32_t i = threadIdx.x; i < kVWaves * nparts; i += blockDim.x) wtab[i] = 0;
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(stage_v);  // ORM: lane masks, in the idle staging area
+    for (uint32_t i = threadIdx.x; i < kVWaves * nparts; i += blockDim.x) {
+      wtab[i] = 0;
+      if (ORM) wmask[i] = 0;
+    }
     __syncthreads();
 
     // 1. partition of each row; stable ranks within the wave's 512 rows
@@ -308,6 +337,25 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
       if (active)
         p = fastmod(hash_combine32(seed, murmur3_fixed(murmur_input_bits((uint64_t)my_val[g], FW), (int)sizeof(T), seed)),
                     fm);
+      if (ORM) {
+        // the lanes of partition p: each lane ORs its bit into the wave's word for p and reads
+        // the word back (LDS instructions of one wave complete in issue order, so the read sees
+        // every lane's bit), then the word is cleared for the next group
+        uint32_t rank_in_group = 0, old = 0;
+        if (active) {
+          uint64_t* w = wmask + wave * nparts + p;
+          __hip_atomic_fetch_or(w, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint64_t same = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          old                 = wtab[wave * nparts + p];
+          const uint64_t below = same & lanes_below;
+          rank_in_group        = (uint32_t)__popcll(below);
+          if (below == 0) wtab[wave * nparts + p] = (uint16_t)(old + (uint32_t)__popcll(same));
+          __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        my_part[g] = (uint16_t)p;
+        my_rank[g] = (uint16_t)(old + rank_in_group);
+        continue;
+      }
       uint64_t same = __ballot(active);
       for (int b = 0; b < pbits; ++b) {
         uint64_t bb = __ballot(active && ((p >> b) & 1));
@@ -421,13 +469,21 @@ static void launch_scatter_v(const ldf_column_view& c, int64_t n, uint32_t seed,
 {
   static bool attr = false;
   if (!attr) {
-    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T, FW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)kHpVLdsMax));
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T, FW, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHpVLdsMax));
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter_v<T, FW, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHpVLdsMax));
     attr = true;
   }
-  hipLaunchKernelGGL((k_hp_scatter_v<T, FW>), dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)), s,
-                     static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
-                     static_cast<T*>(dst));
+  const bool orm = (size_t)kVWaves * nparts * 8 <= (size_t)kVSub * sizeof(T);  // the lane masks fit the staging area
+  if (orm)
+    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, true>), dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)),
+                       s, static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
+                       static_cast<T*>(dst));
+  else
+    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, false>), dim3(nblk), dim3(kVThreads),
+                       scatter_v_lds_bytes(nparts, sizeof(T)), s, static_cast<const T*>(c.data), n, seed, rpb, nparts,
+                       fastmod_make(nparts), pbits, scan, static_cast<T*>(dst));
   LDF_HIP_TRY(hipGetLastError());
 }
 
@@ -437,14 +493,26 @@ static size_t scatter_lds_bytes(uint32_t nparts)
   return (b + 15) & ~size_t(15);
 }
 
-static void set_scatter_lds_attr()
+// The generic scatter; with at most 256 partitions the lane masks of its 16 waves (8 bytes per
+// wave and partition) fit the row staging area and rank the rows.
+static void launch_scatter(int64_t nblk, uint32_t nparts, hipStream_t s, const HashCols& hc, int64_t n, int64_t rpb,
+                           const FastMod& fm, int pbits, const uint64_t* scan, const ScatterCols& sc, int64_t* perm)
 {
   static bool done = false;
   if (!done) {
-    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)scatter_lds_bytes(kHpMaxParts)));
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_hp_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)scatter_lds_bytes(kHpMaxParts)));
     done = true;
   }
+  if ((size_t)kHpWaves * nparts * 8 <= (size_t)kHpSub * 4)  // stage_row and stage_part, contiguous
+    hipLaunchKernelGGL(k_hp_scatter<true>, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes(nparts), s, hc, n, rpb,
+                       nparts, fm, pbits, scan, sc, perm);
+  else
+    hipLaunchKernelGGL(k_hp_scatter<false>, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes(nparts), s, hc, n, rpb,
+                       nparts, fm, pbits, scan, sc, perm);
+  LDF_HIP_TRY(hipGetLastError());
 }
 
 }  // namespace ldf
@@ -540,10 +608,7 @@ static void partition_fixed(const ldf_table_view* table, const HashCols& hc, con
       sc.dst[i]   = dst[i];
       sc.width[i] = type_width(table->cols[i].type_id);
     }
-    set_scatter_lds_attr();
-    hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes((uint32_t)nparts), s, hc, n, rpb,
-                       (uint32_t)nparts, fm, pbits, (const uint64_t*)scan, sc, (int64_t*)nullptr);
-    LDF_HIP_TRY(hipGetLastError());
+    launch_scatter(nblk, (uint32_t)nparts, s, hc, n, rpb, fm, pbits, scan, sc, nullptr);
   }
   hipLaunchKernelGGL(k_hp_pick_offsets, dim3(grid_for(nparts + 1, 256)), dim3(256), 0, s, (const uint64_t*)scan,
                      (uint32_t)nparts, (uint32_t)nblk, offs);
@@ -644,12 +709,9 @@ extern "C" int ldf_hash_partition(const ldf_table_view* table, const int32_t* co
       Buf<int64_t> perm(a, n);
       ScatterCols sc;
       sc.n = 0;
-      set_scatter_lds_attr();
       {
         ProfScope prof("hp_scatter", s);
-        hipLaunchKernelGGL(k_hp_scatter, dim3(nblk), dim3(kHpThreads), scatter_lds_bytes((uint32_t)nparts), s, hc, n, rpb,
-                           (uint32_t)nparts, fm, pbits, (const uint64_t*)scan.get(), sc, perm.get());
-        LDF_HIP_TRY(hipGetLastError());
+        launch_scatter(nblk, (uint32_t)nparts, s, hc, n, rpb, fm, pbits, scan.get(), sc, perm.get());
       }
       Buf<int32_t> map(a, n);
       hipLaunchKernelGGL(k_i64_to_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, perm.get(), n, map.get());

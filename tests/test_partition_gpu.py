@@ -39,6 +39,22 @@ def test_partition_int64_exact(gpu, nparts, seed):
     assert got.equals(exp)
 
 
+@pytest.mark.parametrize("dtype", [np.int64, np.int32, np.uint16, np.int8])
+@pytest.mark.parametrize("nparts", [3, 64, 100, 256, 512, 1000, 2048])
+def test_partition_single_column_exact(gpu, dtype, nparts):
+    """One fixed-width column hashed on itself (the value-staged kernels): every width, power-of-two
+    and other partition counts, partition counts whose per-wave lane masks fit the staging area and
+    ones that do not; the partitioned column equals the oracle's stable partition row for row."""
+    rng = np.random.default_rng(nparts * 7 + np.dtype(dtype).itemsize)
+    info = np.iinfo(dtype)
+    n = 250_007
+    t = pa.table({"k": rng.integers(info.min, info.max, n, dtype=dtype, endpoint=True)})
+    got, offs = gpu_partition(t, ["k"], nparts, 0)
+    exp, eoffs = _expected(t, ["k"], nparts, 0)
+    assert list(offs) == list(eoffs)
+    assert got.equals(exp)
+
+
 def test_partition_nullable_multicolumn_strings(gpu):
     rng = np.random.default_rng(5)
     n = 100_000
