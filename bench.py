@@ -805,6 +805,22 @@ def copy_calibration(torch, nbytes=4 << 30, reps=5):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
+def hbm_ceiling(torch, C, nbytes=8_000_000_000, reps=3):
+    """What HBM gives on this GPU: the library's read and copy probe kernels (16-byte loads, four in
+    flight per thread) over 8 GB, best of `reps` per grid size; TB/s counts read plus write bytes."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    out = {"bytes": nbytes, "read": {}, "copy": {}}
+    for grid in (1024, 2048, 4096, 8192):
+        r = min(C.hbm_probe(src, dst, nbytes, False, grid) for _ in range(reps))
+        c = min(C.hbm_probe(src, dst, nbytes, True, grid) for _ in range(reps))
+        out["read"][grid] = {"ms": r, "tb_s": nbytes / (r * 1e-3) / 1e12}
+        out["copy"][grid] = {"ms": c, "tb_s": 2 * nbytes / (c * 1e-3) / 1e12}
+    out["read_best_tb_s"] = max(v["tb_s"] for v in out["read"].values())
+    out["copy_best_tb_s"] = max(v["tb_s"] for v in out["copy"].values())
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -822,6 +838,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="time the library's HBM read and copy probe kernels over 8 GB and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X (no collectives); "
@@ -829,7 +847,8 @@ def main():
         sys.exit(2)
 
     global CPU_POOL
-    if not args.no_cpu and args.workload in ("groupby", "join") and not args.copy_calibration:
+    if not args.no_cpu and args.workload in ("groupby", "join") and not args.copy_calibration \
+            and not args.hbm_ceiling:
         CPU_POOL = start_cpu_pool()  # before the GPU is initialised
     try:
         _main(args)
@@ -848,6 +867,9 @@ def _main(args):
     torch.cuda.set_device(0)
     if args.copy_calibration:
         copy_calibration(torch)
+        return
+    if args.hbm_ceiling:
+        hbm_ceiling(torch, C)
         return
     if args.workload == "groupby":
         res = run_groupby(args, torch, C)

@@ -264,6 +264,14 @@ void ldf_profile_enable(int on);
 void ldf_profile_reset(void);
 int ldf_profile_get(const char* kernel, int64_t* launches, double* total_ms);
 
+/* Measures what HBM gives on the current GPU: one launch that reads `nbytes` at `src` (copy = 0;
+ * `dst` then only needs 16 bytes) or copies them to `dst` (copy = 1), with 16-byte loads, four in
+ * flight per thread, over `nblocks` blocks of 256 threads; `*ms` is its duration from HIP events
+ * on `stream`.  A measurement helper with no counterpart in the reference: it gives the
+ * practical ceilings the kernels' bandwidth is compared with beside the published peak. */
+int ldf_hbm_probe(const void* src, void* dst, int64_t nbytes, int32_t copy, int32_t nblocks, double* ms,
+                  void* stream);
+
 /* Frees every buffer of an output table through `alloc` (NULL = library default). */
 void ldf_free_table(ldf_table_out* out, const ldf_allocator* alloc, void* stream);
 

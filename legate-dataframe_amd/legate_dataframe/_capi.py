@@ -31,7 +31,7 @@ EXPORTED = [
     "ldf_bools_to_bitmask", "ldf_bitmask_to_bools", "ldf_ranges_to_offsets",
     "ldf_offsets_to_ranges", "ldf_free_table", "ldf_last_error", "ldf_version",
     "ldf_profile_enable", "ldf_profile_reset", "ldf_profile_get", "ldf_pool_release", "ldf_pool_alloc",
-    "ldf_pool_free", "ldf_pool_stats",
+    "ldf_pool_free", "ldf_pool_stats", "ldf_hbm_probe",
     "ldf_batch_join_create", "ldf_batch_groupby_create", "ldf_batch_run", "ldf_batch_result",
     "ldf_batch_config_used", "ldf_batch_destroy", "ldf_server_create", "ldf_server_add_table",
     "ldf_server_submit", "ldf_server_wait", "ldf_server_release", "ldf_server_destroy",
@@ -128,6 +128,9 @@ def load(require: bool = True):
         L.ldf_pool_free.restype = None
         L.ldf_pool_stats.argtypes = [P(ctypes.c_int64)] * 4
         L.ldf_pool_stats.restype = None
+        L.ldf_hbm_probe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                    P(ctypes.c_double), ctypes.c_void_p]
+        L.ldf_hbm_probe.restype = ctypes.c_int
         L.ldf_profile_enable.argtypes = [ctypes.c_int]
         L.ldf_profile_get.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_double)]
         L.ldf_profile_get.restype = ctypes.c_int
@@ -506,6 +509,15 @@ def pool_stats() -> dict:
     v = [ctypes.c_int64() for _ in range(4)]
     load().ldf_pool_stats(*[ctypes.byref(x) for x in v])
     return dict(live_blocks=v[0].value, live_bytes=v[1].value, cached_blocks=v[2].value, cached_bytes=v[3].value)
+
+
+def hbm_probe(src, dst, nbytes: int, copy: bool, nblocks: int) -> float:
+    """Milliseconds of one read (copy=False) or copy (copy=True) of `nbytes` at device tensor
+    `src` (into `dst`) by the library's HBM probe kernels, on the current torch stream."""
+    ms = ctypes.c_double()
+    check(load().ldf_hbm_probe(src.data_ptr(), dst.data_ptr(), nbytes, int(copy), nblocks, ctypes.byref(ms),
+                               stream_ptr()))
+    return ms.value
 
 
 def profile_enable(on: bool = True) -> None:
