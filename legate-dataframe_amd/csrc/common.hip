@@ -433,8 +433,9 @@ void free_column(ldf_column* c, Alloc& a)
 
 // Gather of several columns by one map, so the map (4 B/row) is read once per output row for all
 // of them.  One wave covers 64 consecutive output rows so each column's validity for those rows
-// is one ballot.  String columns get their lengths here (into dst[c], uint64 per row); the
-// characters follow in k_gather_str_chars once the lengths are scanned into offsets.
+// is one ballot.  String columns get their lengths here (into dst[c], uint64 per row) and the sum
+// of each 64-row group's lengths (gsum[c][group]); once the group sums are scanned,
+// k_gather_str_chars writes each row's offset and characters.
 struct GatherSet {
   int ncols;
   int width[kMaxGather];  // 1, 2, 4, 8, or 0 for large_utf8
@@ -444,7 +445,16 @@ struct GatherSet {
   void* dst[kMaxGather];
   uint64_t* dval[kMaxGather];
   int64_t* start[kMaxGather];  // packed path: each output row's source start of a string column
+  uint64_t* gsum[kMaxGather];  // string column: the character count of each 64-row output group
 };
+
+// Sum over the wave's 64 lanes, in every lane.
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
 
 // Packed source rows for a gather through a random map: each source row's fields -- fixed-width
 // values, a string's start (8 bytes) and length (4 bytes), and a byte of validity bits when any
@@ -523,6 +533,10 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
           store_width(g.dst[c], g.width[c], i, m >= 0 ? rec_get<W>(w, ps.off[c], g.width[c]) : 0);
         }
       }
+      if (g.width[c] == 0) {  // lanes past n hold no row (ok is false)
+        const uint64_t t = wave_sum_u64(ok ? rec_get<W>(w, ps.loff[c], 4) : 0);
+        if (lane == 0) g.gsum[c][base >> 6] = t;
+      }
       if (g.dval[c] != nullptr) {
         const uint64_t b = __ballot(ok);
         if (lane == 0) {
@@ -579,6 +593,10 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
         if (gb >= n) break;  // the same for the whole wave
         const int64_t i = gb + lane;
         if (i < n) store_width(g.dst[c], g.width[c], i, val[j]);
+        if (g.width[c] == 0) {  // lanes past n hold 0
+          const uint64_t t = wave_sum_u64(val[j]);
+          if (lane == 0) g.gsum[c][gb >> 6] = t;
+        }
         if (g.dval[c] != nullptr) {
           const uint64_t bal = __ballot(ok[j]);
           if (lane == 0) {
@@ -598,7 +616,9 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
 }
 
 // Characters of gathered strings.  Each wave owns 64 consecutive output rows, whose characters
-// form one contiguous destination range [D0, D1).
+// form one contiguous destination range [D0, D1): D0 is the scanned sum of the lengths of the
+// groups before (gbase[group]), and a wave scan of the rows' lengths gives each row's offset,
+// which the wave also stores (dst_off; the last row's wave stores dst_off[n] too).
 //  * Rows of at most kShortStr bytes (the usual case): each lane copies its own row into the
 //    wave's LDS window -- aligned 16-byte source chunks, up to four loads in flight (an aligned
 //    chunk holding a byte of the string never leaves that byte's page), one LDS byte store per
@@ -622,7 +642,9 @@ __device__ __forceinline__ void wave_lds_sync()
 __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* __restrict__ src_off,
                                                                 const uint8_t* __restrict__ src_chars,
                                                                 const int32_t* __restrict__ map, int64_t n,
-                                                                const int64_t* __restrict__ dst_off,
+                                                                const uint64_t* __restrict__ lens,
+                                                                const uint64_t* __restrict__ gbase,
+                                                                int64_t* __restrict__ dst_off,
                                                                 uint8_t* __restrict__ dst_chars,
                                                                 const int64_t* __restrict__ row_start)
 {
@@ -634,31 +656,47 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
   const int64_t align16 = (int64_t)((uintptr_t)dst_chars & 15);
   // Waves work independently (all LDS state is per wave): wave w of the grid takes the 64-row
   // groups w, w + waves, ...  The group headers are software-pipelined: while a group's
-  // characters are copied, the next group's destination offsets and source starts and the
-  // following group's map entries are already in flight.
+  // characters are copied, the next group's lengths, base and source starts and the following
+  // group's map entries are already in flight.
   const int64_t step = (int64_t)gridDim.x * kStrBlock;
   int64_t base       = ((int64_t)blockIdx.x * (kStrBlock / 64) + wv) * 64;
   auto map_at = [&](int64_t b) -> int32_t { return b + lane < n ? map[b + lane] : -1; };
   auto start_of = [&](int32_t m) -> int64_t { return m >= 0 ? src_off[m] : 0; };
-  int64_t hb = 0, he = 0, hs = 0;  // this lane's destination range and source start
+  uint64_t hl = 0;                 // this lane's length
+  int64_t hd = 0, hs = 0;          // the group's base, this lane's source start
   int32_t m_next = -1;             // the next group's map entry of this lane
-  if (base < n && base + lane < n) {
-    hb = dst_off[base + lane];
-    he = dst_off[base + lane + 1];
-    hs = row_start != nullptr ? row_start[base + lane] : start_of(map[base + lane]);
+  if (base < n) {
+    hd = (int64_t)gbase[base >> 6];
+    if (base + lane < n) {
+      hl = lens[base + lane];
+      hs = row_start != nullptr ? row_start[base + lane] : start_of(map[base + lane]);
+    }
   }
   if (row_start == nullptr) m_next = map_at(base + step);
   for (; base < n; base += step) {
     const int nrows = (int)(n - base < 64 ? n - base : 64);
     const int64_t nb = base + step;
-    int64_t nhb = 0, nhe = 0, nhs = 0;
+    uint64_t nhl = 0;
+    int64_t nhd = 0, nhs = 0;
     int32_t m_after = -1;
-    if (nb + lane < n) {
-      nhb = dst_off[nb + lane];
-      nhe = dst_off[nb + lane + 1];
-      nhs = row_start != nullptr ? row_start[nb + lane] : start_of(m_next);
+    if (nb < n) {
+      nhd = (int64_t)gbase[nb >> 6];
+      if (nb + lane < n) {
+        nhl = lens[nb + lane];
+        nhs = row_start != nullptr ? row_start[nb + lane] : start_of(m_next);
+      }
     }
     if (row_start == nullptr) m_after = map_at(nb + step);
+    // this lane's destination range: the group's base plus the lengths of the lanes below
+    uint64_t incl = hl;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    const int64_t hb = hd + (int64_t)(incl - hl), he = hd + (int64_t)incl;
+    if (lane < nrows) dst_off[base + lane] = hb;
+    if (base + nrows == n && lane == nrows - 1) dst_off[n] = he;
     {
       const int64_t D0 = __shfl(hb, 0), D1 = __shfl(he, nrows - 1);
       const int64_t myb = lane < nrows ? hb : D1;
@@ -746,8 +784,8 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
       }
     }
     wave_lds_sync();  // the wave's LDS state is rewritten by its next group
-    hb     = nhb;
-    he     = nhe;
+    hl     = nhl;
+    hd     = nhd;
     hs     = nhs;
     m_next = m_after;
   }
@@ -759,7 +797,8 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
 {
   hipStream_t s = a.stream();
   int done      = 0;
-  std::vector<Buf<uint64_t>> lens;
+  std::vector<Buf<uint64_t>> lens, gsums;
+  const int64_t ngroups = (n_out + 63) / 64;
   try {
     for (int c0 = 0; c0 < ncols; c0 += kMaxGather) {
       const int nc = ncols - c0 < kMaxGather ? ncols - c0 : kMaxGather;
@@ -786,6 +825,8 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         if (w == 0) {
           lens.emplace_back(a, n_out > 0 ? n_out : 1);
           g.dst[j] = lens.back().get();
+          gsums.emplace_back(a, ngroups + 1);
+          g.gsum[j] = gsums.back().get();
         } else {
           g.dst[j] = o->data;
         }
@@ -856,16 +897,17 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         LDF_HIP_TRY(hipGetLastError());
       }
       recs.reset();
-      // strings: offsets from the lengths, then the characters
+      // strings: the group sums scanned into group bases, then each row's offset and characters
       size_t li = lens.size();
       for (int j = nc - 1; j >= 0; --j) {
         if (g.width[j] != 0) continue;
-        ldf_column* o           = &out[c0 + j];
+        ldf_column* o            = &out[c0 + j];
         const ldf_column_view& v = *src[c0 + j];
-        Buf<uint64_t>& len      = lens[--li];
-        exclusive_scan_u64(len.get(), reinterpret_cast<uint64_t*>(o->offsets), n_out, a);
-        len.reset();
-        const int64_t total = (int64_t)read_u64(reinterpret_cast<uint64_t*>(o->offsets) + n_out, s);
+        Buf<uint64_t>& len       = lens[--li];
+        Buf<uint64_t> gbase(a, ngroups + 1);
+        exclusive_scan_u64(gsums[li].get(), gbase.get(), ngroups, a);
+        gsums[li].reset();
+        const int64_t total = (int64_t)read_u64(gbase.get() + ngroups, s);
         a.deallocate(o->data, 1);
         o->data       = nullptr;
         o->data_bytes = total;
@@ -873,12 +915,17 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         if (total > 0) {
           ProfScope prof("gather_str", s);
           hipLaunchKernelGGL(k_gather_str_chars, dim3(grid_for(n_out, kStrBlock, 256 * 16)), dim3(kStrBlock), 0, s,
-                             v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, o->offsets,
-                             static_cast<uint8_t*>(o->data), (const int64_t*)g.start[j]);
+                             v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, (const uint64_t*)len.get(),
+                             (const uint64_t*)gbase.get(), o->offsets, static_cast<uint8_t*>(o->data),
+                             (const int64_t*)g.start[j]);
           LDF_HIP_TRY(hipGetLastError());
+        } else {
+          fill_u64(reinterpret_cast<uint64_t*>(o->offsets), 0, n_out + 1, s);  // every row empty
         }
+        len.reset();
       }
       lens.clear();
+      gsums.clear();
       starts.clear();
       if (null_counts != nullptr) continue;  // the caller reads them back
       uint64_t hc[kMaxGather];
