@@ -470,20 +470,23 @@ struct PackSet {
 };
 constexpr uint64_t kStartMask = (1ull << 56) - 1;
 
+// Columns are visited by a loop that is not unrolled (each descriptor read with scalar loads
+// when its turn comes), which keeps the scalar registers from spilling.
 __global__ __launch_bounds__(256) void k_pack_rows(GatherSet g, PackSet ps, int64_t n, unsigned char* __restrict__ rec)
 {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     RecWords w;
     uint32_t vb = 0;
-#pragma unroll
-    for (int c = 0; c < kMaxGather; ++c) {
-      if (c >= g.ncols) break;
-      if (g.width[c] > 0) {
-        w.put(ps.off[c], load_bits(g.src[c], g.width[c], r));
+#pragma unroll 1
+    for (int c = 0; c < g.ncols; ++c) {
+      const int width = g.width[c];
+      if (width > 0) {
+        w.put(ps.off[c], load_bits(g.src[c], width, r));
       } else {
-        const int64_t s0 = g.soff[c][r];
+        const int64_t* soff = g.soff[c];
+        const int64_t s0    = soff[r];
         w.put(ps.off[c], (uint64_t)s0);
-        w.put(ps.loff[c], (uint64_t)(uint32_t)(g.soff[c][r + 1] - s0));
+        w.put(ps.loff[c], (uint64_t)(uint32_t)(soff[r + 1] - s0));
       }
       if (valid_at(g.sval[c], r)) vb |= 1u << c;
     }
@@ -502,16 +505,19 @@ __device__ __forceinline__ void store_width(void* dst, int width, int64_t i, uin
   }
 }
 
-// k_gather_multi through packed records (W = record stride / 8).
+// k_gather_multi through packed records (W = record stride / 8).  The column loop is not
+// unrolled: each column's descriptor is read with scalar loads when its turn comes (unrolled,
+// the descriptors of every column slot stayed live at once and spilled scalar registers), and
+// null counts gather in LDS, one device atomic per column and block at the end.
 template <int W>
 __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, const unsigned char* __restrict__ rec,
                                                        const int32_t* __restrict__ map, int64_t n,
                                                        unsigned long long* __restrict__ null_count)
 {
+  __shared__ unsigned long long s_nulls[kMaxGather];
+  if (threadIdx.x < kMaxGather) s_nulls[threadIdx.x] = 0;
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  uint64_t nulls[kMaxGather];
-#pragma unroll
-  for (int c = 0; c < kMaxGather; ++c) nulls[c] = 0;
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = base + lane;
@@ -521,52 +527,51 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
     for (int k = 0; k < W; ++k) w[k] = 0;
     if (m >= 0) rec_load<W>(rec, m, w);
     const uint32_t vb = ps.voff >= 0 ? (uint32_t)rec_get<W>(w, ps.voff, 1) : 0xFFu;
-#pragma unroll
-    for (int c = 0; c < kMaxGather; ++c) {
-      if (c >= g.ncols) break;
-      const bool ok = m >= 0 && ((vb >> c) & 1u);
-      if (i < n) {
-        if (g.width[c] == 0) {
-          static_cast<uint64_t*>(g.dst[c])[i] = ok ? rec_get<W>(w, ps.loff[c], 4) : 0;
+#pragma unroll 1
+    for (int c = 0; c < g.ncols; ++c) {
+      const bool ok    = m >= 0 && ((vb >> c) & 1u);
+      const int width  = g.width[c];
+      if (width == 0) {
+        const uint64_t len = ok ? rec_get<W>(w, ps.loff[c], 4) : 0;  // lanes past n hold no row
+        if (i < n) {
+          static_cast<uint64_t*>(g.dst[c])[i] = len;
           g.start[c][i]                       = (int64_t)(rec_get<W>(w, ps.off[c], 8) & kStartMask);
-        } else {
-          store_width(g.dst[c], g.width[c], i, m >= 0 ? rec_get<W>(w, ps.off[c], g.width[c]) : 0);
         }
-      }
-      if (g.width[c] == 0) {  // lanes past n hold no row (ok is false)
-        const uint64_t t = wave_sum_u64(ok ? rec_get<W>(w, ps.loff[c], 4) : 0);
+        const uint64_t t = wave_sum_u64(len);
         if (lane == 0) g.gsum[c][base >> 6] = t;
+      } else if (i < n) {
+        store_width(g.dst[c], width, i, m >= 0 ? rec_get<W>(w, ps.off[c], width) : 0);
       }
-      if (g.dval[c] != nullptr) {
+      uint64_t* dval = g.dval[c];
+      if (dval != nullptr) {
         const uint64_t b = __ballot(ok);
         if (lane == 0) {
-          g.dval[c][base >> 6]   = b;
+          dval[base >> 6]        = b;
           const int64_t in_range = n - base < 64 ? n - base : 64;
-          nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(b);
+          const uint64_t nn      = (uint64_t)in_range - (uint64_t)__popcll(b);
+          if (nn != 0) atomicAdd(&s_nulls[c], (unsigned long long)nn);
         }
       }
     }
   }
-  if (lane == 0) {
-#pragma unroll
-    for (int c = 0; c < kMaxGather; ++c)
-      if (c < g.ncols && nulls[c] != 0) atomicAdd(&null_count[c], (unsigned long long)nulls[c]);
-  }
+  __syncthreads();
+  if (threadIdx.x < g.ncols && s_nulls[threadIdx.x] != 0) atomicAdd(&null_count[threadIdx.x], s_nulls[threadIdx.x]);
 }
 
 // kGatherGroups groups of 64 output rows per wave and iteration: the map loads of all groups are
 // issued, then each column's value loads for all groups, before the stores that use them, so a
-// wave keeps several random reads in flight.
+// wave keeps several random reads in flight.  Columns are visited by a loop that is not unrolled,
+// as in k_gather_packed.
 constexpr int kGatherGroups = 4;
 
 __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t* __restrict__ map, int64_t n,
                                                       unsigned long long* __restrict__ null_count)
 {
+  __shared__ unsigned long long s_nulls[kMaxGather];
+  if (threadIdx.x < kMaxGather) s_nulls[threadIdx.x] = 0;
+  __syncthreads();
   const int lane       = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  uint64_t nulls[kMaxGather];
-#pragma unroll
-  for (int c = 0; c < kMaxGather; ++c) nulls[c] = 0;
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) - lane; base < n;
        base += kGatherGroups * stride) {
     int32_t m[kGatherGroups];
@@ -575,44 +580,47 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
       const int64_t i = base + j * stride + lane;
       m[j]            = i < n ? map[i] : -1;
     }
-#pragma unroll
-    for (int c = 0; c < kMaxGather; ++c) {
-      if (c >= g.ncols) break;
+#pragma unroll 1
+    for (int c = 0; c < g.ncols; ++c) {
+      const int width       = g.width[c];
+      const uint8_t* sval   = g.sval[c];
+      const void* src       = g.src[c];
+      const int64_t* soff   = g.soff[c];
+      void* dst             = g.dst[c];
+      uint64_t* dval        = g.dval[c];
       uint64_t val[kGatherGroups];
       bool ok[kGatherGroups];
 #pragma unroll
       for (int j = 0; j < kGatherGroups; ++j) {
-        ok[j]  = m[j] >= 0 && valid_at(g.sval[c], m[j]);
+        ok[j]  = m[j] >= 0 && valid_at(sval, m[j]);
         val[j] = 0;
-        if (g.width[c] == 0) val[j] = ok[j] ? (uint64_t)(g.soff[c][m[j] + 1] - g.soff[c][m[j]]) : 0;
-        else if (m[j] >= 0) val[j] = load_bits(g.src[c], g.width[c], m[j]);
+        if (width == 0) val[j] = ok[j] ? (uint64_t)(soff[m[j] + 1] - soff[m[j]]) : 0;
+        else if (m[j] >= 0) val[j] = load_bits(src, width, m[j]);
       }
 #pragma unroll
       for (int j = 0; j < kGatherGroups; ++j) {
         const int64_t gb = base + j * stride;
         if (gb >= n) break;  // the same for the whole wave
         const int64_t i = gb + lane;
-        if (i < n) store_width(g.dst[c], g.width[c], i, val[j]);
-        if (g.width[c] == 0) {  // lanes past n hold 0
+        if (i < n) store_width(dst, width, i, val[j]);
+        if (width == 0) {  // lanes past n hold 0
           const uint64_t t = wave_sum_u64(val[j]);
           if (lane == 0) g.gsum[c][gb >> 6] = t;
         }
-        if (g.dval[c] != nullptr) {
+        if (dval != nullptr) {
           const uint64_t bal = __ballot(ok[j]);
           if (lane == 0) {
-            g.dval[c][gb >> 6]     = bal;
+            dval[gb >> 6]          = bal;
             const int64_t in_range = n - gb < 64 ? n - gb : 64;
-            nulls[c] += (uint64_t)in_range - (uint64_t)__popcll(bal);
+            const uint64_t nn      = (uint64_t)in_range - (uint64_t)__popcll(bal);
+            if (nn != 0) atomicAdd(&s_nulls[c], (unsigned long long)nn);
           }
         }
       }
     }
   }
-  if (lane == 0) {
-#pragma unroll
-    for (int c = 0; c < kMaxGather; ++c)
-      if (c < g.ncols && nulls[c] != 0) atomicAdd(&null_count[c], (unsigned long long)nulls[c]);
-  }
+  __syncthreads();
+  if (threadIdx.x < g.ncols && s_nulls[threadIdx.x] != 0) atomicAdd(&null_count[threadIdx.x], s_nulls[threadIdx.x]);
 }
 
 // Characters of gathered strings.  Each wave owns 64 consecutive output rows, whose characters
@@ -712,27 +720,46 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
         for (int64_t W = ((D0 + align16) & ~(int64_t)15) - align16; W < D1; W += kStrWin) {
           const int64_t We = W + kStrWin;
           const int64_t lo = myb > W ? myb : W, hi = mye < We ? mye : We;
-          if (lo < hi) {
-            const uintptr_t u0 = (uintptr_t)(src_chars + mys + (lo - myb));
-            const uintptr_t u1 = u0 + (uintptr_t)(hi - lo);
-            const int64_t off  = (lo - W) - (int64_t)u0;  // window index of source byte address u
-            // aligned 16-byte source chunks, four loads in flight per batch
-            for (uintptr_t a = u0 & ~(uintptr_t)15; a < u1; a += 64) {
-              uint4 q[4];
+          // the window starts zeroed: rows OR their bytes in, whole dwords at a time, so a dword
+          // shared by the end of one row and the start of the next needs no byte stores
+          reinterpret_cast<uint4*>(&s_win[wv][0])[lane]      = make_uint4(0, 0, 0, 0);
+          reinterpret_cast<uint4*>(&s_win[wv][0])[lane + 64] = make_uint4(0, 0, 0, 0);
+          static_assert(kStrWin == 64 * 32, "two 16-byte stores per lane clear the window");
+          wave_lds_sync();
+          const uintptr_t u0 = (uintptr_t)(src_chars + mys + (lo - myb));
+          const uintptr_t u1 = lo < hi ? u0 + (uintptr_t)(hi - lo) : u0;  // this lane's source bytes [u0, u1)
+          const bool has     = lo < hi;
+          const int64_t off  = (lo - W) - (int64_t)u0;                     // window index of source byte u: u + off
+          uint32_t* wdw      = reinterpret_cast<uint32_t*>(&s_win[wv][0]);
+          uint32_t carry     = 0;  // the source dword before the batch
+          // 64-byte batches of aligned source: four 16-byte loads in flight, then window dwords
+          // V_k (k = 0..16) = source bytes [a - 4 + 4k + sig, +4), sig chosen so that V_k lands on
+          // a whole window dword, each one byte-aligned out of two consecutive source dwords
+          for (uintptr_t a = u0 & ~(uintptr_t)15; __any(has && a < u1); a += 64) {
+            uint4 q[4] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0),
+                          make_uint4(0, 0, 0, 0)};
 #pragma unroll
-              for (int k = 0; k < 4; ++k)
-                if (a + 16 * k < u1) q[k] = *reinterpret_cast<const uint4*>(a + 16 * k);
+            for (int k = 0; k < 4; ++k)  // only chunks holding a byte of the row (never leaves its page)
+              if (has && a + 16 * k < u1) q[k] = *reinterpret_cast<const uint4*>(a + 16 * k);
+            const uint32_t d[18] = {carry,  q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w,
+                                    q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w, 0u};
+            // 32-bit offsets from a: the row's bytes [r0, r1), window index of byte a is wa
+            const int r0       = (int)((int64_t)u0 - (int64_t)a);
+            const int r1       = (int)((int64_t)u1 - (int64_t)a);
+            const int wa       = (int)((int64_t)a + off);
+            const uint32_t sig = (uint32_t)(-wa) & 3u;
 #pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                if (a + 16 * k >= u1) break;
-                const uint32_t wd[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
-#pragma unroll
-                for (int b = 0; b < 16; ++b) {
-                  const uintptr_t u = a + 16 * k + b;
-                  if (u >= u0 && u < u1) s_win[wv][(int64_t)u + off] = (uint8_t)(wd[b >> 2] >> (8 * (b & 3)));
-                }
+            for (int k = 0; k <= 16; ++k) {
+              const int t = 4 * k - 4 + (int)sig;  // V_k's first byte, from a
+              if (!__any(t < r1)) break;         // every lane is past its row's end
+              const int b0 = min(max(r0 - t, 0), 4), b1 = min(max(r1 - t, 0), 4);
+              if (b1 > b0) {
+                const uint32_t m = (b1 == 4 ? 0xFFFFFFFFu : (1u << (8 * b1)) - 1u) & ~((1u << (8 * b0)) - 1u);
+                const uint32_t v = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sig);
+                atomicOr(&wdw[(wa + t) >> 2], v & m);
               }
             }
+            carry = q[3].w;
           }
           wave_lds_sync();
           const int64_t lim = We < D1 ? We : D1;

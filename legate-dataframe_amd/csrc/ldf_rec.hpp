@@ -45,13 +45,23 @@ struct RecWords {
   }
 };
 
+// A field of a loaded record.  `off` and `width` are the same for every lane (launch
+// arguments), so the word is chosen by a scalar branch rather than by selecting over every word.
 template <int W>
 __device__ __forceinline__ uint64_t rec_get(const uint64_t (&w)[W], int off, int width)
 {
-  const int wi = off >> 3;
-  uint64_t x   = 0;
-#pragma unroll
-  for (int k = 0; k < W; ++k) x |= w[k] & (0ull - (uint64_t)(k == wi));
+  const int wi = __builtin_amdgcn_readfirstlane(off) >> 3;
+  uint64_t x;
+  switch (wi) {
+    case 0: x = w[0]; break;
+    case 1: x = w[W > 1 ? 1 : 0]; break;
+    case 2: x = w[W > 2 ? 2 : 0]; break;
+    case 3: x = w[W > 3 ? 3 : 0]; break;
+    case 4: x = w[W > 4 ? 4 : 0]; break;
+    case 5: x = w[W > 5 ? 5 : 0]; break;
+    case 6: x = w[W > 6 ? 6 : 0]; break;
+    default: x = w[W > 7 ? 7 : 0]; break;
+  }
   x >>= (off & 7) * 8;
   return width >= 8 ? x : x & ((1ull << (8 * width)) - 1);
 }
