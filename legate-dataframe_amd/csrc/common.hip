@@ -433,9 +433,9 @@ void free_column(ldf_column* c, Alloc& a)
 
 // Gather of several columns by one map, so the map (4 B/row) is read once per output row for all
 // of them.  One wave covers 64 consecutive output rows so each column's validity for those rows
-// is one ballot.  String columns get their lengths here (into dst[c], uint64 per row) and the sum
-// of each 64-row group's lengths (gsum[c][group]); once the group sums are scanned,
-// k_gather_str_chars writes each row's offset and characters.
+// is one ballot.  String columns get the sum of each 64-row group's lengths here (gsum[c][group])
+// and, through packed records, each row's length (uint32, into dst[c]) and source start; once the
+// group sums are scanned, k_gather_str_chars writes each row's offset and characters.
 struct GatherSet {
   int ncols;
   int width[kMaxGather];  // 1, 2, 4, 8, or 0 for large_utf8
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
       if (width == 0) {
         const uint64_t len = ok ? rec_get<W>(w, ps.loff[c], 4) : 0;  // lanes past n hold no row
         if (i < n) {
-          static_cast<uint64_t*>(g.dst[c])[i] = len;
+          static_cast<uint32_t*>(g.dst[c])[i] = (uint32_t)len;
           g.start[c][i]                       = (int64_t)(rec_get<W>(w, ps.off[c], 8) & kStartMask);
         }
         const uint64_t t = wave_sum_u64(len);
@@ -602,8 +602,8 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
         const int64_t gb = base + j * stride;
         if (gb >= n) break;  // the same for the whole wave
         const int64_t i = gb + lane;
-        if (i < n) store_width(dst, width, i, val[j]);
-        if (width == 0) {  // lanes past n hold 0
+        if (i < n && width != 0) store_width(dst, width, i, val[j]);
+        if (width == 0) {  // lanes past n hold 0; the characters kernel reads the length again
           const uint64_t t = wave_sum_u64(val[j]);
           if (lane == 0) g.gsum[c][gb >> 6] = t;
         }
@@ -650,7 +650,8 @@ __device__ __forceinline__ void wave_lds_sync()
 __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* __restrict__ src_off,
                                                                 const uint8_t* __restrict__ src_chars,
                                                                 const int32_t* __restrict__ map, int64_t n,
-                                                                const uint64_t* __restrict__ lens,
+                                                                const uint32_t* __restrict__ lens,
+                                                                const uint64_t* __restrict__ dvalid,
                                                                 const uint64_t* __restrict__ gbase,
                                                                 int64_t* __restrict__ dst_off,
                                                                 uint8_t* __restrict__ dst_chars,
@@ -669,31 +670,42 @@ __global__ __launch_bounds__(kStrBlock) void k_gather_str_chars(const int64_t* _
   const int64_t step = (int64_t)gridDim.x * kStrBlock;
   int64_t base       = ((int64_t)blockIdx.x * (kStrBlock / 64) + wv) * 64;
   auto map_at = [&](int64_t b) -> int32_t { return b + lane < n ? map[b + lane] : -1; };
-  auto start_of = [&](int32_t m) -> int64_t { return m >= 0 ? src_off[m] : 0; };
-  uint64_t hl = 0;                 // this lane's length
-  int64_t hd = 0, hs = 0;          // the group's base, this lane's source start
-  int32_t m_next = -1;             // the next group's map entry of this lane
-  if (base < n) {
-    hd = (int64_t)gbase[base >> 6];
-    if (base + lane < n) {
-      hl = lens[base + lane];
-      hs = row_start != nullptr ? row_start[base + lane] : start_of(map[base + lane]);
+  // A group's base `d` and this lane's length `l` and source start `st`: with `row_start`, the
+  // packed gather's start and uint32 length of each output row; otherwise from the map entry `m`
+  // and the source offsets, 0 for a null row (the output validity word the gather wrote, one
+  // per group, or no row at all: m < 0).
+  auto head = [&](int64_t b, int32_t m, uint64_t& l, int64_t& d, int64_t& st) {
+    l  = 0;
+    d  = 0;
+    st = 0;
+    if (b < n) {
+      d = (int64_t)gbase[b >> 6];
+      if (b + lane < n) {
+        if (row_start != nullptr) {
+          l  = lens[b + lane];
+          st = row_start[b + lane];
+        } else {
+          const bool ok = m >= 0 && (dvalid == nullptr || ((dvalid[b >> 6] >> lane) & 1));
+          if (ok) {
+            st = src_off[m];
+            l  = (uint64_t)(src_off[m + 1] - st);
+          }
+        }
+      }
     }
-  }
+  };
+  uint64_t hl;             // this lane's length
+  int64_t hd, hs;          // the group's base, this lane's source start
+  int32_t m_next = -1;     // the next group's map entry of this lane
+  head(base, row_start == nullptr ? map_at(base) : -1, hl, hd, hs);
   if (row_start == nullptr) m_next = map_at(base + step);
   for (; base < n; base += step) {
     const int nrows = (int)(n - base < 64 ? n - base : 64);
     const int64_t nb = base + step;
-    uint64_t nhl = 0;
-    int64_t nhd = 0, nhs = 0;
+    uint64_t nhl;
+    int64_t nhd, nhs;
     int32_t m_after = -1;
-    if (nb < n) {
-      nhd = (int64_t)gbase[nb >> 6];
-      if (nb + lane < n) {
-        nhl = lens[nb + lane];
-        nhs = row_start != nullptr ? row_start[nb + lane] : start_of(m_next);
-      }
-    }
+    head(nb, m_next, nhl, nhd, nhs);
     if (row_start == nullptr) m_after = map_at(nb + step);
     // this lane's destination range: the group's base plus the lengths of the lanes below
     uint64_t incl = hl;
@@ -824,7 +836,8 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
 {
   hipStream_t s = a.stream();
   int done      = 0;
-  std::vector<Buf<uint64_t>> lens, gsums;
+  std::vector<Buf<uint32_t>> lens;
+  std::vector<Buf<uint64_t>> gsums;
   const int64_t ngroups = (n_out + 63) / 64;
   try {
     for (int c0 = 0; c0 < ncols; c0 += kMaxGather) {
@@ -850,8 +863,7 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         g.soff[j]  = v.offsets;
         g.dval[j]  = reinterpret_cast<uint64_t*>(o->validity);
         if (w == 0) {
-          lens.emplace_back(a, n_out > 0 ? n_out : 1);
-          g.dst[j] = lens.back().get();
+          g.dst[j] = nullptr;  // lengths only on the packed path (below)
           gsums.emplace_back(a, ngroups + 1);
           g.gsum[j] = gsums.back().get();
         } else {
@@ -895,6 +907,8 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
             ps.loff[j] = offs[f++];
             starts.emplace_back(a, n_out);
             g.start[j] = starts.back().get();
+            lens.emplace_back(a, n_out);
+            g.dst[j] = lens.back().get();
             if (anyval && ps.voff < 0) ps.voff = ps.off[j] + 7;
           }
         }
@@ -925,15 +939,15 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
       }
       recs.reset();
       // strings: the group sums scanned into group bases, then each row's offset and characters
-      size_t li = lens.size();
+      size_t gi = gsums.size(), li = lens.size();
       for (int j = nc - 1; j >= 0; --j) {
         if (g.width[j] != 0) continue;
         ldf_column* o            = &out[c0 + j];
         const ldf_column_view& v = *src[c0 + j];
-        Buf<uint64_t>& len       = lens[--li];
+        const uint32_t* len      = packed ? lens[--li].get() : nullptr;
         Buf<uint64_t> gbase(a, ngroups + 1);
-        exclusive_scan_u64(gsums[li].get(), gbase.get(), ngroups, a);
-        gsums[li].reset();
+        exclusive_scan_u64(gsums[--gi].get(), gbase.get(), ngroups, a);
+        gsums[gi].reset();
         const int64_t total = (int64_t)read_u64(gbase.get() + ngroups, s);
         a.deallocate(o->data, 1);
         o->data       = nullptr;
@@ -942,14 +956,13 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         if (total > 0) {
           ProfScope prof("gather_str", s);
           hipLaunchKernelGGL(k_gather_str_chars, dim3(grid_for(n_out, kStrBlock, 256 * 16)), dim3(kStrBlock), 0, s,
-                             v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, (const uint64_t*)len.get(),
-                             (const uint64_t*)gbase.get(), o->offsets, static_cast<uint8_t*>(o->data),
-                             (const int64_t*)g.start[j]);
+                             v.offsets, static_cast<const uint8_t*>(v.data), map, n_out, len,
+                             (const uint64_t*)o->validity, (const uint64_t*)gbase.get(), o->offsets,
+                             static_cast<uint8_t*>(o->data), (const int64_t*)g.start[j]);
           LDF_HIP_TRY(hipGetLastError());
         } else {
           fill_u64(reinterpret_cast<uint64_t*>(o->offsets), 0, n_out + 1, s);  // every row empty
         }
-        len.reset();
       }
       lens.clear();
       gsums.clear();
