@@ -267,17 +267,18 @@ __global__ __launch_bounds__(kHpThreads) void k_hp_scatter(HashCols hc, int64_t 
 }
 
 // One fixed-width, non-null column: the same stable scatter with 512-thread blocks over
-// 4096-row steps, but each thread keeps the values it loaded for hashing in registers (and loads
-// the next step's values while this step is ranked and staged), and the values themselves are
-// staged in LDS, so the column is read once here and written once as contiguous runs.  The
-// smaller LDS plan lets three blocks share a CU, so one block's ranking overlaps another's
-// memory traffic.
+// 7168-row steps, but each thread keeps the values it loaded for hashing in registers and the
+// values themselves are staged in LDS, so the column is read once here and written once as
+// contiguous runs.  Two blocks share a CU (an int64 step stages 70 KB), so one block's ranking
+// overlaps the other's memory traffic; the step is as long as that allows, because longer steps
+// write longer runs per partition (28 rows of 256 partitions at a time; 4096-row steps took 6.05 ms
+// at 1B keys, 6144 5.37 ms, 7168 5.32 ms; larger steps leave one block per CU or spill registers).
 // LDS layout (dynamic): cursor u64[P] | wtab u16[W][P] | total u32[P] | start u32[P] |
 //                       stage_val T[kVSub] | stage_part u16[kVSub]
 constexpr int kVThreads = 512;
 constexpr int kVWaves   = kVThreads / kWave;
-constexpr int kVSub     = 4096;
-constexpr int kVPerWave = kVSub / kVWaves;  // 512 contiguous rows per wave
+constexpr int kVSub     = 7168;
+constexpr int kVPerWave = kVSub / kVWaves;  // 896 contiguous rows per wave
 constexpr int kVGroups  = kVPerWave / kWave;
 
 template <typename T, int FW, bool ORM>
@@ -304,20 +305,13 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
   const int64_t begin = blockIdx.x * rows_per_block;
   const int64_t end   = begin + rows_per_block < n ? begin + rows_per_block : n;
 
-  T next_val[kVGroups];
-#pragma unroll
-  for (int g = 0; g < kVGroups; ++g) {
-    const int64_t r = begin + wave * kVPerWave + g * kWave + lane;
-    next_val[g]     = r < end ? src[r] : T(0);
-  }
   for (int64_t sub = begin; sub < end; sub += kVSub) {
     const int sub_n = (int)((end - sub) < kVSub ? (end - sub) : kVSub);
     T my_val[kVGroups];
 #pragma unroll
     for (int g = 0; g < kVGroups; ++g) {
-      my_val[g]       = next_val[g];
-      const int64_t r = sub + kVSub + wave * kVPerWave + g * kWave + lane;
-      next_val[g]     = r < end ? src[r] : T(0);
+      const int64_t r = sub + wave * kVPerWave + g * kWave + lane;
+      my_val[g]       = r < end ? src[r] : T(0);
     }
     uint64_t* wmask = reinterpret_cast<uint64_t*>(stage_v);  // ORM: lane masks, in the idle staging area
     for (uint32_t i = threadIdx.x; i < kVWaves * nparts; i += blockDim.x) {
@@ -327,8 +321,7 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
     __syncthreads();
 
     // 1. partition of each row; stable ranks within the wave's 512 rows
-    uint16_t my_part[kVGroups];
-    uint16_t my_rank[kVGroups];
+    uint32_t my_pr[kVGroups];  // partition << 16 | rank, one register per row
 #pragma unroll
     for (int g = 0; g < kVGroups; ++g) {
       const int local   = wave * kVPerWave + g * kWave + lane;
@@ -352,8 +345,7 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
           if (below == 0) wtab[wave * nparts + p] = (uint16_t)(old + (uint32_t)__popcll(same));
           __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        my_part[g] = (uint16_t)p;
-        my_rank[g] = (uint16_t)(old + rank_in_group);
+        my_pr[g] = (p << 16) | (old + rank_in_group);
         continue;
       }
       uint64_t same = __ballot(active);
@@ -370,8 +362,7 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
         wtab[wave * nparts + p] = (uint16_t)(old + cnt);
       }
       old        = __shfl(old, leader < 0 ? 0 : leader, kWave);
-      my_part[g] = (uint16_t)p;
-      my_rank[g] = (uint16_t)(old + rank_in_group);
+      my_pr[g] = (p << 16) | (old + rank_in_group);
     }
     __syncthreads();
 
@@ -410,8 +401,8 @@ __global__ __launch_bounds__(kVThreads, 4) void k_hp_scatter_v(const T* __restri
     for (int g = 0; g < kVGroups; ++g) {
       const int local = wave * kVPerWave + g * kWave + lane;
       if (local < sub_n) {
-        const uint32_t p   = my_part[g];
-        const uint32_t pos = start[p] + wtab[wave * nparts + p] + my_rank[g];
+        const uint32_t p   = my_pr[g] >> 16;
+        const uint32_t pos = start[p] + wtab[wave * nparts + p] + (my_pr[g] & 0xFFFFu);
         stage_v[pos]       = my_val[g];
         stage_p[pos]       = (uint16_t)p;
       }
@@ -475,15 +466,16 @@ static void launch_scatter_v(const ldf_column_view& c, int64_t n, uint32_t seed,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHpVLdsMax));
     attr = true;
   }
-  const bool orm = (size_t)kVWaves * nparts * 8 <= (size_t)kVSub * sizeof(T);  // the lane masks fit the staging area
+  const size_t lds = scatter_v_lds_bytes(nparts, sizeof(T));
+  const bool orm   = (size_t)kVWaves * nparts * 8 <= (size_t)kVSub * sizeof(T);  // the lane masks fit the staging area
   if (orm)
-    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, true>), dim3(nblk), dim3(kVThreads), scatter_v_lds_bytes(nparts, sizeof(T)),
-                       s, static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
+    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, true>), dim3(nblk), dim3(kVThreads), lds, s,
+                       static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
                        static_cast<T*>(dst));
   else
-    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, false>), dim3(nblk), dim3(kVThreads),
-                       scatter_v_lds_bytes(nparts, sizeof(T)), s, static_cast<const T*>(c.data), n, seed, rpb, nparts,
-                       fastmod_make(nparts), pbits, scan, static_cast<T*>(dst));
+    hipLaunchKernelGGL((k_hp_scatter_v<T, FW, false>), dim3(nblk), dim3(kVThreads), lds, s,
+                       static_cast<const T*>(c.data), n, seed, rpb, nparts, fastmod_make(nparts), pbits, scan,
+                       static_cast<T*>(dst));
   LDF_HIP_TRY(hipGetLastError());
 }
 
