@@ -811,12 +811,16 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
     KT k[R];
     bool kv[R], dv[R];  // dv: the row has a COUNT_DISTINCT value (only its presence is counted here)
+    // branch-free loads, all in flight before any is used (rows past the end load row `begin`)
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const int64_t r = r0 + (int64_t)j * blockDim.x;
-      kv[j]           = r < end && valid_at(kc.valid, r);
-      k[j]            = kv[j] ? (KT)load_bits(kc.data, kc.width, r) : (KT)0;
-      dv[j]           = p.dpart && r < end && valid_at(dc.valid, r);
+      const int64_t r   = r0 + (int64_t)j * blockDim.x;
+      const uint32_t in = r < end ? 1u : 0u;
+      const int64_t rr  = in ? r : begin;
+      const uint32_t k1 = valid_bit_nb(kc.valid, kc.data, rr) & in;
+      kv[j]             = k1 != 0;
+      k[j]              = (KT)(load_bits_nb(kc.data, kc.width, rr) & (0ull - (uint64_t)k1));
+      dv[j]             = (valid_bit_nb(p.dpart ? dc.valid : nullptr, kc.data, rr) & in & (uint32_t)p.dpart) != 0;
     }
     int hh[R];
     uint32_t cpj[R];
@@ -1008,29 +1012,28 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
     uint64_t kb[kGbRowsPerThread];
     bool kv[kGbRowsPerThread];
     uint32_t code[kGbRowsPerThread];
+    // every load of the thread's rows is issued before any is used (branch-free loads combined by
+    // masks; rows past the end load the block's first row and are masked afterwards)
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
-      const int64_t r = sub + (int64_t)j * blockDim.x + threadIdx.x;
-      v[j]            = RowVals{0, 0, 0, 0, 0};
-      kb[j]           = 0;
-      kv[j]           = false;
-      code[j]         = 0;
-      if (r < end) {
-        code[j]              = codes[r];
-        kv[j]                = valid_at(kc.valid, r);
-        kb[j]                = kv[j] ? load_bits(kc.data, kc.width, r) : 0;
-        uint32_t vm          = 0;
-        uint64_t x[kMaxCols] = {0, 0, 0, 0};
+      const int64_t r  = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      const int64_t rr = r < end ? r : begin;
+      const uint64_t in = 0ull - (uint64_t)(r < end);
+      code[j]          = (uint32_t)codes[rr] & (uint32_t)in;
+      const uint32_t k1 = valid_bit_nb(kc.valid, kc.data, rr) & (uint32_t)in;
+      kv[j]            = k1 != 0;
+      kb[j]            = load_bits_nb(kc.data, kc.width, rr) & (0ull - (uint64_t)k1);
+      uint32_t vm      = 0;
+      uint64_t x[kMaxCols];
 #pragma unroll
-        for (int c = 0; c < kMaxCols; ++c) {
-          if (c < p.ncol) {
-            const ColDev& cd = p.col[c];
-            if (valid_at(cd.valid, r)) vm |= 1u << c;
-            x[c] = load_bits(cd.data, cd.width, r);
-          }
-        }
-        v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
+      for (int c = 0; c < kMaxCols; ++c) {  // columns past p.ncol load the key column's row
+        const uint32_t use = c < p.ncol ? 1u : 0u;
+        const ColDev& cd   = p.col[use ? c : 0];
+        const void* d      = use ? cd.data : kc.data;
+        x[c]               = load_bits_nb(d, use ? cd.width : kc.width, rr) & (0ull - (uint64_t)use) & in;
+        vm |= (valid_bit_nb(use ? cd.valid : nullptr, d, rr) & use & (uint32_t)in) << c;
       }
+      v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
     }
     int hx[kGbRowsPerThread];  // hot index; -1 not hot; -2 past the end
     uint32_t part[kGbRowsPerThread];

@@ -28,6 +28,36 @@ __device__ __forceinline__ uint64_t load_bits(const void* data, int width, int64
   }
 }
 
+// Branch-free forms of valid_at / load_bits for loops that load several rows and columns before
+// using any of them.  With a branch around each load (the null bitmap test, the switch on the
+// width) the compiler waits for every load before issuing the next one.  These issue plain loads
+// whose results are combined only by masks (no selects, which the compiler would turn back into
+// branches around the loads), so a thread's loads are all in flight together.  `i` / `row` must
+// be a row of the column (callers clamp rows past the end).
+
+// Validity bit of row i; a column without a bitmap reads byte i / 8 of `any` instead (any buffer
+// of at least that many bytes: the column's values) and reports 1.
+__device__ __forceinline__ uint32_t valid_bit_nb(const uint8_t* v, const void* any, int64_t i)
+{
+  const uint8_t* p = (v != nullptr ? v : static_cast<const uint8_t*>(any)) + (i >> 3);
+  return (((uint32_t)*p >> (uint32_t)(i & 7)) & 1u) | (uint32_t)(v == nullptr);
+}
+
+// width 8: two dwords; width 1, 2, 4: the dword holding the value (in the same page as the
+// value, so never out of bounds) shifted and masked, and that dword loaded a second time
+__device__ __forceinline__ uint64_t load_bits_nb(const void* data, int width, int64_t row)
+{
+  const uint8_t* b     = static_cast<const uint8_t*>(data) + row * (int64_t)width;
+  const uint32_t mis   = (uint32_t)reinterpret_cast<uintptr_t>(b) & 3u;  // pointer arithmetic only:
+  const uint32_t* w    = reinterpret_cast<const uint32_t*>(b - mis);    // the loads stay global loads
+  const uint32_t wide  = (uint32_t)width >> 3;                           // 1 for width 8
+  const uint32_t lo    = w[0];
+  const uint32_t hi    = w[wide];
+  const uint32_t m     = width >= 4 ? 0xffffffffu : ((1u << (8 * width)) - 1u);
+  const uint64_t hpart = (uint64_t)(hi & (0u - wide)) << 32;
+  return hpart | (uint64_t)((lo >> (8u * mis)) & m);
+}
+
 // Sign-extended integer value of a fixed-width integer column.
 __device__ __forceinline__ int64_t load_signed(const void* data, int width, int64_t row)
 {
