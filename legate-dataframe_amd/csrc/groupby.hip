@@ -115,6 +115,9 @@ struct GbPlan {
   int32_t part_bits;  // partitioned path: 2^part_bits partitions
   int32_t acc_end;    // LDS bytes of the hot accumulators (the hot key table follows them)
   int32_t stage;      // k_gb_main3 packs partition records through LDS (0: in registers)
+  // per value column, the index in `a` of its VCOUNT, ISUM, FSUM, MIN and MAX accumulator (-1:
+  // none; the lowering keeps at most one accumulator per kind and column)
+  int8_t cacc[kMaxCols][5];
 };
 
 struct HotList {
@@ -873,20 +876,109 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   if (p.dpart && threadIdx.x == 0) hcounts[blockIdx.x] = (uint32_t)block_pairs;
 }
 
+template <int C>
+__device__ __forceinline__ uint64_t colv(const RowVals& v)
+{
+  if constexpr (C == 0) return v.b0;
+  else if constexpr (C == 1) return v.b1;
+  else if constexpr (C == 2) return v.b2;
+  else return v.b3;
+}
+
+// The VCOUNT / ISUM / FSUM / MIN / MAX accumulators of value column C (a compile-time index, so a
+// row's value is a fixed register) over R rows of one thread; s[j] is the row's LDS slot (< 0:
+// not aggregated here).  `off` gives each accumulator's LDS byte offset (the hot table's or the
+// partition table's layout).
+template <int C, int R, typename Off>
+__device__ __forceinline__ void acc_col_rows(const GbPlan& p, const Off& off, unsigned char* lds, uint32_t* flags,
+                                             const int (&s)[R], const RowVals (&v)[R])
+{
+  const int ct = p.col[C].type;
+  const int cw = p.col[C].width;
+  bool ok[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) ok[j] = s[j] >= 0 && ((v[j].valid >> C) & 1);
+  int ai = p.cacc[C][0];
+  if (ai >= 0) {  // VCOUNT
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + off[ai]);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (ok[j]) atomicAdd(&cnt[s[j]], 1u);
+  }
+  ai = p.cacc[C][1];
+  if (ai >= 0) {  // ISUM
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + off[ai]);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (ok[j]) atomicAdd(&acc[s[j]], (unsigned long long)bits_as_int(colv<C>(v[j]), ct, cw));
+  }
+  ai = p.cacc[C][2];
+  if (ai >= 0) {  // FSUM
+    unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + off[ai]);
+    bool bad                 = false;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (ok[j] && !lds_fsum_add(&base[6 * s[j]], bits_as_double(colv<C>(v[j]), ct, cw))) bad = true;
+    if (bad) *p.a[ai].err = 1;
+  }
+#pragma unroll
+  for (int mm = 0; mm < 2; ++mm) {  // MIN, MAX
+    ai = p.cacc[C][3 + mm];
+    if (ai < 0) continue;
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + off[ai]);
+    const uint32_t fbit     = 1u << p.a[ai].flag_bit;
+    const bool fl           = is_float_type(ct);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (!ok[j]) continue;
+      const uint64_t vb = colv<C>(v[j]);
+      if (fl && isnan(bits_as_double(vb, ct, cw))) {
+        atomicOr(&flags[s[j]], fbit);
+        continue;
+      }
+      const uint64_t o = ordered_of(vb, ct, cw);
+      if (mm == 0) atomicMin(&acc[s[j]], (unsigned long long)o);
+      else atomicMax(&acc[s[j]], (unsigned long long)o);
+    }
+  }
+}
+
+template <int R, typename Off>
+__device__ __forceinline__ void acc_cols_rows(const GbPlan& p, const Off& off, unsigned char* lds, uint32_t* flags,
+                                              const int (&s)[R], const RowVals (&v)[R])
+{
+  if (p.ncol > 0) acc_col_rows<0, R>(p, off, lds, flags, s, v);
+  if (p.ncol > 1) acc_col_rows<1, R>(p, off, lds, flags, s, v);
+  if (p.ncol > 2) acc_col_rows<2, R>(p, off, lds, flags, s, v);
+  if (p.ncol > 3) acc_col_rows<3, R>(p, off, lds, flags, s, v);
+}
+
+// LDS byte offset of accumulator i: in the hot-key tables (the plan's own offsets) or in a
+// partition table (k_gb_part3's layout)
+struct HotOff {
+  const GbPlan& p;
+  __device__ int32_t operator[](int i) const { return p.a[i].lds_off; }
+};
+struct PartOff {
+  const int32_t* o;
+  __device__ int32_t operator[](int i) const { return o[i]; }
+};
+
 // LDS accumulators of R rows of one thread at once (h[j] = hot index, < 0 for rows that are not
-// hot): the accumulator loop is outside and the row loop inside, so each accumulator's kind,
-// column and type are resolved once per R rows.
+// hot): per value column (acc_col_rows), then the PRODUCT and COUNT_DISTINCT accumulators.
 template <bool DPART, int R>
 __device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds, const DKeys& keys, int64_t r0,
                                              const int (&h)[R], const HotList& hot, const RowVals (&v)[R])
 {
+  const HotOff off{p};
   uint32_t* rows = reinterpret_cast<uint32_t*>(lds + p.rows_off);
 #pragma unroll
   for (int j = 0; j < R; ++j)
     if (h[j] >= 0) atomicAdd(&rows[h[j]], 1u);
+  acc_cols_rows<R>(p, off, lds, reinterpret_cast<uint32_t*>(lds + p.flags_off), h, v);
   for (int i = 0; i < p.nacc; ++i) {
     const AccDev& a = p.a[i];
-    if (a.kind == ACC_ROWS) continue;
+    if (a.kind != ACC_PROD && a.kind != ACC_DISTINCT) continue;
     const int col = a.col;
     const int ct  = p.col[col].type;
     const int cw  = p.col[col].width;
@@ -894,48 +986,6 @@ __device__ __forceinline__ void acc_hot_rows(const GbPlan& p, unsigned char* lds
 #pragma unroll
     for (int j = 0; j < R; ++j) ok[j] = h[j] >= 0 && ((v[j].valid >> col) & 1);
     switch (a.kind) {
-      case ACC_VCOUNT: {
-        uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + a.lds_off);
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j]) atomicAdd(&cnt[h[j]], 1u);
-        break;
-      }
-      case ACC_ISUM: {
-        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j]) atomicAdd(&acc[h[j]], (unsigned long long)bits_as_int(pick(v[j], col), ct, cw));
-        break;
-      }
-      case ACC_FSUM: {
-        unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
-        bool bad                 = false;
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j] && !lds_fsum_add(&base[6 * h[j]], bits_as_double(pick(v[j], col), ct, cw))) bad = true;
-        if (bad) *a.err = 1;
-        break;
-      }
-      case ACC_MIN:
-      case ACC_MAX: {
-        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + a.lds_off);
-        uint32_t* flags         = reinterpret_cast<uint32_t*>(lds + p.flags_off);
-        const bool fl           = is_float_type(ct);
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          if (!ok[j]) continue;
-          const uint64_t vb = pick(v[j], col);
-          if (fl && isnan(bits_as_double(vb, ct, cw))) {
-            atomicOr(&flags[h[j]], 1u << a.flag_bit);
-            continue;
-          }
-          const uint64_t o = ordered_of(vb, ct, cw);
-          if (a.kind == ACC_MIN) atomicMin(&acc[h[j]], (unsigned long long)o);
-          else atomicMax(&acc[h[j]], (unsigned long long)o);
-        }
-        break;
-      }
       case ACC_PROD: {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -1234,68 +1284,14 @@ __device__ __forceinline__ void acc_part_rows(const GbPlan& p, const PartPlan& q
     if (slot[j] >= 0) atomicAdd(&rows[slot[j]], 1u);
   int di = 0;
   for (int i = 0; i < p.nacc; ++i) {
-    const AccDev& a = p.a[i];
-    if (a.kind == ACC_ROWS) continue;
-    if (a.kind == ACC_DISTINCT) {
-      uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + q.acc_off[i]);
+    if (p.a[i].kind != ACC_DISTINCT) continue;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + q.acc_off[i]);
 #pragma unroll
-      for (int j = 0; j < R; ++j)
-        if (slot[j] >= 0 && ((fl[j] >> (kFlagDistinct + di)) & 1)) atomicAdd(&cnt[slot[j]], 1u);
-      ++di;
-      continue;
-    }
-    const int col = a.col;
-    const int ct  = p.col[col].type;
-    const int cw  = p.col[col].width;
-    bool ok[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) ok[j] = slot[j] >= 0 && ((v[j].valid >> col) & 1);
-    switch (a.kind) {
-      case ACC_VCOUNT: {
-        uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + q.acc_off[i]);
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j]) atomicAdd(&cnt[slot[j]], 1u);
-        break;
-      }
-      case ACC_ISUM: {
-        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j]) atomicAdd(&acc[slot[j]], (unsigned long long)bits_as_int(pick(v[j], col), ct, cw));
-        break;
-      }
-      case ACC_FSUM: {
-        unsigned long long* base = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
-        bool bad                 = false;
-#pragma unroll
-        for (int j = 0; j < R; ++j)
-          if (ok[j] && !lds_fsum_add(&base[6 * slot[j]], bits_as_double(pick(v[j], col), ct, cw))) bad = true;
-        if (bad) *a.err = 1;
-        break;
-      }
-      case ACC_MIN:
-      case ACC_MAX: {
-        unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds + q.acc_off[i]);
-        uint32_t* flags         = reinterpret_cast<uint32_t*>(lds + q.flags_off);
-        const bool isf          = is_float_type(ct);
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          if (!ok[j]) continue;
-          const uint64_t vb = pick(v[j], col);
-          if (isf && isnan(bits_as_double(vb, ct, cw))) {
-            atomicOr(&flags[slot[j]], 1u << a.flag_bit);
-            continue;
-          }
-          const uint64_t o = ordered_of(vb, ct, cw);
-          if (a.kind == ACC_MIN) atomicMin(&acc[slot[j]], (unsigned long long)o);
-          else atomicMax(&acc[slot[j]], (unsigned long long)o);
-        }
-        break;
-      }
-      default: break;
-    }
+    for (int j = 0; j < R; ++j)
+      if (slot[j] >= 0 && ((fl[j] >> (kFlagDistinct + di)) & 1)) atomicAdd(&cnt[slot[j]], 1u);
+    ++di;
   }
+  acc_cols_rows<R>(p, PartOff{q.acc_off}, lds, reinterpret_cast<uint32_t*>(lds + q.flags_off), slot, v);
 }
 
 // the same into the global arrays at slot s (rows a full partition table could not take)
@@ -1382,9 +1378,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   for (int64_t i0 = begin + threadIdx.x; i0 < end; i0 += (int64_t)R * blockDim.x) {
     uint64_t w[R][W];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {  // issue the loads of R rows before using any
-      const int64_t i = i0 + (int64_t)j * blockDim.x;
-      if (i < end) rec_load<W>(pb.rec, i, w[j]);
+    for (int j = 0; j < R; ++j) {  // issue the loads of R rows before using any (unconditional:
+      const int64_t i = i0 + (int64_t)j * blockDim.x;  // a row past the end loads row `begin`)
+      rec_load<W>(pb.rec, i < end ? i : begin, w[j]);
     }
     int slot[R];  // LDS slot; -1 spilled to the global table; -2 not a row of this block
     uint32_t fl[R];
@@ -1531,6 +1527,7 @@ __device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned
 // scan of those counts (partition-major).  Each tile of kPairTile pairs is sorted by partition in
 // LDS and every partition's pairs of the tile are stored as one contiguous run (order inside a
 // partition does not matter to the deduplication).
+constexpr int kPairBatch = 8;   // pairs a thread loads before using any (deduplication loops)
 constexpr int kPairTile = 16384;  // 16 pairs per partition and tile on average: runs of whole lines
 
 __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long* __restrict__ src,
@@ -1779,9 +1776,23 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
     if (tries == kColdKeySlots) bad |= 4;
   }
   __syncthreads();
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    unsigned long long pr;
-    if (cold_pair(pb, dcol, dwidth, r, pr)) atomicAdd(&gh[cold_group(pair_hash(pr), ng)], 1u);
+  if (pb.cpairs != nullptr) {  // appended pairs: a thread's loads in flight together
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kPairBatch * blockDim.x) {
+      unsigned long long pr[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const int64_t i = r + (int64_t)u * blockDim.x;
+        pr[u]           = pb.cpairs[i < r1 ? i : r0];
+      }
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u)
+        if (r + (int64_t)u * blockDim.x < r1) atomicAdd(&gh[cold_group(pair_hash(pr[u]), ng)], 1u);
+    }
+  } else {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      unsigned long long pr;
+      if (cold_pair(pb, dcol, dwidth, r, pr)) atomicAdd(&gh[cold_group(pair_hash(pr), ng)], 1u);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1794,33 +1805,58 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
     gstart[ng] = run;
   }
   __syncthreads();
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    unsigned long long pr;
-    if (cold_pair(pb, dcol, dwidth, r, pr)) fine[atomicAdd(&gcur[cold_group(pair_hash(pr), ng)], 1ull)] = pr;
+  if (pb.cpairs != nullptr) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kPairBatch * blockDim.x) {
+      unsigned long long pr[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const int64_t i = r + (int64_t)u * blockDim.x;
+        pr[u]           = pb.cpairs[i < r1 ? i : r0];
+      }
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u)
+        if (r + (int64_t)u * blockDim.x < r1) fine[atomicAdd(&gcur[cold_group(pair_hash(pr[u]), ng)], 1ull)] = pr[u];
+    }
+  } else {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      unsigned long long pr;
+      if (cold_pair(pb, dcol, dwidth, r, pr)) fine[atomicAdd(&gcur[cold_group(pair_hash(pr), ng)], 1ull)] = pr;
+    }
   }
   __syncthreads();
   for (int g = 0; g < ng; ++g) {
     for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
     __syncthreads();
-    for (int64_t r = r0 + gstart[g] + threadIdx.x; r < r0 + gstart[g + 1]; r += blockDim.x) {
-      const unsigned long long pr = fine[r];
-      const int k                 = pair_set_insert(set, pr, pair_hash(pr));
-      if (k < 0) bad |= 2;
-      if (k <= 0) continue;
-      const unsigned long long kb = pr >> 32;
-      uint32_t i                  = (uint32_t)(mix64(kb) & (kColdKeySlots - 1));
-      int found                   = -1;
-      for (int tries = 0; tries < kColdKeySlots; ++tries) {
-        unsigned long long cur = kkeys[i];
-        if (cur == kGEmpty) cur = atomicCAS(&kkeys[i], kGEmpty, kb);
-        if (cur == kGEmpty || cur == kb) {
-          found = (int)i;
-          break;
-        }
-        i = (i + 1) & (kColdKeySlots - 1);
+    const int64_t g0 = r0 + gstart[g], g1 = r0 + gstart[g + 1];
+    for (int64_t r = g0 + threadIdx.x; r < g1; r += (int64_t)kPairBatch * blockDim.x) {
+      unsigned long long prs[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const int64_t i = r + (int64_t)u * blockDim.x;
+        prs[u]          = fine[i < g1 ? i : g0];
       }
-      if (found < 0) bad |= 4;
-      else atomicAdd(&kcnt[found], 1u);
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        if (r + (int64_t)u * blockDim.x >= g1) continue;
+        const unsigned long long pr = prs[u];
+        const int k                 = pair_set_insert(set, pr, pair_hash(pr));
+        if (k < 0) bad |= 2;
+        if (k <= 0) continue;
+        const unsigned long long kb = pr >> 32;
+        uint32_t i                  = (uint32_t)(mix64(kb) & (kColdKeySlots - 1));
+        int found                   = -1;
+        for (int tries = 0; tries < kColdKeySlots; ++tries) {
+          unsigned long long cur = kkeys[i];
+          if (cur == kGEmpty) cur = atomicCAS(&kkeys[i], kGEmpty, kb);
+          if (cur == kGEmpty || cur == kb) {
+            found = (int)i;
+            break;
+          }
+          i = (i + 1) & (kColdKeySlots - 1);
+        }
+        if (found < 0) bad |= 4;
+        else atomicAdd(&kcnt[found], 1u);
+      }
     }
     __syncthreads();
   }
@@ -1888,14 +1924,27 @@ __global__ __launch_bounds__(1024) void k_gb_sample_lds(DKeys keys, int64_t n, i
   const DCol& c      = keys.c[0];
   const int64_t i0   = (int64_t)blockIdx.x * kSampleBlockRows;
   const int64_t iend = i0 + kSampleBlockRows < S ? i0 + kSampleBlockRows : S;
-  for (int64_t i = i0 + threadIdx.x; i < iend; i += blockDim.x) {
-    const int64_t r = i * stride;
-    if (r >= n) continue;
-    if (!valid_at(c.valid, r)) {
+  constexpr int U = kSampleBlockRows / 1024;  // rows per thread, loaded together
+  static_assert(U * 1024 == kSampleBlockRows, "one pass of 1024 threads");
+  unsigned long long ks[U];
+  uint32_t kv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {  // branch-free loads (rows outside the sample load row 0)
+    const int64_t i  = i0 + threadIdx.x + (int64_t)u * 1024;
+    const int64_t r  = i * stride;
+    const bool in    = i < iend && r < n;
+    const int64_t rr = in ? r : 0;
+    kv[u]            = valid_bit_nb(c.valid, c.data, rr) | (in ? 0u : 2u);  // bit 1: not a sampled row
+    ks[u]            = load_bits_nb(c.data, c.width, rr);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (kv[u] & 2u) continue;
+    if (!(kv[u] & 1u)) {
       atomicAdd(&s_null, 1u);
       continue;
     }
-    const unsigned long long k = load_bits(c.data, c.width, r);
+    const unsigned long long k = ks[u];
     if (k == kGEmpty) {
       atomicAdd(&s_empty, 1u);
       continue;
@@ -2406,11 +2455,20 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     plan.flags_off = take(hot_cap * 4);
   }
   int flag_bits = 0;
+  std::memset(plan.cacc, -1, sizeof(plan.cacc));
   for (int i = 0; i < nacc; ++i) {
     AccDev& d = plan.a[i];
     d.kind    = low.acc[i].kind;
     d.col     = low.acc[i].col < 0 ? -1 : (int32_t)(std::find(colmap.begin(), colmap.end(), low.acc[i].col) - colmap.begin());
     d.lds_off = -1;
+    {
+      const int ki = d.kind == ACC_VCOUNT ? 0 : d.kind == ACC_ISUM ? 1 : d.kind == ACC_FSUM ? 2 : d.kind == ACC_MIN ? 3
+                   : d.kind == ACC_MAX  ? 4 : -1;
+      if (ki >= 0) {
+        LDF_EXPECTS(plan.cacc[d.col][ki] < 0, "groupby: two accumulators of one kind on one column");
+        plan.cacc[d.col][ki] = (int8_t)i;
+      }
+    }
     if (hot_cap > 0) {
       switch (d.kind) {
         case ACC_VCOUNT:
