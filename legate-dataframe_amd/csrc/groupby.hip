@@ -810,6 +810,8 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   const DCol& kc      = keys.c[0];
   const ColDev& dc    = p.col[p.dpart ? p.dcol : 0];
   constexpr int R = PACK32 ? 4 : 8;  // rows per thread whose loads are in flight together
+  const BlockCol bk = block_col(kc.data, kc.valid, kc.width, begin < end ? begin : 0);
+  const BlockCol bd = block_col(kc.data, p.dpart ? dc.valid : nullptr, kc.width, begin < end ? begin : 0);
   using KT = typename std::conditional<PACK32, uint32_t, uint64_t>::type;
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)R * blockDim.x) {
     KT k[R];
@@ -819,11 +821,11 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
     for (int j = 0; j < R; ++j) {
       const int64_t r   = r0 + (int64_t)j * blockDim.x;
       const uint32_t in = r < end ? 1u : 0u;
-      const int64_t rr  = in ? r : begin;
-      const uint32_t k1 = valid_bit_nb(kc.valid, kc.data, rr) & in;
+      const uint32_t o  = in ? (uint32_t)(r - begin) : 0u;
+      const uint32_t k1 = bc_valid(bk, o) & in;
       kv[j]             = k1 != 0;
-      k[j]              = (KT)(load_bits_nb(kc.data, kc.width, rr) & (0ull - (uint64_t)k1));
-      dv[j]             = (valid_bit_nb(p.dpart ? dc.valid : nullptr, kc.data, rr) & in & (uint32_t)p.dpart) != 0;
+      k[j]              = (KT)(bc_bits(bk, o) & (0ull - (uint64_t)k1));
+      dv[j]             = (bc_valid(bd, o) & in & (uint32_t)p.dpart) != 0;
     }
     int hh[R];
     uint32_t cpj[R];
@@ -1032,7 +1034,7 @@ __device__ __forceinline__ void lds_put(unsigned char* at, int width, uint64_t b
 // fields are written before it is stored as 16-byte pieces.  With DPART the hot rows'
 // (hot index, value) pairs are appended in row order to the block's range of pb.hpairs (one
 // LDS counter, coalesced stores); k_gb_pair_part then splits them by pair partition.
-template <bool DPART>
+template <bool DPART, int NC>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp, const uint16_t* __restrict__ codes)
 {
@@ -1057,31 +1059,34 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
   const int64_t begin = (int64_t)blockIdx.x * p.rows_per_block;
   const int64_t end   = begin + p.rows_per_block < n ? begin + p.rows_per_block : n;
   const DCol& kc      = keys.c[0];
+  // the block's rows through 32-bit offsets from `begin` (rows_per_block < 2^31)
+  const BlockCol bk         = block_col(kc.data, kc.valid, kc.width, begin);
+  const uint16_t* cbase     = codes + begin;
+  BlockCol bcol[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) bcol[c] = block_col(p.col[c].data, p.col[c].valid, p.col[c].width, begin);
   for (int64_t sub = begin; sub < end; sub += kSub3) {
     RowVals v[kGbRowsPerThread];
     uint64_t kb[kGbRowsPerThread];
     bool kv[kGbRowsPerThread];
     uint32_t code[kGbRowsPerThread];
-    // every load of the thread's rows is issued before any is used (branch-free loads combined by
-    // masks; rows past the end load the block's first row and are masked afterwards)
+    // every load of the thread's rows is issued before any is used (one branch-free load per
+    // value, combined by masks; rows past the end load the block's first row and are masked)
 #pragma unroll
     for (int j = 0; j < kGbRowsPerThread; ++j) {
-      const int64_t r  = sub + (int64_t)j * blockDim.x + threadIdx.x;
-      const int64_t rr = r < end ? r : begin;
-      const uint64_t in = 0ull - (uint64_t)(r < end);
-      code[j]          = (uint32_t)codes[rr] & (uint32_t)in;
-      const uint32_t k1 = valid_bit_nb(kc.valid, kc.data, rr) & (uint32_t)in;
-      kv[j]            = k1 != 0;
-      kb[j]            = load_bits_nb(kc.data, kc.width, rr) & (0ull - (uint64_t)k1);
-      uint32_t vm      = 0;
-      uint64_t x[kMaxCols];
+      const int64_t r   = sub + (int64_t)j * blockDim.x + threadIdx.x;
+      const uint32_t in = r < end ? 1u : 0u;
+      const uint32_t o  = in ? (uint32_t)(r - begin) : 0u;
+      code[j]           = (uint32_t)cbase[o] & (0u - in);
+      const uint32_t k1 = bc_valid(bk, o) & in;
+      kv[j]             = k1 != 0;
+      kb[j]             = bc_bits(bk, o) & (0ull - (uint64_t)k1);
+      uint32_t vm       = 0;
+      uint64_t x[kMaxCols] = {0, 0, 0, 0};
 #pragma unroll
-      for (int c = 0; c < kMaxCols; ++c) {  // columns past p.ncol load the key column's row
-        const uint32_t use = c < p.ncol ? 1u : 0u;
-        const ColDev& cd   = p.col[use ? c : 0];
-        const void* d      = use ? cd.data : kc.data;
-        x[c]               = load_bits_nb(d, use ? cd.width : kc.width, rr) & (0ull - (uint64_t)use) & in;
-        vm |= (valid_bit_nb(use ? cd.valid : nullptr, d, rr) & use & (uint32_t)in) << c;
+      for (int c = 0; c < NC; ++c) {
+        x[c] = bc_bits(bcol[c], o) & (0ull - (uint64_t)in);
+        vm |= (bc_valid(bcol[c], o) & in) << c;
       }
       v[j] = RowVals{x[0], x[1], x[2], x[3], vm};
     }
@@ -2287,6 +2292,42 @@ static Lowered lower(const ldf_table_view* tv, const ldf_agg* aggs, int32_t nagg
   return L;
 }
 
+// k_gb_main3 for the plan's number of value columns (a template argument: the row loads are
+// unrolled over exactly the columns read); the LDS limit is raised once per instantiation
+template <bool DPART, int NC>
+static void launch_main3_nc(dim3 grid, int lds, hipStream_t s, const DKeys& keys, int64_t n, const GbPlan& p,
+                            const HotList& hot, const PartBufs& pb, const HotParts& hp, const uint16_t* codes)
+{
+  static bool attr = [] {
+    LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<DPART, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL((k_gb_main3<DPART, NC>), grid, dim3(kGbThreads), lds, s, keys, n, p, hot, pb, hp, codes);
+  LDF_HIP_TRY(hipGetLastError());
+}
+
+template <bool DPART>
+static void launch_main3_d(int nc, dim3 grid, int lds, hipStream_t s, const DKeys& keys, int64_t n, const GbPlan& p,
+                           const HotList& hot, const PartBufs& pb, const HotParts& hp, const uint16_t* codes)
+{
+  switch (nc) {
+    case 0: launch_main3_nc<DPART, 0>(grid, lds, s, keys, n, p, hot, pb, hp, codes); break;
+    case 1: launch_main3_nc<DPART, 1>(grid, lds, s, keys, n, p, hot, pb, hp, codes); break;
+    case 2: launch_main3_nc<DPART, 2>(grid, lds, s, keys, n, p, hot, pb, hp, codes); break;
+    case 3: launch_main3_nc<DPART, 3>(grid, lds, s, keys, n, p, hot, pb, hp, codes); break;
+    default: launch_main3_nc<DPART, 4>(grid, lds, s, keys, n, p, hot, pb, hp, codes); break;
+  }
+}
+
+static void launch_main3(bool dpart, int nc, dim3 grid, int lds, hipStream_t s, const DKeys& keys, int64_t n,
+                         const GbPlan& p, const HotList& hot, const PartBufs& pb, const HotParts& hp,
+                         const uint16_t* codes)
+{
+  if (dpart) launch_main3_d<true>(nc, grid, lds, s, keys, n, p, hot, pb, hp, codes);
+  else launch_main3_d<false>(nc, grid, lds, s, keys, n, p, hot, pb, hp, codes);
+}
+
 template <bool GENERIC>
 static void launch_main(const DKeys& keys, int64_t n, const KeyTab& t, const GbPlan& p, const HotList& hot, int nblk,
                         hipStream_t s)
@@ -2521,6 +2562,14 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     const char* env = std::getenv("LDF_GROUPBY_PARTITIONED");
     if (env != nullptr && env[0] == '0') use_v3 = false;
   }
+  // its row passes load each value as the aligned 8-byte word holding it: values must be
+  // naturally aligned (Arrow buffers are)
+  {
+    auto aligned = [](const void* d, int w) { return w <= 0 || (reinterpret_cast<uintptr_t>(d) % (uintptr_t)w) == 0; };
+    if (!aligned(keys.c[0].data, keys.c[0].width)) use_v3 = false;
+    for (size_t c = 0; c < colmap.size(); ++c)
+      if (!aligned(tv->cols[colmap[c]].data, type_width(tv->cols[colmap[c]].type_id))) use_v3 = false;
+  }
   int ndistinct = 0;
   for (auto& h : low.acc) {
     if (h.kind == ACC_PROD) use_v3 = false;
@@ -2706,8 +2755,6 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         if (!attr) {
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_hist3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
-          LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_main3<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kMain3Lds));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<1>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
           LDF_HIP_TRY(hipFuncSetAttribute((const void*)k_gb_part3<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
@@ -2833,13 +2880,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       // 5. the pass over the rows
       {
         ProfScope prof("gb_aggregate", s);
-        if (dpart)
-          hipLaunchKernelGGL(k_gb_main3<true>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp,
-                             (const uint16_t*)codes.get());
-        else
-          hipLaunchKernelGGL(k_gb_main3<false>, dim3(nblk), dim3(kGbThreads), main_lds, s, keys, n, plan, hot, pb, hp,
-                             (const uint16_t*)codes.get());
-        LDF_HIP_TRY(hipGetLastError());
+        launch_main3(dpart, plan.ncol, dim3(nblk), main_lds, s, keys, n, plan, hot, pb, hp, (const uint16_t*)codes.get());
       }
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
       LDF_HIP_TRY(hipGetLastError());

@@ -58,6 +58,45 @@ __device__ __forceinline__ uint64_t load_bits_nb(const void* data, int width, in
   return hpart | (uint64_t)((lo >> (8u * mis)) & m);
 }
 
+// A fixed-width column seen from a block that reads rows [begin, begin + 2^31): row begin + o is
+// addressed by the 32-bit offset o.  A value is one aligned 8-byte load -- the aligned word that
+// holds a naturally aligned value of width <= 8 (column data must be aligned to its width) lies in
+// the value's page -- shifted and masked: one load instruction and no branch per value.
+struct BlockCol {
+  const uint8_t* data;   // the value of row `begin`
+  const uint8_t* valid;  // the bitmap byte of row begin & ~7 (the values when there is no bitmap)
+  uint64_t mask;         // all ones for width 8
+  uint32_t width;
+  uint32_t vshift;  // begin & 7
+  uint32_t nonull;  // 1: no bitmap (every row valid)
+};
+
+__device__ __forceinline__ BlockCol block_col(const void* data, const uint8_t* valid, int width, int64_t begin)
+{
+  BlockCol c;
+  c.data   = static_cast<const uint8_t*>(data) + begin * (int64_t)width;
+  c.valid  = valid != nullptr ? valid + (begin >> 3) : c.data;
+  c.mask   = width >= 8 ? ~0ull : ((1ull << (8 * width)) - 1ull);
+  c.width  = (uint32_t)width;
+  c.vshift = (uint32_t)(begin & 7);
+  c.nonull = valid == nullptr ? 1u : 0u;
+  return c;
+}
+
+__device__ __forceinline__ uint64_t bc_bits(const BlockCol& c, uint32_t o)
+{
+  const uint8_t* a   = c.data + (uint64_t)o * c.width;
+  const uint32_t mis = (uint32_t)reinterpret_cast<uintptr_t>(a) & 7u;
+  const uint64_t w   = *reinterpret_cast<const uint64_t*>(a - mis);
+  return (w >> (8u * mis)) & c.mask;
+}
+
+__device__ __forceinline__ uint32_t bc_valid(const BlockCol& c, uint32_t o)
+{
+  const uint32_t i = o + c.vshift;
+  return (((uint32_t)c.valid[i >> 3] >> (i & 7u)) & 1u) | c.nonull;
+}
+
 // Sign-extended integer value of a fixed-width integer column.
 __device__ __forceinline__ int64_t load_signed(const void* data, int width, int64_t row)
 {
