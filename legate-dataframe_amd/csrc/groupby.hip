@@ -1557,7 +1557,7 @@ __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long*
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = threadIdx.x + j * kHPairParts;
-      pr[j]       = i < tn ? src[t0 + i] : 0ull;
+      pr[j]       = src[t0 + (i < tn ? i : 0)];  // unconditional (past the tile: its first pair)
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -1613,7 +1613,17 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
   if (threadIdx.x < kHPairFine) fh[threadIdx.x] = 0;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x) atomicAdd(&fh[fine_part(pairs[r])], 1u);
+  for (int64_t r = b0 + threadIdx.x; r < b1; r += (int64_t)kPairBatch * blockDim.x) {
+    unsigned long long pr[kPairBatch];  // a thread's loads in flight together (past the end: pair b0)
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) {
+      const int64_t i = r + (int64_t)u * blockDim.x;
+      pr[u]           = pairs[i < b1 ? i : b0];
+    }
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u)
+      if (r + (int64_t)u * blockDim.x < b1) atomicAdd(&fh[fine_part(pr[u])], 1u);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t run = 0;
@@ -1636,9 +1646,9 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
     unsigned long long pr[R];
     uint32_t fb[R], rank[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
+    for (int j = 0; j < R; ++j) {  // unconditional loads (past the tile: its first pair), masked
       const int i = threadIdx.x + j * 1024;
-      pr[j]       = i < tn ? pairs[t0 + i] : 0ull;
+      pr[j]       = pairs[t0 + (i < tn ? i : 0)];
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -1674,13 +1684,23 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
     for (int pass = 0; pass < passes; ++pass) {
       for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
       __syncthreads();
-      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const unsigned long long pr = fine[r];
-        const uint64_t hsh          = pair_hash(pr);
-        if (passes > 1 && (int)((hsh >> 24) % (uint64_t)passes) != pass) continue;
-        const int k = pair_set_insert(set, pr, hsh);
-        if (k > 0) atomicAdd(&cnt[(uint32_t)(pr >> 32)], 1u);
-        else if (k < 0) bad = 1;
+      constexpr int U = 4;  // a fine partition holds ~4 pairs per thread: its loads in one batch
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)U * blockDim.x) {
+        unsigned long long pr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t i = r + (int64_t)u * blockDim.x;
+          pr[u]           = fine[i < r1 ? i : r0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (r + (int64_t)u * blockDim.x >= r1) break;
+          const uint64_t hsh = pair_hash(pr[u]);
+          if (passes > 1 && (int)((hsh >> 24) % (uint64_t)passes) != pass) continue;
+          const int k = pair_set_insert(set, pr[u], hsh);
+          if (k > 0) atomicAdd(&cnt[(uint32_t)(pr[u] >> 32)], 1u);
+          else if (k < 0) bad = 1;
+        }
       }
       __syncthreads();
     }
