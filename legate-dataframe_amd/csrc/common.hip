@@ -472,26 +472,60 @@ constexpr uint64_t kStartMask = (1ull << 56) - 1;
 
 // Columns are visited by a loop that is not unrolled (each descriptor read with scalar loads
 // when its turn comes), which keeps the scalar registers from spilling.
+// Each thread packs kPackRows rows per pass: a column's loads for all of them are issued (branch
+// free, rows past the end read row n - 1) before any is put into its record.
+constexpr int kPackRows = 4;
+
 __global__ __launch_bounds__(256) void k_pack_rows(GatherSet g, PackSet ps, int64_t n, unsigned char* __restrict__ rec)
 {
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    RecWords w;
-    uint32_t vb = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r0 < n; r0 += kPackRows * stride) {
+    RecWords w[kPackRows];
+    uint32_t vb[kPackRows];
+    int64_t rr[kPackRows];
+#pragma unroll
+    for (int k = 0; k < kPackRows; ++k) {
+      const int64_t r = r0 + k * stride;
+      rr[k]           = r < n ? r : n - 1;
+      vb[k]           = 0;
+    }
 #pragma unroll 1
     for (int c = 0; c < g.ncols; ++c) {
-      const int width = g.width[c];
+      const int width   = g.width[c];
+      const void* any   = width > 0 ? g.src[c] : static_cast<const void*>(g.soff[c]);
+      const uint8_t* sv = g.sval[c];
+      uint32_t v[kPackRows];
+#pragma unroll
+      for (int k = 0; k < kPackRows; ++k) v[k] = valid_bit_nb(sv, any, rr[k]);
       if (width > 0) {
-        w.put(ps.off[c], load_bits(g.src[c], width, r));
+        uint64_t x[kPackRows];
+#pragma unroll
+        for (int k = 0; k < kPackRows; ++k) x[k] = load_bits_nb(g.src[c], width, rr[k]);
+#pragma unroll
+        for (int k = 0; k < kPackRows; ++k) w[k].put(ps.off[c], x[k]);
       } else {
         const int64_t* soff = g.soff[c];
-        const int64_t s0    = soff[r];
-        w.put(ps.off[c], (uint64_t)s0);
-        w.put(ps.loff[c], (uint64_t)(uint32_t)(soff[r + 1] - s0));
+        int64_t a0[kPackRows], a1[kPackRows];
+#pragma unroll
+        for (int k = 0; k < kPackRows; ++k) {
+          a0[k] = soff[rr[k]];
+          a1[k] = soff[rr[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kPackRows; ++k) {
+          w[k].put(ps.off[c], (uint64_t)a0[k]);
+          w[k].put(ps.loff[c], (uint64_t)(uint32_t)(a1[k] - a0[k]));
+        }
       }
-      if (valid_at(g.sval[c], r)) vb |= 1u << c;
+#pragma unroll
+      for (int k = 0; k < kPackRows; ++k) vb[k] |= v[k] << c;
     }
-    if (ps.voff >= 0) w.put(ps.voff, vb);
-    w.store(rec, ps.stride, r);
+#pragma unroll
+    for (int k = 0; k < kPackRows; ++k) {
+      if (r0 + k * stride >= n) break;
+      if (ps.voff >= 0) w[k].put(ps.voff, vb[k]);
+      w[k].store(rec, ps.stride, r0 + k * stride);
+    }
   }
 }
 
@@ -565,7 +599,7 @@ __global__ __launch_bounds__(256) void k_gather_packed(GatherSet g, PackSet ps, 
 constexpr int kGatherGroups = 4;
 
 __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t* __restrict__ map, int64_t n,
-                                                      unsigned long long* __restrict__ null_count)
+                                                      int64_t n_src, unsigned long long* __restrict__ null_count)
 {
   __shared__ unsigned long long s_nulls[kMaxGather];
   if (threadIdx.x < kMaxGather) s_nulls[threadIdx.x] = 0;
@@ -576,9 +610,10 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
        base += kGatherGroups * stride) {
     int32_t m[kGatherGroups];
 #pragma unroll
-    for (int j = 0; j < kGatherGroups; ++j) {
-      const int64_t i = base + j * stride + lane;
-      m[j]            = i < n ? map[i] : -1;
+    for (int j = 0; j < kGatherGroups; ++j) {  // unconditional loads (rows past n read row n - 1)
+      const int64_t i  = base + j * stride + lane;
+      const int32_t mv = map[i < n ? i : n - 1];
+      m[j]             = i < n ? mv : -1;
     }
 #pragma unroll 1
     for (int c = 0; c < g.ncols; ++c) {
@@ -590,12 +625,34 @@ __global__ __launch_bounds__(256) void k_gather_multi(GatherSet g, const int32_t
       uint64_t* dval        = g.dval[c];
       uint64_t val[kGatherGroups];
       bool ok[kGatherGroups];
+      // every group's loads before any use: branch-free (a row without a source row loads source
+      // row 0 and is masked); no loads at all from an empty source (every map entry is then -1)
 #pragma unroll
       for (int j = 0; j < kGatherGroups; ++j) {
-        ok[j]  = m[j] >= 0 && valid_at(sval, m[j]);
+        ok[j]  = false;
         val[j] = 0;
-        if (width == 0) val[j] = ok[j] ? (uint64_t)(soff[m[j] + 1] - soff[m[j]]) : 0;
-        else if (m[j] >= 0) val[j] = load_bits(src, width, m[j]);
+      }
+      if (n_src > 0) {
+        const void* any = width == 0 ? static_cast<const void*>(soff) : src;
+        if (width == 0) {
+#pragma unroll
+          for (int j = 0; j < kGatherGroups; ++j) {
+            const int32_t mm  = m[j] >= 0 ? m[j] : 0;
+            const uint32_t vb = valid_bit_nb(sval, any, mm);
+            const int64_t a0 = soff[mm], a1 = soff[mm + 1];
+            ok[j]             = m[j] >= 0 && vb != 0;
+            val[j]            = ok[j] ? (uint64_t)(a1 - a0) : 0;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < kGatherGroups; ++j) {
+            const int32_t mm  = m[j] >= 0 ? m[j] : 0;
+            const uint32_t vb = valid_bit_nb(sval, any, mm);
+            const uint64_t x  = load_bits_nb(src, width, mm);
+            ok[j]             = m[j] >= 0 && vb != 0;
+            val[j]            = m[j] >= 0 ? x : 0;
+          }
+        }
       }
 #pragma unroll
       for (int j = 0; j < kGatherGroups; ++j) {
@@ -922,7 +979,7 @@ void gather_columns(const ldf_column_view* const* src, int ncols, const int32_t*
         ProfScope prof("gather", s);
         const dim3 grid(grid_for(n_out, 256, 256 * 16));
         if (!packed)
-          hipLaunchKernelGGL(k_gather_multi, grid, dim3(256), 0, s, g, map, n_out, cnt_p);
+          hipLaunchKernelGGL(k_gather_multi, grid, dim3(256), 0, s, g, map, n_out, n_src, cnt_p);
         else if (ps.stride == 8)
           hipLaunchKernelGGL(k_gather_packed<1>, grid, dim3(256), 0, s, g, ps, (const unsigned char*)recs.get(), map,
                              n_out, cnt_p);

@@ -367,6 +367,10 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
   __shared__ uint64_t smem[17];
   int64_t base = (int64_t)blockIdx.x * kJRows;
   uint64_t sum = 0;
+  if (np <= 0) {  // no probe rows (the grid still has one block): nothing to load
+    if (threadIdx.x == 0) block_tot[blockIdx.x] = 0;
+    return;
+  }
   if (!GENERIC) {
     // All kJItems keys are read, then their filter words (tiled tables), then the home slots of
     // the rows the filter passes (one 16-byte load: key, count and start together), each step's
@@ -376,41 +380,44 @@ __global__ __launch_bounds__(kJBlock) void k_join_count(DKeys pk, DKeys bk, int6
     int64_t slot[kJItems];
     unsigned long long key[kJItems];
     uint32_t fw[kJItems], fb[kJItems];
+    // (the loads are unconditional -- rows past the end read row np - 1, rows without a filter
+    // word or home slot read word / slot 0 -- and masked afterwards: a load under a branch is
+    // waited for before the next one is issued)
+    uint32_t vb[kJItems];
 #pragma unroll
     for (int j = 0; j < kJItems; ++j) {
-      const int64_t r = base + j * kJBlock + threadIdx.x;
-      slot[j]         = -1;
-      key[j]          = 0;
-      fw[j]           = 0;
-      fb[j]           = 0;
-      if (r < np) {
-        if (!valid_at(c.valid, r)) {
-          slot[j] = nulls_equal ? (int64_t)cap + 1 : -1;
-        } else {
-          key[j]            = load_bits(c.data, c.width, r);
-          const uint64_t hk = mix64(key[j]);
-          slot[j]           = key[j] == kEmpty ? (int64_t)cap : (int64_t)(hk & mask);
-          if (filt != nullptr && key[j] != kEmpty) {
-            fw[j] = filt_word(hk, tbits, mask);
-            fb[j] = filt_bits(hk);
-          }
-        }
-      }
+      const int64_t r  = base + j * kJBlock + threadIdx.x;
+      const int64_t rr = r < np ? r : np - 1;
+      vb[j]            = valid_bit_nb(c.valid, c.data, rr);
+      key[j]           = load_bits_nb(c.data, c.width, rr);
+    }
+#pragma unroll
+    for (int j = 0; j < kJItems; ++j) {
+      const int64_t r   = base + j * kJBlock + threadIdx.x;
+      const bool in     = r < np;
+      const bool valid  = vb[j] != 0;
+      key[j]            = valid ? key[j] : 0;
+      const uint64_t hk = mix64(key[j]);
+      const int64_t sv  = key[j] == kEmpty ? (int64_t)cap : (int64_t)(hk & mask);
+      slot[j]           = !in ? -1 : (valid ? sv : (nulls_equal ? (int64_t)cap + 1 : -1));
+      const bool useft  = filt != nullptr && in && valid && key[j] != kEmpty;
+      fw[j]             = useft ? filt_word(hk, tbits, mask) : 0;
+      fb[j]             = useft ? filt_bits(hk) : 0;
     }
     if (filt != nullptr) {
       uint32_t word[kJItems];
 #pragma unroll
-      for (int j = 0; j < kJItems; ++j) word[j] = fb[j] != 0 ? filt[fw[j]] : 0u;
+      for (int j = 0; j < kJItems; ++j) word[j] = filt[fw[j]];
 #pragma unroll
       for (int j = 0; j < kJItems; ++j)
         if (fb[j] != 0 && (word[j] & fb[j]) != fb[j]) slot[j] = -1;  // not in the table
     }
     uint4 home[kJItems];
 #pragma unroll
-    for (int j = 0; j < kJItems; ++j) {
-      home[j] = make_uint4(0, 0, 0, 0);
-      if (slot[j] >= 0) home[j] = *reinterpret_cast<const uint4*>(&tab[slot[j]]);
-    }
+    for (int j = 0; j < kJItems; ++j) home[j] = *reinterpret_cast<const uint4*>(&tab[slot[j] >= 0 ? slot[j] : 0]);
+#pragma unroll
+    for (int j = 0; j < kJItems; ++j)
+      if (slot[j] < 0) home[j] = make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (int j = 0; j < kJItems; ++j) {
       const int64_t r = base + j * kJBlock + threadIdx.x;
