@@ -1379,6 +1379,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   const uint32_t mask = (uint32_t)q.tcap - 1;
   // the LDS exact-sum words take fewer than 2^31 addends: a larger partition uses the global slots
   const bool big  = end - begin >= (int64_t(1) << 31);
+  // this block's dense slots (after the hash table and its two special slots): every key outside
+  // the hot set lives in exactly one (partition, sub-pass) block, which stores its group there
+  const int64_t dbase = (int64_t)t.cap + 2 + ((int64_t)part * q.S + sub) * q.tcap;
   constexpr int R = W >= 8 ? 2 : 4;  // rows per thread in flight (R * W record words in registers)
   for (int64_t i0 = begin + threadIdx.x; i0 < end; i0 += (int64_t)R * blockDim.x) {
     uint64_t w[R][W];
@@ -1433,11 +1436,20 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
       }
     }
     acc_part_rows<R>(p, q, lds, slot, vr, fl);
-    if (pb.cpairs != nullptr) {  // the rows' COUNT_DISTINCT pairs, appended wave by wave
+    int64_t gs[R];  // the row's accumulator slot: its key's dense slot, or its global slot if spilled
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      gs[j] = slot[j] >= 0 ? dbase + slot[j] : -1;
+      if (slot[j] != -1) continue;
+      const int64_t s = gb_insert_bits(kr[j], (fl[j] & kFlagKeyValid) != 0, t);
+      if (s >= 0) acc_spill(p, s, vr[j], fl[j]);
+      gs[j] = s;
+    }
+    if (pb.cpairs != nullptr) {  // the rows' COUNT_DISTINCT pairs (slot, value), appended wave by wave
       const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const bool has = slot[j] != -2 && ((fl[j] >> p.dcol) & 1);
+        const bool has = gs[j] >= 0 && ((fl[j] >> p.dcol) & 1);
         const uint64_t m = __ballot(has);
         if (m == 0) continue;
         const int leader = __ffsll((unsigned long long)m) - 1;
@@ -1447,15 +1459,9 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
         if (has) {
           const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
           pb.cpairs[begin + base + (uint32_t)__popcll(m & below)] =
-              ((unsigned long long)(uint32_t)kr[j] << 32) | (uint32_t)pick(vr[j], p.dcol);
+              ((unsigned long long)(uint32_t)gs[j] << 32) | (uint32_t)pick(vr[j], p.dcol);
         }
       }
-    }
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      if (slot[j] != -1) continue;
-      const int64_t s = gb_insert_bits(kr[j], (fl[j] & kFlagKeyValid) != 0, t);
-      if (s >= 0) acc_spill(p, s, vr[j], fl[j]);
     }
   }
   __syncthreads();
@@ -1463,7 +1469,6 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_part3(GbPlan p, PartPlan q, P
   // Store this block's groups in its own dense slots, after the hash table and its two special
   // slots: every key outside the hot set lives in exactly one (partition, sub-pass) block, so the
   // slot needs no hashing and no atomics.
-  const int64_t dbase = (int64_t)t.cap + 2 + ((int64_t)part * q.S + sub) * q.tcap;
   for (int h = threadIdx.x; h < q.tcap; h += blockDim.x) {
     const unsigned long long k = keys[h];
     if (k == kGEmpty) continue;
@@ -1896,6 +1901,101 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
     else bad |= 8;
   }
   __syncthreads();
+  if (threadIdx.x == 0 && bad) atomicOr(err, bad);
+}
+
+// The same for partitions whose pairs k_gb_part3 appended (one sub-pass): each pair carries its
+// key's accumulator slot instead of the key -- the key's dense slot, or its global slot when it
+// spilled -- so a new pair counts in an LDS array over the partition's dense slots (dynamic LDS,
+// tcap words; a spilled key's count goes straight to its slot) and no key table is needed.
+__global__ __launch_bounds__(1024) void k_gb_cold_distinct_slots(PartBufs pb, KeyTab t, int tcap,
+                                                                 unsigned long long* __restrict__ fine,
+                                                                 unsigned long long* __restrict__ dist,
+                                                                 int32_t* __restrict__ err)
+{
+  extern __shared__ uint32_t dcnt[];  // new pairs per dense slot of this partition
+  __shared__ unsigned long long set[kPairSet];
+  __shared__ uint32_t gh[kColdGroups];
+  __shared__ unsigned long long gcur[kColdGroups];
+  __shared__ int64_t gstart[kColdGroups + 1];
+  __shared__ int bad;
+  const int part   = blockIdx.x;
+  const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
+  const int64_t r1 = r0 + pb.cpair_count[part];
+  const int64_t d0 = (int64_t)t.cap + 2 + (int64_t)part * tcap;
+  const int ng     = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+  if (ng > kColdGroups) {  // more rows than this kernel plans for: the caller falls back
+    if (threadIdx.x == 0) atomicOr(err, 16);
+    return;
+  }
+  for (int i = threadIdx.x; i < tcap; i += blockDim.x) dcnt[i] = 0;
+  if (threadIdx.x < kColdGroups) gh[threadIdx.x] = 0;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kPairBatch * blockDim.x) {
+    unsigned long long pr[kPairBatch];
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) {
+      const int64_t i = r + (int64_t)u * blockDim.x;
+      pr[u]           = pb.cpairs[i < r1 ? i : r0];
+    }
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u)
+      if (r + (int64_t)u * blockDim.x < r1) atomicAdd(&gh[cold_group(pair_hash(pr[u]), ng)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int g = 0; g < ng; ++g) {
+      gstart[g] = run;
+      gcur[g]   = (unsigned long long)(r0 + run);
+      run += gh[g];
+    }
+    gstart[ng] = run;
+  }
+  __syncthreads();
+  if (ng > 1) {  // one group: deduplicate the appended pairs where they are
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kPairBatch * blockDim.x) {
+      unsigned long long pr[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const int64_t i = r + (int64_t)u * blockDim.x;
+        pr[u]           = pb.cpairs[i < r1 ? i : r0];
+      }
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u)
+        if (r + (int64_t)u * blockDim.x < r1) fine[atomicAdd(&gcur[cold_group(pair_hash(pr[u]), ng)], 1ull)] = pr[u];
+    }
+    __syncthreads();
+  }
+  const unsigned long long* src = ng > 1 ? fine : pb.cpairs;
+  for (int g = 0; g < ng; ++g) {
+    for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
+    __syncthreads();
+    const int64_t g0 = r0 + gstart[g], g1 = r0 + gstart[g + 1];
+    constexpr int U  = 4;
+    for (int64_t r = g0 + threadIdx.x; r < g1; r += (int64_t)U * blockDim.x) {
+      unsigned long long prs[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = r + (int64_t)u * blockDim.x;
+        prs[u]          = src[i < g1 ? i : g0];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (r + (int64_t)u * blockDim.x >= g1) break;
+        const int k = pair_set_insert(set, prs[u], pair_hash(prs[u]));
+        if (k < 0) bad |= 2;
+        if (k <= 0) continue;
+        const int64_t sl = (int64_t)(prs[u] >> 32);
+        if (sl >= d0 && sl < d0 + tcap) atomicAdd(&dcnt[sl - d0], 1u);
+        else atomicAdd(&dist[sl], 1ull);  // a key that spilled to the hash table
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < tcap; i += blockDim.x)  // this block is the only writer of its dense slots
+    if (dcnt[i] != 0) dist[d0 + i] = dcnt[i];
   if (threadIdx.x == 0 && bad) atomicOr(err, bad);
 }
 
@@ -2952,7 +3052,11 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       if (dpart) {
         ProfScope prof("gb_distinct", s);
         const AccDev& d = plan.a[dacc];
-        if (ncold > 0) {
+        if (ncold > 0 && pb.cpairs != nullptr) {
+          hipLaunchKernelGGL(k_gb_cold_distinct_slots, dim3(nparts), dim3(1024), (size_t)q.tcap * 4, s, pb, t, q.tcap,
+                             hfine.get(), d.g0, d.err);
+          LDF_HIP_TRY(hipGetLastError());
+        } else if (ncold > 0) {
           hipLaunchKernelGGL(k_gb_cold_distinct, dim3(nparts), dim3(1024), 0, s, pb, plan.dcol,
                              plan.col[plan.dcol].width, t, q.S, q.tcap, hfine.get(), d.g0, d.err);
           LDF_HIP_TRY(hipGetLastError());
