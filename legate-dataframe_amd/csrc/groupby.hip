@@ -839,20 +839,35 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
         cpj[j]            = cold_part(hk, p.part_bits);
         e[j]              = tab32[slot[j]];
       }
+      // every row's probe sequence advanced together: each round reads the next slot of all rows
+      // still looking (their LDS reads in flight at once) until a row finds its key or an empty slot
+      bool look[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        uint32_t i = slot[j];
-        hh[j]      = -1;
-        while (e[j] != kGEmpty) {
-          if ((e[j] >> 32) == k[j]) {
-            hh[j] = (int)(uint32_t)e[j];
-            break;
-          }
-          i    = (i + 1) & (uint32_t)(t32 - 1);
-          e[j] = tab32[i];
-        }
-        if (!kv[j]) hh[j] = s_null_hot;
+        hh[j]   = -1;
+        look[j] = true;
       }
+      for (int round = 0; round < t32; ++round) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!look[j]) continue;
+          if (e[j] == kGEmpty) look[j] = false;
+          else if ((e[j] >> 32) == k[j]) {
+            hh[j]   = (int)(uint32_t)e[j];
+            look[j] = false;
+          } else {
+            slot[j] = (slot[j] + 1) & (uint32_t)(t32 - 1);
+          }
+          any |= look[j];
+        }
+        if (!any) break;
+#pragma unroll
+        for (int j = 0; j < R; ++j) e[j] = tab32[look[j] ? slot[j] : 0u];
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (!kv[j]) hh[j] = s_null_hot;
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
