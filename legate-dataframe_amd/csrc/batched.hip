@@ -506,6 +506,29 @@ __device__ __forceinline__ uint32_t part_of(uint64_t kb, bool kv, int pbits)
   return (uint32_t)(mix64(kb) >> (64 - pbits));
 }
 
+// Rows a thread of the groupby scatter kernels loads before using any.
+constexpr int kBgRows = 4;
+
+// The partition counts of rows [r0, r1) of one instance into `hist` (branch-free loads, all of a
+// thread's rows in flight together; r1 > r0).
+__device__ __forceinline__ void bg_tile_hist(const GInst& I, const GPlanB& p, int64_t r0, int64_t r1, uint32_t* hist)
+{
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kBgRows * kGT) {
+    uint64_t kb[kBgRows];
+    uint32_t kv[kBgRows];
+#pragma unroll
+    for (int u = 0; u < kBgRows; ++u) {
+      const int64_t rr = r + (int64_t)u * kGT;
+      const int64_t rc = rr < r1 ? rr : r1 - 1;
+      kb[u]            = load_bits_nb(I.key, p.kwidth, rc);
+      kv[u]            = valid_bit_nb(I.kvalid, I.key, rc);
+    }
+#pragma unroll
+    for (int u = 0; u < kBgRows; ++u)
+      if (r + (int64_t)u * kGT < r1) atomicAdd(&hist[part_of(kb[u], kv[u] != 0, I.pbits)], 1u);
+  }
+}
+
 // LDS hash partitions per tile: counts (first pass) or reserved positions (second pass)
 __global__ __launch_bounds__(kGT) void k_bg_hist(const GInst* __restrict__ inst, const Item* __restrict__ items,
                                                  int nitems, unsigned int* cursor, int tile, GPlanB p)
@@ -522,8 +545,7 @@ __global__ __launch_bounds__(kGT) void k_bg_hist(const GInst* __restrict__ inst,
     __syncthreads();
     const int64_t r0 = (int64_t)m.tile * tile;
     const int64_t r1 = r0 + tile < I.n ? r0 + tile : I.n;
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kGT)
-      atomicAdd(&hist[part_of(load_bits(I.key, p.kwidth, r), valid_at(I.kvalid, r), I.pbits)], 1u);
+    bg_tile_hist(I, p, r0, r1, hist);
     __syncthreads();
     for (int q = threadIdx.x; q < np; q += kGT)
       if (hist[q]) atomicAdd(&p.pcount[I.pbase + q], hist[q]);
@@ -546,27 +568,39 @@ __global__ __launch_bounds__(kGT) void k_bg_scatter(const GInst* __restrict__ in
     __syncthreads();
     const int64_t r0 = (int64_t)m.tile * tile;
     const int64_t r1 = r0 + tile < I.n ? r0 + tile : I.n;
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kGT)
-      atomicAdd(&hist[part_of(load_bits(I.key, p.kwidth, r), valid_at(I.kvalid, r), I.pbits)], 1u);
+    bg_tile_hist(I, p, r0, r1, hist);
     __syncthreads();
     for (int q = threadIdx.x; q < np; q += kGT)
       if (hist[q]) cur[q] = atomicAdd(&p.pcursor[I.pbase + q], (unsigned long long)hist[q]);
     __syncthreads();
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kGT) {
-      const uint64_t kb = load_bits(I.key, p.kwidth, r);
-      const bool kv     = valid_at(I.kvalid, r);
-      const uint32_t q  = part_of(kb, kv, I.pbits);
-      const uint64_t d  = atomicAdd(&cur[q], 1ull);
-      uint32_t fl       = kv ? 1u : 0u;
-      p.pkey[d]         = kb;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)kBgRows * kGT) {
+      uint64_t kb[kBgRows], x[kBgRows][kMaxVal];
+      uint32_t fl[kBgRows];
 #pragma unroll
-      for (int c = 0; c < kMaxVal; ++c) {
-        if (c < p.ncol) {
-          p.pval[c][d] = load_bits(I.val[c], p.cwidth[c], r);
-          if (valid_at(I.vvalid[c], r)) fl |= 2u << c;
+      for (int u = 0; u < kBgRows; ++u) {  // every load of the thread's rows in flight together
+        const int64_t rr = r + (int64_t)u * kGT;
+        const int64_t rc = rr < r1 ? rr : r1 - 1;
+        kb[u]            = load_bits_nb(I.key, p.kwidth, rc);
+        fl[u]            = valid_bit_nb(I.kvalid, I.key, rc);
+#pragma unroll
+        for (int c = 0; c < kMaxVal; ++c) {  // columns past p.ncol read the key column again
+          const bool use = c < p.ncol;
+          const void* d  = use ? I.val[c] : I.key;
+          x[u][c]        = load_bits_nb(d, use ? p.cwidth[c] : p.kwidth, rc);
+          fl[u] |= (use ? valid_bit_nb(I.vvalid[c], d, rc) : 0u) << (c + 1);
         }
       }
-      p.pflag[d] = (uint8_t)fl;
+#pragma unroll
+      for (int u = 0; u < kBgRows; ++u) {
+        if (r + (int64_t)u * kGT >= r1) break;
+        const uint32_t q = part_of(kb[u], (fl[u] & 1u) != 0, I.pbits);
+        const uint64_t d = atomicAdd(&cur[q], 1ull);
+        p.pkey[d]        = kb[u];
+#pragma unroll
+        for (int c = 0; c < kMaxVal; ++c)
+          if (c < p.ncol) p.pval[c][d] = x[u][c];
+        p.pflag[d] = (uint8_t)fl[u];
+      }
     }
   }
 }
