@@ -783,10 +783,14 @@ static void run_join(const ldf_table_view* lhs, const ldf_table_view* rhs, const
     gather_columns(rsrc.data(), n_rhs_out, rmap.get(), n_out, jt == LDF_JOIN_LEFT || jt == LDF_JOIN_FULL, a,
                    out->cols + n_lhs_out, /*random_map=*/true, ncnt.get() + n_lhs_out);
     allocated = nout_cols;
-    if (nout_cols > 0) {
+    bool any_nullable = false;  // no output bitmap: every null count is 0, no readback
+    for (int i = 0; i < nout_cols; ++i) any_nullable |= out->cols[i].validity != nullptr;
+    if (any_nullable) {
       std::vector<unsigned long long> hc(nout_cols);
       copy_to_host(hc.data(), ncnt.get(), hc.size() * sizeof(unsigned long long), s);
       for (int i = 0; i < nout_cols; ++i) out->cols[i].null_count = out->cols[i].validity ? (int64_t)hc[i] : 0;
+    } else {
+      for (int i = 0; i < nout_cols; ++i) out->cols[i].null_count = 0;
     }
   } catch (...) {
     for (int i = 0; i < allocated; ++i) free_column(&out->cols[i], a);
