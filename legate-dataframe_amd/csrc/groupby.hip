@@ -40,6 +40,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 
 #include "ldf_agg.hpp"
 #include "ldf_common.hpp"
@@ -2428,6 +2429,28 @@ static Lowered lower(const ldf_table_view* tv, const ldf_agg* aggs, int32_t nagg
   return L;
 }
 
+// A second stream per (host thread, device) with two events, for work that runs beside the
+// caller's stream inside one call (default priority, non-blocking; created on first use).
+struct SideStream {
+  hipStream_t s    = nullptr;
+  hipEvent_t fork  = nullptr;
+  hipEvent_t join  = nullptr;
+};
+
+static SideStream& side_stream()
+{
+  thread_local std::map<int, SideStream> per_device;
+  int dev = 0;
+  LDF_HIP_TRY(hipGetDevice(&dev));
+  SideStream& ss = per_device[dev];
+  if (ss.s == nullptr) {
+    LDF_HIP_TRY(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+    LDF_HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+    LDF_HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+  }
+  return ss;
+}
+
 // k_gb_main3 for the plan's number of value columns (a template argument: the row loads are
 // unrolled over exactly the columns read); the LDS limit is raised once per instantiation
 template <bool DPART, int NC>
@@ -2858,7 +2881,8 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     Buf<uint16_t> codes;
     Buf<uint64_t> hoffs, hpoffs;  // hot pairs: per histogram block; per (pair partition, main block)
     Buf<uint32_t> hpcnt;          // hot pairs per (pair partition, main block)
-    Buf<unsigned long long> hpairs, hfine, cpairs;
+    Buf<unsigned long long> hpairs, hfine, cpairs, cfine;
+    Buf<uint32_t> part_cnt;  // hot pairs' new-pair counts per (pair partition, hot key)
     Buf<uint32_t> cpair_count;
     plan.dpart = 0;
     plan.dcol  = 0;
@@ -2926,7 +2950,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         const char* e = std::getenv("LDF_GROUPBY_TRACE");
         return e != nullptr && e[0] == '1';
       }();
-      const bool read_totals = trace || (ndistinct > 0 && !dpart) || (uint64_t)n * (uint64_t)(rec_stride + 32) > kNoSyncBytes;
+      const bool read_totals = trace || (ndistinct > 0 && !dpart) || (uint64_t)n * (uint64_t)(rec_stride + 40) > kNoSyncBytes;
       uint64_t totals[2] = {(uint64_t)n, dpart ? (uint64_t)n : 0};
       if (read_totals) {  // both totals queued into the pinned stage, one synchronisation
         uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
@@ -3025,14 +3049,32 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       }
       hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
       LDF_HIP_TRY(hipGetLastError());
-      // the hot pairs by pair partition (before k_gb_cold_distinct reuses hfine)
+      // The hot pairs' COUNT_DISTINCT chain (split by pair partition, deduplicated, summed per hot
+      // key) reads only the hot pairs and writes only the hot keys' counts: it runs on a second
+      // stream beside the partition pass and the cold pairs' deduplication, which each leave much
+      // of the GPU idle (one or two blocks per CU, waits on memory).  Every buffer either chain
+      // uses lives until the end of this attempt, after the caller's stream has waited for both.
+      SideStream& side = side_stream();
+      bool forked      = false;
       if (dpart && nhp > 0) {
-        ProfScope prof("gb_pairs", s);
         hpoffs = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
         exclusive_scan_u32_to_u64(hpcnt.get(), hpoffs.get(), (int64_t)kHPairParts * nblk, a);
-        hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, s, (const unsigned long long*)hfine.get(),
+        part_cnt = Buf<uint32_t>(a, (size_t)kHPairParts * hot_cap);
+        LDF_HIP_TRY(hipEventRecord(side.fork, s));
+        LDF_HIP_TRY(hipStreamWaitEvent(side.s, side.fork, 0));
+        forked = true;
+        ProfScope prof("gb_hot_distinct", side.s);
+        const AccDev& d = plan.a[dacc];
+        hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const unsigned long long*)hfine.get(),
                            (const uint64_t*)hoffs.get(), kHistSplit, (const uint64_t*)hpoffs.get(), hpairs.get());
         LDF_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, side.s, (const unsigned long long*)hpairs.get(),
+                           (const uint64_t*)hpoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
+        LDF_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, side.s, (const uint32_t*)part_cnt.get(),
+                           kHPairParts, hot_cap, hot, d.g0);
+        LDF_HIP_TRY(hipGetLastError());
+        LDF_HIP_TRY(hipEventRecord(side.join, side.s));
       }
       // 6. one block per (partition, sub-pass)
       int32_t loff = 0;
@@ -3068,30 +3110,23 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         }
         LDF_HIP_TRY(hipGetLastError());
       }
-      // 7. COUNT_DISTINCT: pairs of the other keys by (partition, sub-pass), pairs of the hot keys
-      //    by pair partition; both after the partition pass has given every key its slot
-      if (dpart) {
+      // 7. COUNT_DISTINCT of the keys outside the hot set: their pairs by (partition, sub-pass), after
+      //    the partition pass has given every key its slot (own scratch: the hot chain uses hfine)
+      if (dpart && ncold > 0) {
         ProfScope prof("gb_distinct", s);
         const AccDev& d = plan.a[dacc];
-        if (ncold > 0 && pb.cpairs != nullptr) {
+        cfine           = Buf<unsigned long long>(a, (size_t)ncold);
+        if (pb.cpairs != nullptr) {
           hipLaunchKernelGGL(k_gb_cold_distinct_slots, dim3(nparts), dim3(1024), (size_t)q.tcap * 4, s, pb, t, q.tcap,
-                             hfine.get(), d.g0, d.err);
+                             cfine.get(), d.g0, d.err);
           LDF_HIP_TRY(hipGetLastError());
-        } else if (ncold > 0) {
+        } else {
           hipLaunchKernelGGL(k_gb_cold_distinct, dim3(nparts), dim3(1024), 0, s, pb, plan.dcol,
-                             plan.col[plan.dcol].width, t, q.S, q.tcap, hfine.get(), d.g0, d.err);
-          LDF_HIP_TRY(hipGetLastError());
-        }
-        if (nhp > 0) {
-          Buf<uint32_t> part_cnt(a, (size_t)kHPairParts * hot_cap);
-          hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, s, (const unsigned long long*)hpairs.get(),
-                             (const uint64_t*)hpoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
-          LDF_HIP_TRY(hipGetLastError());
-          hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, s, (const uint32_t*)part_cnt.get(),
-                             kHPairParts, hot_cap, hot, d.g0);
+                             plan.col[plan.dcol].width, t, q.S, q.tcap, cfine.get(), d.g0, d.err);
           LDF_HIP_TRY(hipGetLastError());
         }
       }
+      if (forked) LDF_HIP_TRY(hipStreamWaitEvent(s, side.join, 0));
     } else if (n > 0) {
       // one wave of resident blocks (256 CUs; two 1024-thread blocks per CU when LDS allows)
       int per_cu = plan.lds_bytes > 0 ? std::max(1, std::min(2, (160 * 1024) / plan.lds_bytes)) : 2;
