@@ -610,7 +610,6 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main(DKeys keys, int64_t n, K
 // LDS table.  Each key therefore lives in exactly one place -- a hot LDS slot, a partition
 // block's LDS slot, or (only when a partition block's table is full) the global table -- and
 // the final values are stored, not added, into the global slot arrays.
-constexpr uint64_t kNoSyncBytes = 16ull << 30;  // groupby buffers sized for every row, at most
 constexpr int kMain3Lds    = 160 * 1024 - 256;  // k_gb_main3 dynamic LDS limit (a CU holds 160 KB; a little for its static LDS)
 constexpr int kMinPartBits = 10;
 constexpr int kMaxPartBits = 12;
@@ -2942,17 +2941,8 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);
       if (dpart) exclusive_scan_u32_to_u64(hcounts.get(), hoffs.get(), (int64_t)nh, a);
-      // The totals of rows outside the hot set and of hot pairs size the buffers below.  Sized for
-      // every row instead (each total is at most n), they need no read-back -- the GPU does not wait
-      // for the host here -- unless the per-set pair tables need them or that would take over
-      // kNoSyncBytes of buffer space.
-      static const bool trace = [] {
-        const char* e = std::getenv("LDF_GROUPBY_TRACE");
-        return e != nullptr && e[0] == '1';
-      }();
-      const bool read_totals = trace || (ndistinct > 0 && !dpart) || (uint64_t)n * (uint64_t)(rec_stride + 40) > kNoSyncBytes;
-      uint64_t totals[2] = {(uint64_t)n, dpart ? (uint64_t)n : 0};
-      if (read_totals) {  // both totals queued into the pinned stage, one synchronisation
+      uint64_t totals[2] = {0, 0};
+      {  // both totals queued into the pinned stage, one synchronisation
         uint64_t* st = static_cast<uint64_t*>(pinned_stage(16));
         st[1]        = 0;
         LDF_HIP_TRY(hipMemcpyAsync(st, poffs.get() + (size_t)nparts * nh, 8, hipMemcpyDeviceToHost, s));
@@ -2961,8 +2951,12 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         totals[0] = st[0];
         totals[1] = st[1];
       }
-      const int64_t ncold = (int64_t)totals[0];  // rows outside the hot set (or an upper bound)
-      const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart; or an upper bound)
+      static const bool trace = [] {
+        const char* e = std::getenv("LDF_GROUPBY_TRACE");
+        return e != nullptr && e[0] == '1';
+      }();
+      const int64_t ncold = (int64_t)totals[0];
+      const int64_t nhp   = (int64_t)totals[1];  // hot rows' distinct pairs (dpart)
       {  // LDF_GROUPBY_TRACE=1: one line per call with the partitioned path's plan
         if (trace)
           std::fprintf(stderr,
