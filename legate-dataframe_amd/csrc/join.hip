@@ -487,16 +487,22 @@ __global__ __launch_bounds__(kJBlock) void k_join_emit(const uint64_t* __restric
   // each thread owns kJItems consecutive rows for the block scan
   uint64_t v[kJItems];
   uint64_t s = 0;
+  uint64_t xs[kJItems];
+#pragma unroll
+  for (int j = 0; j < kJItems; ++j) {  // unconditional loads (past the block's rows: its last row)
+    const int li = threadIdx.x * kJItems + j;
+    xs[j]        = info[base + (li < nrows ? li : nrows - 1)];
+  }
 #pragma unroll
   for (int j = 0; j < kJItems; ++j) {
     int li = threadIdx.x * kJItems + j;
     uint64_t o = 0;
     if (li < nrows) {
-      uint64_t x = info[base + li];
-      uint32_t c = (uint32_t)(x >> 32);
-      st[li]     = (uint32_t)x;
-      mc[li]     = c;
-      o          = out_rows_for(c, jt);
+      const uint64_t x = xs[j];
+      uint32_t c       = (uint32_t)(x >> 32);
+      st[li]           = (uint32_t)x;
+      mc[li]           = c;
+      o                = out_rows_for(c, jt);
     }
     v[j] = o;
     s += o;
