@@ -761,6 +761,34 @@ constexpr int kHistSplit   = 3;  // 3 blocks of 512 per CU (76 VGPRs): the grid 
 constexpr int kHistThreads = 512;
 constexpr uint32_t kCodeCold = 0x8000u;
 
+// Class table of a single key of at most 4 bytes whose sampled values lie in [kmin, kmin + range):
+// code[key - kmin] is the key's row class, a hot index or kCodeCold | its partition (range 0: no
+// table; code then still points at one readable entry).  Keys outside the range are classified by
+// hashing and probing the hot key table.
+struct DenseCodes {
+  const uint16_t* code;
+  uint32_t kmin;
+  uint32_t range;
+};
+constexpr uint32_t kDenseMaxRange = 1u << 22;  // 8 MB of codes at most
+
+__global__ void k_gb_dense_fill(uint16_t* code, uint32_t kmin, uint32_t range, int part_bits)
+{
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < range; i += gridDim.x * blockDim.x)
+    code[i] = (uint16_t)(kCodeCold | cold_part(mix64((uint64_t)(uint32_t)(kmin + i)), part_bits));
+}
+
+// after k_gb_dense_fill: each hot key (other than the null key) inside the range takes its index
+__global__ void k_gb_dense_hot(uint16_t* code, uint32_t kmin, uint32_t range, HotList hot, int cap)
+{
+  const int nh = (int)min(*hot.count, (unsigned int)cap);
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+    if (hot.special[h] != 0) continue;
+    const uint32_t off = (uint32_t)hot.keys[h] - kmin;
+    if (off < range) code[off] = (uint16_t)h;
+  }
+}
+
 // Hot key table of k_gb_hist3 for keys of at most 4 bytes: one 8-byte LDS word per slot holding
 // (key << 32 | hot index), so a probe is one LDS read; 4 * hot_cap slots, at most 4096.
 constexpr int kHot32MaxSlots = 4096;
@@ -768,7 +796,8 @@ __host__ __device__ inline int hot32_slots(int hot_cap) { return 4 * hot_cap < k
 
 template <bool PACK32>
 __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n, GbPlan p, HotList hot, uint32_t* counts,
-                                                         uint32_t* hcounts, uint16_t* __restrict__ codes)
+                                                         uint32_t* hcounts, uint16_t* __restrict__ codes,
+                                                         DenseCodes dense)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
@@ -801,6 +830,9 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
   } else {
     hot_setup(p, lds, hot, nhot, false, true, &s_null_hot, &s_sent_hot);
   }
+  // the class of a null key: its hot index, or the partition of key bits 0 (where the partition
+  // pass keeps null keys apart by their flag)
+  const uint32_t null_code = s_null_hot >= 0 ? (uint32_t)s_null_hot : (kCodeCold | cold_part(mix64(0ull), p.part_bits));
   const int64_t mb0   = (int64_t)(blockIdx.x / kHistSplit) * p.rows_per_block;  // main-pass block's rows
   const int64_t share = (p.rows_per_block + kHistSplit - 1) / kHistSplit;
   const int sub       = (int)(blockIdx.x % kHistSplit);
@@ -830,23 +862,45 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
     }
     int hh[R];
     uint32_t cpj[R];
-    if (PACK32) {  // every row's first probe is issued before any is resolved
+    bool tabd[R];  // the row's code came from the dense table (or it is a null key)
+    if (PACK32) {
+      // Keys inside the dense range take their code from the table (loads all in flight); a null
+      // key takes the block's null code; only other keys (outside the sampled range) are hashed
+      // and probed below, by the whole wave when any of its rows needs it.
+      uint32_t tc[R];
+      bool need = false;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint32_t off = (uint32_t)k[j] - (uint32_t)dense.kmin;
+        tabd[j]            = !kv[j] || off < dense.range;
+        tc[j]              = dense.code[off < dense.range ? off : 0u];
+        need |= !tabd[j];
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint32_t c = kv[j] ? tc[j] : null_code;
+        hh[j]            = (c & kCodeCold) ? -1 : (int)c;
+        cpj[j]           = c & (kCodeCold - 1);
+      }
+      if (__any(need)) {  // every row's first probe is issued before any is resolved
       unsigned long long e[R];
       uint32_t slot[R];
+      uint32_t cph[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const uint64_t hk = mix64((uint64_t)k[j]);
         slot[j]           = (uint32_t)(hk & (uint64_t)(t32 - 1));
-        cpj[j]            = cold_part(hk, p.part_bits);
+        cph[j]            = cold_part(hk, p.part_bits);
         e[j]              = tab32[slot[j]];
       }
       // every row's probe sequence advanced together: each round reads the next slot of all rows
       // still looking (their LDS reads in flight at once) until a row finds its key or an empty slot
       bool look[R];
+      int hp[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        hh[j]   = -1;
-        look[j] = true;
+        hp[j]   = -1;
+        look[j] = !tabd[j];
       }
       for (int round = 0; round < t32; ++round) {
         bool any = false;
@@ -855,7 +909,7 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
           if (!look[j]) continue;
           if (e[j] == kGEmpty) look[j] = false;
           else if ((e[j] >> 32) == k[j]) {
-            hh[j]   = (int)(uint32_t)e[j];
+            hp[j]   = (int)(uint32_t)e[j];
             look[j] = false;
           } else {
             slot[j] = (slot[j] + 1) & (uint32_t)(t32 - 1);
@@ -868,7 +922,11 @@ __global__ __launch_bounds__(kHistThreads) void k_gb_hist3(DKeys keys, int64_t n
       }
 #pragma unroll
       for (int j = 0; j < R; ++j)
-        if (!kv[j]) hh[j] = s_null_hot;
+        if (!tabd[j]) {
+          hh[j]  = hp[j];
+          cpj[j] = cph[j];
+        }
+      }
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -2117,12 +2175,18 @@ __global__ __launch_bounds__(1024) void k_gb_sample_lds(DKeys keys, int64_t n, i
   }
 }
 
-// stats[0] = distinct, [1] = singletons, [2] = doubletons, [3 + c] = keys seen c times (c < 64)
-__global__ __launch_bounds__(256) void k_gb_sample_stats(const unsigned int* cnt, int64_t n, unsigned long long* stats)
+// stats[0] = distinct, [1] = singletons, [2] = doubletons, [3 + c] = keys seen c times (c < 64),
+// [67] = ~(smallest key), [68] = largest key, over the table's ordinary slots (i < scap; stats
+// starts zeroed, so [67] > ~[68] means no ordinary key)
+__global__ __launch_bounds__(256) void k_gb_sample_stats(const unsigned int* cnt, const unsigned long long* slots,
+                                                         int64_t scap, int64_t n, unsigned long long* stats)
 {
   __shared__ unsigned int loc[3 + 64];
+  __shared__ unsigned long long mm[2];
   for (int i = threadIdx.x; i < 3 + 64; i += blockDim.x) loc[i] = 0;
+  if (threadIdx.x < 2) mm[threadIdx.x] = 0;
   __syncthreads();
+  unsigned long long nmin = 0, kmax = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     unsigned c = cnt[i];
     if (c == 0) continue;
@@ -2130,10 +2194,18 @@ __global__ __launch_bounds__(256) void k_gb_sample_stats(const unsigned int* cnt
     if (c == 1) atomicAdd(&loc[1], 1u);
     if (c == 2) atomicAdd(&loc[2], 1u);
     atomicAdd(&loc[3 + (c < 63 ? c : 63)], 1u);
+    if (i < scap) {
+      const unsigned long long k = slots[i];
+      nmin                       = max(nmin, ~k);
+      kmax                       = max(kmax, k);
+    }
   }
+  if (nmin) atomicMax(&mm[0], nmin);
+  if (kmax) atomicMax(&mm[1], kmax);
   __syncthreads();
   for (int i = threadIdx.x; i < 3 + 64; i += blockDim.x)
     if (loc[i]) atomicAdd(&stats[i], (unsigned long long)loc[i]);
+  if (threadIdx.x < 2 && mm[threadIdx.x]) atomicMax(&stats[3 + 64 + threadIdx.x], mm[threadIdx.x]);
 }
 
 __global__ void k_gb_hot_select(const unsigned long long* sslots, const unsigned int* cnt, int64_t scap, unsigned int thr,
@@ -2569,10 +2641,10 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
   uint64_t scap    = next_pow2((uint64_t)std::max<int64_t>(2 * S, 64));
   Buf<unsigned long long> sslots(a, scap + 2);
   Buf<unsigned int> scnt(a, scap + 2);
-  Buf<unsigned long long> smisc(a, 2 + 3 + 64);  // nkeys, overflow, stats
+  Buf<unsigned long long> smisc(a, 2 + 3 + 64 + 2);  // nkeys, overflow, stats
   fill_u64((uint64_t*)sslots.get(), kGEmpty, scap + 2, s);
   fill_u32(scnt.get(), 0, scap + 2, s);
-  fill_u64((uint64_t*)smisc.get(), 0, 2 + 3 + 64, s);
+  fill_u64((uint64_t*)smisc.get(), 0, 2 + 3 + 64 + 2, s);
   {
     KeyTab st{sslots.get(), scap - 1, scap, smisc.get(), reinterpret_cast<int32_t*>(smisc.get() + 1)};
     if (S > 0) {
@@ -2583,13 +2655,14 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         hipLaunchKernelGGL(k_gb_sample_lds, dim3((unsigned)((S + kSampleBlockRows - 1) / kSampleBlockRows)), dim3(1024), 0,
                            s, keys, n, S, stride, st, scnt.get());
       LDF_HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(k_gb_sample_stats, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, scnt.get(), (int64_t)scap + 2,
-                         smisc.get() + 2);
+      hipLaunchKernelGGL(k_gb_sample_stats, dim3(grid_for(scap + 2, 256)), dim3(256), 0, s, scnt.get(), sslots.get(),
+                         (int64_t)scap, (int64_t)scap + 2, smisc.get() + 2);
       LDF_HIP_TRY(hipGetLastError());
     }
   }
-  unsigned long long stats[3 + 64] = {};
+  unsigned long long stats[3 + 64 + 2] = {};
   copy_to_host(stats, smisc.get() + 2, sizeof(stats), s);
+  const uint64_t smin = ~stats[3 + 64], smax = stats[3 + 64 + 1];  // sampled key range (smin > smax: none)
   uint64_t est;
   {
     double d = (double)stats[0], f1 = (double)stats[1];
@@ -2878,6 +2951,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     Buf<unsigned long long> hparts;
     Buf<uint32_t> hcounts;
     Buf<uint16_t> codes;
+    Buf<uint16_t> dcodes;  // the key's dense class table (k_gb_hist3)
     Buf<uint64_t> hoffs, hpoffs;  // hot pairs: per histogram block; per (pair partition, main block)
     Buf<uint32_t> hpcnt;          // hot pairs per (pair partition, main block)
     Buf<unsigned long long> hpairs, hfine, cpairs, cfine;
@@ -2931,12 +3005,34 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         hplan.tab_off   = 0;
         hplan.tidx_off  = 2 * plan.hot_cap * 8;
         hplan.lds_bytes = pack ? hot32_slots(plan.hot_cap) * 8 : (hplan.tidx_off + 2 * plan.hot_cap * 2 + 15) & ~15;
+        // a key of at most 4 bytes whose sampled values span at most n / 4 values (and 2^22) is
+        // classified through a table of codes over that range (LDF_GROUPBY_DENSE_CODES=0 turns the
+        // table off, a tuning knob)
+        static const bool dense_on = [] {
+          const char* e = std::getenv("LDF_GROUPBY_DENSE_CODES");
+          return e == nullptr || e[0] != '0';
+        }();
+        DenseCodes dense{nullptr, 0, 0};
+        dcodes = Buf<uint16_t>(a, 2);
+        fill_u32(reinterpret_cast<uint32_t*>(dcodes.get()), 0, 1, s);
+        if (pack && dense_on && smin <= smax && smax - smin < (uint64_t)kDenseMaxRange &&
+            (int64_t)(smax - smin + 1) <= n / 4) {
+          dense.kmin  = (uint32_t)smin;
+          dense.range = (uint32_t)(smax - smin + 1);
+          dcodes      = Buf<uint16_t>(a, dense.range);
+          hipLaunchKernelGGL(k_gb_dense_fill, dim3(grid_for(dense.range, 256)), dim3(256), 0, s, dcodes.get(), dense.kmin,
+                             dense.range, plan.part_bits);
+          hipLaunchKernelGGL(k_gb_dense_hot, dim3(grid_for(std::max(plan.hot_cap, 1), 256)), dim3(256), 0, s, dcodes.get(), dense.kmin,
+                             dense.range, hot, plan.hot_cap);
+          LDF_HIP_TRY(hipGetLastError());
+        }
+        dense.code = dcodes.get();
         if (pack)
           hipLaunchKernelGGL(k_gb_hist3<true>, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n,
-                             hplan, hot, pcounts.get(), hcounts.get(), codes.get());
+                             hplan, hot, pcounts.get(), hcounts.get(), codes.get(), dense);
         else
           hipLaunchKernelGGL(k_gb_hist3<false>, dim3(nh), dim3(kHistThreads), hplan.lds_bytes + nparts * 4, s, keys, n,
-                             hplan, hot, pcounts.get(), hcounts.get(), codes.get());
+                             hplan, hot, pcounts.get(), hcounts.get(), codes.get(), dense);
         LDF_HIP_TRY(hipGetLastError());
       }
       exclusive_scan_u32_to_u64(pcounts.get(), poffs.get(), (int64_t)nparts * nh, a);

@@ -80,14 +80,18 @@ def test_groupby_c2_mix_vs_oracle(gpu, seed):
 
 
 @pytest.mark.parametrize("keys_kind", ["uniform_300k", "zipf_1M", "few_16", "nullkeys_int64", "int64_wide",
-                                       "uniform_1200k", "uniform_wide_4M", "int16_signed", "int8_signed"])
+                                       "uniform_1200k", "uniform_wide_4M", "int16_signed", "int8_signed",
+                                       "int32_outliers_nulls"])
 def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatch):
     """Inputs of >= 1M rows take the partitioned path (hot keys in LDS, the rest hash-partitioned
     and aggregated per partition in LDS); LDF_GROUPBY_PARTITIONED=0 forces the global-table path.
     Both must equal the oracle (integers exactly, FP64 sums within 1e-12).  The key counts span
     2^10, 2^11 and 2^12 partitions, and (uniform_wide_4M, ~2.9M keys) sub-passes per partition;
     1- and 2-byte signed keys (negative values zero-extend the same way in the hot key table
-    and in the rows) take the packed hot-key lookup of the row classification."""
+    and in the rows) take the packed hot-key lookup of the row classification.  Keys of at most
+    4 bytes whose sampled range spans at most n / 4 values are classified through a table over
+    that range; int32_outliers_nulls adds null keys and rare keys far outside the sampled range,
+    which the table does not cover and the hash probe classifies."""
     rng = np.random.default_rng(21)
     n = 4_500_000 if keys_kind == "uniform_wide_4M" else 1_500_000
     if keys_kind == "uniform_1200k":
@@ -104,6 +108,11 @@ def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatc
         k = pa.array(rng.integers(-20_000, 20_000, n).astype(np.int16))
     elif keys_kind == "int8_signed":
         k = pa.array(rng.integers(-128, 128, n).astype(np.int8))
+    elif keys_kind == "int32_outliers_nulls":
+        kk = rng.integers(0, 100_000, n).astype(np.int32)
+        far = rng.choice(n, 40, replace=False)
+        kk[far] = rng.choice(np.array([-7, -(2**31), 2**31 - 1, 5_000_000, 100_000, 100_001], np.int32), 40)
+        k = pa.array(kk, mask=rng.random(n) < 0.02)
     elif keys_kind == "nullkeys_int64":
         k = pa.array(rng.integers(0, 50_000, n), pa.int64(), mask=rng.random(n) < 0.02)
     else:
@@ -115,7 +124,7 @@ def test_groupby_partitioned_path_matches_global_path(gpu, keys_kind, monkeypatc
     aggs = [("f", "sum", "fs"), ("f", "mean", "fm"), ("i", "sum", "is"), ("i", "min", "imin"), ("i", "max", "imax"),
             ("i", "count", "ic"), ("f", "max", "fmax"), (None, "count_all", "n")]
     if keys_kind in ("uniform_300k", "zipf_1M", "few_16", "uniform_1200k", "uniform_wide_4M", "int16_signed",
-                     "int8_signed"):
+                     "int8_signed", "int32_outliers_nulls"):
         aggs.append(("u", "count_distinct", "nu"))
     got = gpu_groupby(t, ["k"], aggs)
     monkeypatch.setenv("LDF_GROUPBY_PARTITIONED", "0")
