@@ -667,19 +667,55 @@ __device__ __forceinline__ uint32_t cold_sub(uint64_t hk, int S)
 }
 
 // COUNT_DISTINCT through pair partitions: the (hot index, value) pairs of hot rows are spread by
-// a hash of the pair over kHPairParts coarse partitions (then kHPairFine fine ones), so each
-// fine partition's pairs fit an LDS hash set.  Bits of one 64-bit hash: top 10 = coarse
-// partition, next 4 = fine partition, low bits = LDS set slot.
+// a hash of the pair over kHPairParts coarse partitions (then fine ones), so each fine
+// partition's pairs fit an LDS hash set.
 constexpr int kHPairBits = 10;
 constexpr int kHPairParts = 1 << kHPairBits;
-constexpr int kHPairFine  = 16;
 constexpr int kPairSet    = 8192;  // LDS pair-set slots of the dedupe kernels (64 KB)
 __device__ __forceinline__ uint64_t pair_hash(unsigned long long pr) { return mix64(pr ^ 0x94d049bb133111ebull); }
-__device__ __forceinline__ uint32_t pair_part(unsigned long long pr) { return (uint32_t)(pair_hash(pr) >> (64 - kHPairBits)); }
-__device__ __forceinline__ uint32_t fine_part(unsigned long long pr)
+
+// A hot row's pair p = (hot index << 32 | value) has at most 42 bits (hot index < 2^10).  The
+// hot pairs go to their coarse partition by the top 10 bits of a bijection of the 42-bit pair
+// space (xor-shifts by half the width and multiplications by odd constants, each invertible), and
+// a partition stores only the low 32 bits of the image: the pair is recovered from the partition
+// index and those bits.  Inside a coarse partition the top kHFineBits of the stored word pick the
+// fine partition and its low bits the LDS set slot.
+constexpr int kPairHotMax = 1 << (42 - 32);  // hot keys of a groupby with a COUNT_DISTINCT
+static_assert(kHPairBits == 42 - 32, "coarse partition bits + stored bits = pair bits");
+constexpr uint64_t kP42 = (1ull << 42) - 1;
+constexpr uint64_t kPM1 = 0x9E3779B97F4A7C15ull & kP42;  // odd
+constexpr uint64_t kPM2 = 0xBF58476D1CE4E5B9ull & kP42;  // odd
+constexpr uint64_t inv_odd(uint64_t c)  // c^-1 mod 2^64 by Newton's iteration (c odd)
 {
-  return (uint32_t)(pair_hash(pr) >> (64 - kHPairBits - 4)) & (kHPairFine - 1);
+  uint64_t x = c;
+  for (int i = 0; i < 6; ++i) x *= 2 - c * x;
+  return x;
 }
+constexpr uint64_t kPM1i = inv_odd(kPM1) & kP42;
+constexpr uint64_t kPM2i = inv_odd(kPM2) & kP42;
+static_assert(((kPM1 * kPM1i) & kP42) == 1 && ((kPM2 * kPM2i) & kP42) == 1, "inverse multipliers");
+__host__ __device__ __forceinline__ uint64_t pair_mix(uint64_t p)
+{
+  p ^= p >> 21;
+  p = (p * kPM1) & kP42;
+  p ^= p >> 21;
+  p = (p * kPM2) & kP42;
+  return p ^ (p >> 21);
+}
+__host__ __device__ __forceinline__ uint64_t pair_unmix(uint64_t x)
+{
+  x ^= x >> 21;
+  x = (x * kPM2i) & kP42;
+  x ^= x >> 21;
+  x = (x * kPM1i) & kP42;
+  return x ^ (x >> 21);
+}
+__device__ __forceinline__ uint32_t pair_part(unsigned long long pr) { return (uint32_t)(pair_mix(pr) >> 32); }
+constexpr int kHFineBits   = 5;
+constexpr int kHFine       = 1 << kHFineBits;  // fine partitions of a coarse hot-pair partition
+constexpr int kPairSet32   = 8192;             // LDS set slots of the hot dedupe (32 KB of words)
+constexpr uint32_t kWEmpty = 0xffffffffu;      // a set word holds the stored word without its fine bits
+constexpr uint32_t kWMask  = (1u << (32 - kHFineBits)) - 1;
 
 // Builds the LDS table of hot keys when `table` and clears the hot accumulators when `accs`.
 __device__ inline void hot_setup(const GbPlan& p, unsigned char* lds, const HotList& hot, int nhot, bool accs,
@@ -1609,14 +1645,15 @@ __device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned
 // hoffs[(b + 1) * split])), split into the kHPairParts pair partitions, one block per main-pass
 // block (1024 threads = kHPairParts): k_gb_main3 counted its pairs per partition, and `off` is the
 // scan of those counts (partition-major).  Each tile of kPairTile pairs is sorted by partition in
-// LDS and every partition's pairs of the tile are stored as one contiguous run (order inside a
-// partition does not matter to the deduplication).
+// LDS and every partition's stored words (the low 32 bits of the pair's image under pair_mix)
+// of the tile are written as one contiguous run (order inside a partition does not matter to the
+// deduplication).
 constexpr int kPairBatch = 8;   // pairs a thread loads before using any (deduplication loops)
 constexpr int kPairTile = 16384;  // 16 pairs per partition and tile on average: runs of whole lines
 
 __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long* __restrict__ src,
                                                        const uint64_t* __restrict__ hoffs, int split,
-                                                       const uint64_t* __restrict__ off, unsigned long long* __restrict__ dst)
+                                                       const uint64_t* __restrict__ off, uint32_t* __restrict__ dst)
 {
   __shared__ uint32_t tcnt[kHPairParts];
   __shared__ uint32_t tstart[kHPairParts];
@@ -1641,7 +1678,8 @@ __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long*
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = threadIdx.x + j * kHPairParts;
-      part[j]     = pair_part(pr[j]);
+      pr[j]       = pair_mix(pr[j]);  // the image: partition in bits 32..41, stored word below
+      part[j]     = (uint32_t)(pr[j] >> 32);
       rank[j]     = i < tn ? atomicAdd(&tcnt[part[j]], 1u) : 0u;
     }
     __syncthreads();
@@ -1658,87 +1696,102 @@ __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long*
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += kHPairParts) {
       const unsigned long long v = stage[i];
-      const uint32_t q           = pair_part(v);  // recomputed: no LDS for the staged partitions
-      dst[cur[q] + (uint64_t)(i - (int)tstart[q])] = v;
+      const uint32_t q           = (uint32_t)(v >> 32);
+      dst[cur[q] + (uint64_t)(i - (int)tstart[q])] = (uint32_t)v;
     }
     __syncthreads();
     cur[threadIdx.x] += tcnt[threadIdx.x];
   }
 }
 
-// One block per coarse hot-pair partition: split its pairs into kHPairFine fine partitions
-// (within the coarse range of `fine`), then deduplicate each fine partition in an LDS hash set
-// (in several hash passes when it holds more pairs than the set takes at ~70% load), counting
-// new pairs per hot index in LDS; each hot key's count is added to its slot once per block.
-__global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long long* __restrict__ pairs,
+// Inserts the word w (< kWMask + 1) into an LDS set of kPairSet32 words from slot i; 1 = new,
+// 0 = present, -1 = no room.
+__device__ __forceinline__ int word_set_insert(uint32_t* set, uint32_t w, uint32_t i)
+{
+  i &= kPairSet32 - 1;
+  for (int t = 0; t < kPairSet32; ++t) {
+    const uint32_t old = atomicCAS(&set[i], kWEmpty, w);
+    if (old == kWEmpty) return 1;
+    if (old == w) return 0;
+    i = (i + 1) & (kPairSet32 - 1);
+  }
+  return -1;
+}
+
+// One block per coarse hot-pair partition q: split its stored words into kHFine fine partitions
+// (within the coarse range of `fine`), then deduplicate each fine partition in an LDS set of
+// words (in several passes when it holds more words than the set takes at ~70% load), counting
+// new pairs per hot index in LDS (the hot index is recovered from q and the word through
+// pair_unmix); each hot key's count is added to its slot once per block.
+__global__ __launch_bounds__(1024) void k_gb_hot_distinct(const uint32_t* __restrict__ pairs,
                                                           const uint64_t* __restrict__ hoffsets, int nblk,
-                                                          unsigned long long* __restrict__ fine, int hot_cap,
+                                                          uint32_t* __restrict__ fine, int hot_cap,
                                                           HotList hot, uint32_t* __restrict__ part_cnt,
                                                           int32_t* __restrict__ err)
 {
-  __shared__ unsigned long long set[kPairSet];
-  __shared__ uint32_t cnt[2048];
-  __shared__ uint32_t fh[kHPairFine];
-  __shared__ uint32_t tcnt[kHPairFine];
-  __shared__ uint32_t tstart[kHPairFine];
-  __shared__ unsigned long long fcur[kHPairFine];
-  __shared__ int64_t fstart[kHPairFine + 1];
+  __shared__ uint32_t set[kPairSet32];
+  __shared__ uint32_t cnt[kPairHotMax];
+  __shared__ uint32_t fh[kHFine];
+  __shared__ uint32_t tcnt[kHFine];
+  __shared__ uint32_t tstart[kHFine];
+  __shared__ unsigned long long fcur[kHFine];
+  __shared__ int64_t fstart[kHFine + 1];
   __shared__ int bad;
   const int q      = blockIdx.x;
   const int64_t b0 = (int64_t)hoffsets[(int64_t)q * nblk];
   const int64_t b1 = (int64_t)hoffsets[(int64_t)(q + 1) * nblk];
-  const int nhot   = (int)min(*hot.count, (unsigned int)hot_cap);
+  const int nhot   = (int)min(min(*hot.count, (unsigned int)hot_cap), (unsigned int)kPairHotMax);
   for (int i = threadIdx.x; i < nhot; i += blockDim.x) cnt[i] = 0;
-  if (threadIdx.x < kHPairFine) fh[threadIdx.x] = 0;
+  if (threadIdx.x < kHFine) fh[threadIdx.x] = 0;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
   for (int64_t r = b0 + threadIdx.x; r < b1; r += (int64_t)kPairBatch * blockDim.x) {
-    unsigned long long pr[kPairBatch];  // a thread's loads in flight together (past the end: pair b0)
+    uint32_t pw[kPairBatch];  // a thread's loads in flight together (past the end: word b0)
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
       const int64_t i = r + (int64_t)u * blockDim.x;
-      pr[u]           = pairs[i < b1 ? i : b0];
+      pw[u]           = pairs[i < b1 ? i : b0];
     }
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u)
-      if (r + (int64_t)u * blockDim.x < b1) atomicAdd(&fh[fine_part(pr[u])], 1u);
+      if (r + (int64_t)u * blockDim.x < b1) atomicAdd(&fh[pw[u] >> (32 - kHFineBits)], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t run = 0;
-    for (int f = 0; f < kHPairFine; ++f) {
+    for (int f = 0; f < kHFine; ++f) {
       fstart[f] = run;
       fcur[f]   = (unsigned long long)(b0 + run);
       run += fh[f];
     }
-    fstart[kHPairFine] = run;
+    fstart[kHFine] = run;
   }
   __syncthreads();
-  // Tiles of kFineTile pairs are sorted by fine partition in LDS (in the set's memory, which the
+  // Tiles of kFineTile words are sorted by fine partition in LDS (in the set's memory, which the
   // deduplication clears first) and each fine partition's run of a tile is stored contiguously.
-  constexpr int kFineTile = 4096;  // 256 pairs per fine partition and tile on average
-  constexpr int R         = kFineTile / 1024;  // pairs per thread and tile (2 blocks per CU)
+  constexpr int kFineTile = 8192;              // 256 words per fine partition and tile on average
+  constexpr int R         = kFineTile / 1024;  // words per thread and tile
+  static_assert(kFineTile <= kPairSet32, "the tile is staged in the set");
   for (int64_t t0 = b0; t0 < b1; t0 += kFineTile) {
     const int tn = (int)(b1 - t0 < kFineTile ? b1 - t0 : kFineTile);
-    if (threadIdx.x < kHPairFine) tcnt[threadIdx.x] = 0;
+    if (threadIdx.x < kHFine) tcnt[threadIdx.x] = 0;
     __syncthreads();
-    unsigned long long pr[R];
-    uint32_t fb[R], rank[R];
+    uint32_t pw[R], fb[R], rank[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {  // unconditional loads (past the tile: its first pair), masked
+    for (int j = 0; j < R; ++j) {  // unconditional loads (past the tile: its first word), masked
       const int i = threadIdx.x + j * 1024;
-      pr[j]       = pairs[t0 + (i < tn ? i : 0)];
+      pw[j]       = pairs[t0 + (i < tn ? i : 0)];
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = threadIdx.x + j * 1024;
-      fb[j]       = fine_part(pr[j]);
+      fb[j]       = pw[j] >> (32 - kHFineBits);
       rank[j]     = i < tn ? atomicAdd(&tcnt[fb[j]], 1u) : 0u;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       uint32_t run = 0;
-      for (int f = 0; f < kHPairFine; ++f) {
+      for (int f = 0; f < kHFine; ++f) {
         tstart[f] = run;
         run += tcnt[f];
       }
@@ -1746,39 +1799,42 @@ __global__ __launch_bounds__(1024) void k_gb_hot_distinct(const unsigned long lo
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < R; ++j)
-      if (threadIdx.x + j * 1024 < tn) set[tstart[fb[j]] + rank[j]] = pr[j];
+      if (threadIdx.x + j * 1024 < tn) set[tstart[fb[j]] + rank[j]] = pw[j];
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += 1024) {
-      const unsigned long long v = set[i];
-      const uint32_t f           = fine_part(v);  // recomputed: no LDS for the staged partitions
+      const uint32_t v = set[i];
+      const uint32_t f = v >> (32 - kHFineBits);
       fine[fcur[f] + (uint64_t)(i - (int)tstart[f])] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kHPairFine) fcur[threadIdx.x] += tcnt[threadIdx.x];
+    if (threadIdx.x < kHFine) fcur[threadIdx.x] += tcnt[threadIdx.x];
   }
   __syncthreads();
-  for (int f = 0; f < kHPairFine; ++f) {
+  for (int f = 0; f < kHFine; ++f) {
     const int64_t r0 = b0 + fstart[f], r1 = b0 + fstart[f + 1];
-    const int passes = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+    const int passes = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet32)) + 1;
     for (int pass = 0; pass < passes; ++pass) {
-      for (int i = threadIdx.x; i < kPairSet; i += blockDim.x) set[i] = kGEmpty;
+      for (int i = threadIdx.x; i < kPairSet32; i += blockDim.x) set[i] = kWEmpty;
       __syncthreads();
-      constexpr int U = 4;  // a fine partition holds ~4 pairs per thread: its loads in one batch
+      constexpr int U = 2;  // a fine partition holds ~2 words per thread: its loads in one batch
       for (int64_t r = r0 + threadIdx.x; r < r1; r += (int64_t)U * blockDim.x) {
-        unsigned long long pr[U];
+        uint32_t pw[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int64_t i = r + (int64_t)u * blockDim.x;
-          pr[u]           = fine[i < r1 ? i : r0];
+          pw[u]           = fine[i < r1 ? i : r0];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if (r + (int64_t)u * blockDim.x >= r1) break;
-          const uint64_t hsh = pair_hash(pr[u]);
-          if (passes > 1 && (int)((hsh >> 24) % (uint64_t)passes) != pass) continue;
-          const int k = pair_set_insert(set, pr[u], hsh);
-          if (k > 0) atomicAdd(&cnt[(uint32_t)(pr[u] >> 32)], 1u);
-          else if (k < 0) bad = 1;
+          const uint32_t w = pw[u] & kWMask;
+          if (passes > 1 && (int)((w >> 14) % (uint32_t)passes) != pass) continue;
+          const int k = word_set_insert(set, w, w);
+          if (k > 0) {
+            const uint32_t h = (uint32_t)(pair_unmix(((uint64_t)q << 32) | pw[u]) >> 32);
+            if (h < (uint32_t)nhot) atomicAdd(&cnt[h], 1u);
+            else bad = 1;
+          } else if (k < 0) bad = 1;
         }
       }
       __syncthreads();
@@ -2699,6 +2755,8 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     const int budget = 124 * 1024;  // leaves room for the partition staging of k_gb_main3
     int most         = 2048;        // LDF_GROUPBY_HOT_CAP lowers the ceiling (a tuning knob)
     if (const char* e = std::getenv("LDF_GROUPBY_HOT_CAP")) most = std::max(64, std::min(2048, std::atoi(e)));
+    for (auto& h : low.acc)  // a hot pair (hot index, value) of COUNT_DISTINCT has at most 42 bits
+      if (h.kind == ACC_DISTINCT) most = std::min(most, kPairHotMax);
     for (int h = 2048; h >= 64; h >>= 1)
       if (h <= most && h * (acc_bytes + 20) + 64 <= budget) {
         hot_cap = h;
@@ -2814,7 +2872,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     if (aggs[i].kind == LDF_AGG_VARIANCE || aggs[i].kind == LDF_AGG_STDDEV) use_v3 = false;
   // COUNT_DISTINCT through pair partitions deduplicated in LDS (one such accumulator on the
   // partitioned path); a dedupe table that ever overflows sends the call back to the pair-set path
-  bool dpart = use_v3 && ndistinct == 1;
+  bool dpart = use_v3 && ndistinct == 1 && hot_cap <= kPairHotMax;
   {
     const char* env = std::getenv("LDF_GROUPBY_DPART");
     if (env != nullptr && env[0] == '0') dpart = false;
@@ -2954,7 +3012,8 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     Buf<uint16_t> dcodes;  // the key's dense class table (k_gb_hist3)
     Buf<uint64_t> hoffs, hpoffs;  // hot pairs: per histogram block; per (pair partition, main block)
     Buf<uint32_t> hpcnt;          // hot pairs per (pair partition, main block)
-    Buf<unsigned long long> hpairs, hfine, cpairs, cfine;
+    Buf<uint32_t> hpairs;  // hot pairs' stored words by pair partition
+    Buf<unsigned long long> hfine, cpairs, cfine;
     Buf<uint32_t> part_cnt;  // hot pairs' new-pair counts per (pair partition, hot key)
     Buf<uint32_t> cpair_count;
     plan.dpart = 0;
@@ -3104,7 +3163,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       precs  = Buf<unsigned char>(a, (size_t)std::max<int64_t>(1, ncold) * (size_t)pb.stride);
       pb.rec = precs.get();
       if (dpart) {
-        hpairs      = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, nhp));
+        hpairs      = Buf<uint32_t>(a, (size_t)std::max<int64_t>(1, nhp));
         hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
         pb.hpairs   = hfine.get();  // row order; split by pair partition into hpairs after the pass
         pb.hoffsets = hoffs.get();
@@ -3158,8 +3217,9 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const unsigned long long*)hfine.get(),
                            (const uint64_t*)hoffs.get(), kHistSplit, (const uint64_t*)hpoffs.get(), hpairs.get());
         LDF_HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, side.s, (const unsigned long long*)hpairs.get(),
-                           (const uint64_t*)hpoffs.get(), nblk, hfine.get(), hot_cap, hot, part_cnt.get(), d.err);
+        hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, side.s, (const uint32_t*)hpairs.get(),
+                           (const uint64_t*)hpoffs.get(), nblk, reinterpret_cast<uint32_t*>(hfine.get()), hot_cap, hot,
+                           part_cnt.get(), d.err);
         LDF_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_gb_hot_dist_sum, dim3(hot_cap), dim3(256), 0, side.s, (const uint32_t*)part_cnt.get(),
                            kHPairParts, hot_cap, hot, d.g0);
