@@ -144,6 +144,10 @@ struct KeyTab {
 __device__ __forceinline__ uint64_t probe_limit(const KeyTab& t) { return t.mask < 4095 ? t.mask + 1 : 4096; }
 
 // slot of a single fixed-width key given by its bits (kv = the key is valid)
+// COUNT = false: the table's key count is not kept (a table that holds at most half its slots,
+// such as the sample's, cannot pass the 3/4 load check; a single counter that every new key
+// increments serialises the inserts)
+template <bool COUNT = true>
 __device__ __forceinline__ int64_t gb_insert_bits(unsigned long long key, bool kv, const KeyTab& t)
 {
   if (!kv) {
@@ -161,8 +165,10 @@ __device__ __forceinline__ int64_t gb_insert_bits(unsigned long long key, bool k
     if (cur == kGEmpty) {
       unsigned long long old = atomicCAS(&t.slots[i], kGEmpty, key);
       if (old == kGEmpty) {
-        unsigned long long nk = atomicAdd(t.nkeys, 1ull);
-        if (nk + 1 > (t.cap / 4) * 3) *t.overflow = 1;
+        if (COUNT) {
+          unsigned long long nk = atomicAdd(t.nkeys, 1ull);
+          if (nk + 1 > (t.cap / 4) * 3) *t.overflow = 1;
+        }
         return (int64_t)i;
       }
       if (old == key) return (int64_t)i;
@@ -2216,16 +2222,16 @@ __global__ __launch_bounds__(1024) void k_gb_sample_lds(DKeys keys, int64_t n, i
   __syncthreads();
   for (int j = threadIdx.x; j < kSampleSlots; j += blockDim.x) {
     if (tc[j] == 0) continue;
-    const int64_t sl = gb_insert_bits(tk[j], true, t);
+    const int64_t sl = gb_insert_bits<false>(tk[j], true, t);
     if (sl >= 0) atomicAdd(&cnt[sl], tc[j]);
   }
   if (threadIdx.x == 0) {
     if (s_null) {
-      const int64_t sl = gb_insert_bits(0, false, t);
+      const int64_t sl = gb_insert_bits<false>(0, false, t);
       if (sl >= 0) atomicAdd(&cnt[sl], s_null);
     }
     if (s_empty) {
-      const int64_t sl = gb_insert_bits(kGEmpty, true, t);
+      const int64_t sl = gb_insert_bits<false>(kGEmpty, true, t);
       if (sl >= 0) atomicAdd(&cnt[sl], s_empty);
     }
   }
@@ -2925,6 +2931,10 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     const double keys_per_part = 1.25 * (double)est / nparts;
     q.S                        = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
     dense                      = (int64_t)nparts * q.S * q.tcap;
+    // The hash table then holds only the hot keys and the keys that spill from their partition's
+    // LDS table (rare at the planned load): room for a quarter of the estimate (a table that
+    // overflows is rebuilt four times larger, as on the other path).
+    cap = std::min(cap, next_pow2(std::max<uint64_t>(1024, (uint64_t)(2.5 * ((double)hot_cap + 0.25 * (double)est)))));
   }
 
   for (;;) {
