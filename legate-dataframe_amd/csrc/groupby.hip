@@ -2165,11 +2165,11 @@ __global__ void k_gb_sample(DKeys keys, int64_t n, int64_t S, int64_t stride, Ke
   }
 }
 
-// The same sample for a single fixed-width key, counted first in each block's LDS table (4096
+// The same sample for a single fixed-width key, counted first in each block's LDS table (1024
 // sampled rows per block), so the global table takes one insert and one add per distinct key
 // per block instead of one per sampled row (the heaviest keys would serialise on them).
-constexpr int kSampleBlockRows = 4096;
-constexpr int kSampleSlots     = 8192;
+constexpr int kSampleBlockRows = 1024;  // 64 blocks for the 65536-row sample
+constexpr int kSampleSlots     = 2048;
 __global__ __launch_bounds__(1024) void k_gb_sample_lds(DKeys keys, int64_t n, int64_t S, int64_t stride, KeyTab t,
                                                         unsigned int* cnt)
 {
