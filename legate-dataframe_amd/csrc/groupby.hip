@@ -637,7 +637,10 @@ struct PartBufs {
   int32_t flags_off;
   int32_t key_width;
   int32_t nblk;  // histogram blocks: the stride of offsets / hoffsets
-  unsigned long long* hpairs;  // dpart: (hot index << 32 | value) pairs of hot rows, by pair partition
+  // dpart: the hot rows' pairs (hot index << 32 | value) in row order, as the image under
+  // pair_mix: its low 32 bits (hwords) and its pair partition, bits 32..41 (hparts)
+  uint32_t* hwords;
+  uint16_t* hparts;
   const uint64_t* hoffsets;    // hot pairs before each histogram block (exclusive scan)
   uint32_t* pcounts;           // k_gb_main3's hot pairs per (pair partition, block)
   // dpart with one sub-pass: k_gb_part3 appends each partition's (key << 32 | value) pairs of rows
@@ -716,7 +719,6 @@ __host__ __device__ __forceinline__ uint64_t pair_unmix(uint64_t x)
   x = (x * kPM1i) & kP42;
   return x ^ (x >> 21);
 }
-__device__ __forceinline__ uint32_t pair_part(unsigned long long pr) { return (uint32_t)(pair_mix(pr) >> 32); }
 constexpr int kHFineBits   = 5;
 constexpr int kHFine       = 1 << kHFineBits;  // fine partitions of a coarse hot-pair partition
 constexpr int kPairSet32   = 8192;             // LDS set slots of the hot dedupe (32 KB of words)
@@ -1148,8 +1150,9 @@ __device__ __forceinline__ void lds_put(unsigned char* at, int width, uint64_t b
 // accumulator is exact and order-independent).  LDS: [0, p.acc_end) hot accumulators, then
 // cursor u64[2^part_bits], and with p.stage one record-sized slot per thread where a record's
 // fields are written before it is stored as 16-byte pieces.  With DPART the hot rows'
-// (hot index, value) pairs are appended in row order to the block's range of pb.hpairs (one
-// LDS counter, coalesced stores); k_gb_pair_part then splits them by pair partition.
+// (hot index, value) pairs are appended in row order to the block's range of pb.hwords /
+// pb.hparts as their image under pair_mix (one LDS counter, coalesced stores); k_gb_pair_part then
+// splits them by pair partition.
 template <bool DPART, int NC>
 __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, GbPlan p, HotList hot, PartBufs pb,
                                                          HotParts hp, const uint16_t* __restrict__ codes)
@@ -1228,9 +1231,11 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         if (lane == leader) at = atomicAdd(&s_pcur, (uint32_t)__popcll(m));
         at = __shfl(at, leader, kWave);
         if (has) {
-          const unsigned long long pr = ((unsigned long long)(uint32_t)hx[j] << 32) | (uint32_t)pick(v[j], p.dcol);
-          pb.hpairs[pair0 + at + (uint32_t)__popcll(m & below)] = pr;
-          atomicAdd(&phist[pair_part(pr)], 1u);
+          const uint64_t x = pair_mix(((uint64_t)(uint32_t)hx[j] << 32) | (uint32_t)pick(v[j], p.dcol));
+          const int64_t o  = pair0 + at + (uint32_t)__popcll(m & below);
+          pb.hwords[o]     = (uint32_t)x;
+          pb.hparts[o]     = (uint16_t)(x >> 32);
+          atomicAdd(&phist[(uint32_t)(x >> 32)], 1u);
         }
       }
     }
@@ -1647,24 +1652,25 @@ __device__ __forceinline__ int pair_set_insert(unsigned long long* set, unsigned
   return -1;
 }
 
-// The hot pairs, appended in row order by k_gb_main3 (block b's at [hoffs[b * split],
-// hoffs[(b + 1) * split])), split into the kHPairParts pair partitions, one block per main-pass
-// block (1024 threads = kHPairParts): k_gb_main3 counted its pairs per partition, and `off` is the
-// scan of those counts (partition-major).  Each tile of kPairTile pairs is sorted by partition in
-// LDS and every partition's stored words (the low 32 bits of the pair's image under pair_mix)
-// of the tile are written as one contiguous run (order inside a partition does not matter to the
-// deduplication).
+// The hot pairs, appended in row order by k_gb_main3 as their stored words and pair partitions
+// (block b's at [hoffs[b * split], hoffs[(b + 1) * split])), split into the kHPairParts pair
+// partitions, one block per main-pass block (1024 threads = kHPairParts): k_gb_main3 counted its
+// pairs per partition, and `off` is the scan of those counts (partition-major).  Each tile of
+// kPairTile pairs is sorted by partition in LDS and every partition's words of the tile are
+// written as one contiguous run (order inside a partition does not matter to the deduplication).
 constexpr int kPairBatch = 8;   // pairs a thread loads before using any (deduplication loops)
-constexpr int kPairTile = 16384;  // 16 pairs per partition and tile on average: runs of whole lines
+constexpr int kPairTile = 20480;  // 20 pairs per partition and tile on average (LDS: 6 bytes a pair)
 
-__global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long* __restrict__ src,
+__global__ __launch_bounds__(1024) void k_gb_pair_part(const uint32_t* __restrict__ words,
+                                                       const uint16_t* __restrict__ parts,
                                                        const uint64_t* __restrict__ hoffs, int split,
                                                        const uint64_t* __restrict__ off, uint32_t* __restrict__ dst)
 {
   __shared__ uint32_t tcnt[kHPairParts];
   __shared__ uint32_t tstart[kHPairParts];
   __shared__ unsigned long long cur[kHPairParts];
-  __shared__ unsigned long long stage[kPairTile];
+  __shared__ uint32_t stage_w[kPairTile];
+  __shared__ uint16_t stage_q[kPairTile];
   __shared__ uint64_t smem[17];
   constexpr int R  = kPairTile / kHPairParts;  // pairs per thread and tile
   const int64_t b0 = (int64_t)hoffs[(int64_t)blockIdx.x * split];
@@ -1674,18 +1680,16 @@ __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long*
     const int tn     = (int)(b1 - t0 < kPairTile ? b1 - t0 : kPairTile);
     tcnt[threadIdx.x] = 0;
     __syncthreads();
-    unsigned long long pr[R];
-    uint32_t part[R], rank[R];
+    uint32_t w[R], part[R], rank[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
+    for (int j = 0; j < R; ++j) {  // unconditional (past the tile: its first pair)
       const int i = threadIdx.x + j * kHPairParts;
-      pr[j]       = src[t0 + (i < tn ? i : 0)];  // unconditional (past the tile: its first pair)
+      w[j]        = words[t0 + (i < tn ? i : 0)];
+      part[j]     = parts[t0 + (i < tn ? i : 0)];
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = threadIdx.x + j * kHPairParts;
-      pr[j]       = pair_mix(pr[j]);  // the image: partition in bits 32..41, stored word below
-      part[j]     = (uint32_t)(pr[j] >> 32);
       rank[j]     = i < tn ? atomicAdd(&tcnt[part[j]], 1u) : 0u;
     }
     __syncthreads();
@@ -1696,14 +1700,15 @@ __global__ __launch_bounds__(1024) void k_gb_pair_part(const unsigned long long*
     for (int j = 0; j < R; ++j) {
       const int i = threadIdx.x + j * kHPairParts;
       if (i < tn) {
-        stage[tstart[part[j]] + rank[j]] = pr[j];
+        const uint32_t pos = tstart[part[j]] + rank[j];
+        stage_w[pos]       = w[j];
+        stage_q[pos]       = (uint16_t)part[j];
       }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += kHPairParts) {
-      const unsigned long long v = stage[i];
-      const uint32_t q           = (uint32_t)(v >> 32);
-      dst[cur[q] + (uint64_t)(i - (int)tstart[q])] = (uint32_t)v;
+      const uint32_t q = stage_q[i];
+      dst[cur[q] + (uint64_t)(i - (int)tstart[q])] = stage_w[i];
     }
     __syncthreads();
     cur[threadIdx.x] += tcnt[threadIdx.x];
@@ -3175,7 +3180,10 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       if (dpart) {
         hpairs      = Buf<uint32_t>(a, (size_t)std::max<int64_t>(1, nhp));
         hfine       = Buf<unsigned long long>(a, (size_t)std::max<int64_t>(1, std::max(nhp, ncold)));
-        pb.hpairs   = hfine.get();  // row order; split by pair partition into hpairs after the pass
+        // row order (words, then partitions: 6 bytes a pair); split by pair partition into hpairs
+        // after the pass, after which the hot dedupe reuses the buffer
+        pb.hwords   = reinterpret_cast<uint32_t*>(hfine.get());
+        pb.hparts   = reinterpret_cast<uint16_t*>(pb.hwords + std::max<int64_t>(1, nhp));
         pb.hoffsets = hoffs.get();
         hpcnt       = Buf<uint32_t>(a, (size_t)kHPairParts * nblk);
         pb.pcounts  = hpcnt.get();
@@ -3224,8 +3232,9 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         forked = true;
         ProfScope prof("gb_hot_distinct", side.s);
         const AccDev& d = plan.a[dacc];
-        hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const unsigned long long*)hfine.get(),
-                           (const uint64_t*)hoffs.get(), kHistSplit, (const uint64_t*)hpoffs.get(), hpairs.get());
+        hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const uint32_t*)pb.hwords,
+                           (const uint16_t*)pb.hparts, (const uint64_t*)hoffs.get(), kHistSplit,
+                           (const uint64_t*)hpoffs.get(), hpairs.get());
         LDF_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_gb_hot_distinct, dim3(kHPairParts), dim3(1024), 0, side.s, (const uint32_t*)hpairs.get(),
                            (const uint64_t*)hpoffs.get(), nblk, reinterpret_cast<uint32_t*>(hfine.get()), hot_cap, hot,
