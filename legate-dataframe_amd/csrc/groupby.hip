@@ -1876,7 +1876,7 @@ __global__ __launch_bounds__(256) void k_gb_hot_dist_sum(const uint32_t* __restr
 // LDS set, and each new pair counts for its key in an LDS key table that holds all of the
 // partition's keys; every key's count is added to its slot at the end.
 constexpr int kColdKeySlots = 4096;
-constexpr int kColdGroups   = 64;
+constexpr int kColdGroups   = 128;
 
 // the (key, value) pair of partition row r when its value is valid (from the appended pairs when
 // k_gb_part3 wrote them: then every index below the partition's pair count holds one)
@@ -1917,8 +1917,8 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct(PartBufs pb, int dcol
   const int part   = blockIdx.x;
   const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
   const int64_t r1 = pb.cpairs != nullptr ? r0 + pb.cpair_count[part] : (int64_t)pb.offsets[(int64_t)(part + 1) * pb.nblk];
-  // groups of at most 0.7 * kPairSet pairs when every row has a pair
-  const int ng = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+  // groups of at most 0.35 * kPairSet pairs when every row has a pair (short probe runs)
+  const int ng = (int)((r1 - r0) / (int64_t)(0.35 * kPairSet)) + 1;
   if (ng > kColdGroups) {  // more rows than this kernel plans for: the caller falls back
     if (threadIdx.x == 0) atomicOr(err, 16);
     return;
@@ -2064,7 +2064,9 @@ __global__ __launch_bounds__(1024) void k_gb_cold_distinct_slots(PartBufs pb, Ke
   const int64_t r0 = (int64_t)pb.offsets[(int64_t)part * pb.nblk];
   const int64_t r1 = r0 + pb.cpair_count[part];
   const int64_t d0 = (int64_t)t.cap + 2 + (int64_t)part * tcap;
-  const int ng     = (int)((r1 - r0) / (int64_t)(0.7 * kPairSet)) + 1;
+  // groups of at most 35% of the set: an insert's probe run (and so a wave's longest, which the
+  // wave waits for) stays short
+  const int ng     = (int)((r1 - r0) / (int64_t)(0.35 * kPairSet)) + 1;
   if (ng > kColdGroups) {  // more rows than this kernel plans for: the caller falls back
     if (threadIdx.x == 0) atomicOr(err, 16);
     return;

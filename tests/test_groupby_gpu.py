@@ -284,6 +284,31 @@ def test_groupby_c2_full_size_count_distinct(gpu, keys_kind):
     assert int(got.sum()) == int(ref.sum())
 
 
+def test_groupby_count_distinct_group_overflow(gpu):
+    """COUNT_DISTINCT where the keys outside the hot set have ~8K rows each and nearly all of a
+    key's rows share one value: ~2 such keys per partition, so a partition's ~16K pairs split by
+    pair hash into three dedupe groups very unevenly (the repeated pairs all land in one group,
+    far past a third of the partition).  Expected counts from torch.unique over the (key, value)
+    pairs."""
+    import torch
+
+    from legate_dataframe import _capi as C
+
+    n, nk = 24_000_000, 3_000
+    g = torch.Generator(device="cuda").manual_seed(11)
+    k = torch.randint(0, nk, (n,), device="cuda", generator=g, dtype=torch.int32)
+    few = torch.randint(0, 100, (n,), device="cuda", generator=g, dtype=torch.int32)
+    u = torch.where(torch.rand(n, device="cuda", generator=g) < 0.9, torch.full_like(few, 5), few)
+    dc = lambda t, tid: C.DeviceColumn(tid, t.numel(), t.view(torch.uint8))  # noqa: E731
+    out = C.groupby([dc(k, C.INT32), dc(u, C.INT32)], [0], [(1, "count_distinct", None)])
+    keys = out[0].data.view(torch.int32)[: out[0].length].to(torch.int64)
+    got = out[1].data.view(torch.int64)[: out[0].length]
+    pairs = torch.unique((k.to(torch.int64) << 32) | u.to(torch.int64))
+    ref = torch.bincount(pairs >> 32, minlength=nk)
+    assert out[0].length == nk
+    assert torch.equal(got, ref[keys])
+
+
 @pytest.mark.parametrize("n", [5_000, 1_500_000])
 def test_groupby_approximate_median_vs_oracle(gpu, n):
     """approximate_median is the exact median (cudf's, which the reference's GPU variant uses):
