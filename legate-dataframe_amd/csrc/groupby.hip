@@ -1148,7 +1148,7 @@ __device__ __forceinline__ void lds_put(unsigned char* at, int width, uint64_t b
 // each takes the next position of its partition's region with an LDS atomic on the region
 // cursor (no barrier per tile; order inside a partition is irrelevant because every
 // accumulator is exact and order-independent).  LDS: [0, p.acc_end) hot accumulators, then
-// cursor u64[2^part_bits], and with p.stage one record-sized slot per thread where a record's
+// cursor u32[2^part_bits] (record indices: fewer than 2^31 rows), and with p.stage one record-sized slot per thread where a record's
 // fields are written before it is stored as 16-byte pieces.  With DPART the hot rows'
 // (hot index, value) pairs are appended in row order to the block's range of pb.hwords /
 // pb.hparts as their image under pair_mix (one LDS counter, coalesced stores); k_gb_pair_part then
@@ -1159,7 +1159,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
 {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_null_hot, s_sent_hot;
-  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(lds + p.acc_end);
+  uint32_t* cursor           = reinterpret_cast<uint32_t*>(lds + p.acc_end);
   const int nparts           = 1 << p.part_bits;
   uint32_t* phist            = reinterpret_cast<uint32_t*>(cursor + nparts);  // DPART: pairs per pair partition
   unsigned char* stage = reinterpret_cast<unsigned char*>(phist + (DPART ? kHPairParts : 0)) +
@@ -1167,7 +1167,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
 
   const int nhot = (int)min(*hot.count, (unsigned int)p.hot_cap);
   const int64_t hb0 = (int64_t)blockIdx.x * kHistSplit;  // first histogram block of this block's rows
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = pb.offsets[(int64_t)i * pb.nblk + hb0];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) cursor[i] = (uint32_t)pb.offsets[(int64_t)i * pb.nblk + hb0];
   __shared__ uint32_t s_pcur;  // hot pairs this block has appended
   if (threadIdx.x == 0) s_pcur = 0;
   if (DPART)
@@ -1263,7 +1263,7 @@ __global__ __launch_bounds__(kGbThreads) void k_gb_main3(DKeys keys, int64_t n, 
         }
         ++di;
       }
-      const int64_t dest = (int64_t)atomicAdd(&cursor[part[j]], 1ull);
+      const int64_t dest = (int64_t)atomicAdd(&cursor[part[j]], 1u);
       if (p.stage) {  // fields into this thread's LDS slot, then the record as 16-byte pieces
         lds_put(stage + pb.key_off, pb.key_width, kb[j]);
         stage[pb.flags_off] = (uint8_t)fl;
@@ -2927,11 +2927,11 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     int stride = 8;
     while (stride < rec) stride *= 2;
     const int stage_b = kGbThreads * stride;
-    plan.stage        = plan.acc_end + (1 << kMinPartBits) * 8 + hc_bytes + stage_b <= kMain3Lds ? 1 : 0;
+    plan.stage        = plan.acc_end + (1 << kMinPartBits) * 4 + hc_bytes + stage_b <= kMain3Lds ? 1 : 0;
     const int main_fixed = plan.acc_end + hc_bytes + (plan.stage ? stage_b : 0);
     int pbits            = kMinPartBits;
     while (pbits < kMaxPartBits && 1.25 * (double)est / (double)(1 << pbits) > 0.45 * q.tcap &&
-           main_fixed + (1 << (pbits + 1)) * 8 <= kMain3Lds)
+           main_fixed + (1 << (pbits + 1)) * 4 <= kMain3Lds)
       ++pbits;
     plan.part_bits             = pbits;
     const int nparts           = 1 << pbits;
@@ -3047,7 +3047,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
       for (int c = 0; c < plan.ncol; ++c) rec_bytes += plan.col[c].width;
       int rec_stride = 8;
       while (rec_stride < rec_bytes) rec_stride *= 2;
-      const int main_lds = plan.acc_end + nparts * 8 + hc_bytes + (plan.stage ? kGbThreads * rec_stride : 0);
+      const int main_lds = plan.acc_end + nparts * 4 + hc_bytes + (plan.stage ? kGbThreads * rec_stride : 0);
       int per_cu            = std::max(1, std::min(2, (160 * 1024) / main_lds));
       int nblk              = 256 * per_cu;
       int64_t rpb           = (n + nblk - 1) / nblk;
