@@ -304,13 +304,17 @@ void ldf_pool_stats(int64_t* live_blocks, int64_t* live_bytes, int64_t* cached_b
  * instances' results concatenated, instance i owning rows [row_offsets[i], row_offsets[i+1]);
  * the buffers belong to the batch (valid until the next run or destroy).  Results are those
  * of ldf_join / ldf_groupby on each instance (rows in another order).
- * Limits: one fixed-width key column; join types INNER, LEFT, FULL, SEMI, ANTI (FULL: the
- * unmatched build rows of an instance follow its probe rows); fixed-width output
+ * The work-queue kernels take: one fixed-width key column; join types INNER, LEFT, FULL, SEMI,
+ * ANTI (FULL: the unmatched build rows of an instance follow its probe rows); fixed-width output
  * columns; groupby kinds SUM, MEAN, COUNT, COUNT_ALL, MIN, MAX over at most 4 value columns
  * (an FP64 SUM / MEAN input outside the exact range |x| < 2^62 makes ldf_batch_result return
- * LDF_E_UNSUPPORTED).  A run over inputs whose sizes changed after create may make
- * ldf_batch_result return LDF_E_INVALID (create the batch again).  No counterpart in the
- * reference. */
+ * LDF_E_UNSUPPORTED).  Any other schema ldf_join / ldf_groupby take (several or string keys,
+ * string outputs, COUNT_DISTINCT, PRODUCT, VARIANCE, STDDEV, APPROX_MEDIAN, or more groups than
+ * the LDS plan holds) is batched instance by instance: ldf_batch_run calls the plain entry point
+ * on each instance (allocating, one host synchronisation each; no graph) and concatenates the
+ * outputs, each instance's rows in the plain call's order; ldf_batch_config_used then reports
+ * all zeros.  A run over inputs whose sizes changed after create may make ldf_batch_result
+ * return LDF_E_INVALID (create the batch again).  No counterpart in the reference. */
 typedef struct ldf_batch_config {
   double load_factor; /* hash-table load factor in (0, 0.9]; 0 = autotuned */
   int32_t tile_rows;  /* rows per work item: 1024, 2048 or 4096; 0 = autotuned */

@@ -1503,7 +1503,8 @@ GroupbyBatch* create_groupby(int32_t ninst, const ldf_table_view* tables, const 
 {
   LDF_EXPECTS(ninst >= 1 && tables != nullptr, "batch groupby: no instances");
   LDF_EXPECTS(nkeys == 1 && keys != nullptr, "batch groupby: exactly one key column");
-  LDF_EXPECTS(naggs >= 0 && naggs <= kMaxBAgg && (naggs == 0 || aggs != nullptr), "batch groupby: at most 16 aggregations");
+  LDF_EXPECTS(naggs >= 0 && (naggs == 0 || aggs != nullptr), "batch groupby: NULL aggregations");
+  if (naggs > kMaxBAgg) throw Error(LDF_E_UNSUPPORTED, "batch groupby: at most 16 aggregations");
   check_config(cfg);
   const ldf_table_view& T0 = tables[0];
   LDF_EXPECTS(keys[0] >= 0 && keys[0] < T0.ncols, "groupby: key index out of range");
@@ -1514,7 +1515,7 @@ GroupbyBatch* create_groupby(int32_t ninst, const ldf_table_view* tables, const 
   auto vslot = [&](int col) {
     for (size_t c = 0; c < sc.vcols.size(); ++c)
       if (sc.vcols[c] == col) return (int)c;
-    LDF_EXPECTS(sc.vcols.size() < (size_t)kMaxVal, "batch groupby: at most 4 value columns");
+    if (sc.vcols.size() >= (size_t)kMaxVal) throw Error(LDF_E_UNSUPPORTED, "batch groupby: at most 4 value columns");
     sc.vcols.push_back(col);
     sc.vtype.push_back(T0.cols[col].type_id);
     sc.vnull.push_back(false);
@@ -1961,14 +1962,26 @@ ldf_batch_config resolve_groupby_config(const ldf_batch_config* in, int32_t nins
   std::vector<ldf_table_view> st;
   for (int i : smp) st.push_back(tables[i]);
   std::vector<ldf_batch_config> ok;
+  int first_code = 0;
+  std::string first_msg;
   for (auto& k : cands) {  // drop the LDS layouts that do not fit
     try {
       std::unique_ptr<BatchImpl> b(create_groupby(1, st.data(), keys, nkeys, aggs, naggs, k, s));
       ok.push_back(k);
-    } catch (const Error&) {
+    } catch (const Error& e) {
+      if (first_code == 0 || e.code == LDF_E_UNSUPPORTED) {
+        first_code = e.code;
+        first_msg  = e.what();
+      }
     }
   }
-  LDF_EXPECTS(!ok.empty(), "batch groupby: no LDS table layout fits these aggregations");
+  // every layout failed: a limit of the work-queue form (UNSUPPORTED: the caller batches instance
+  // by instance), or the request itself
+  if (ok.empty()) {
+    if (first_code == LDF_E_UNSUPPORTED) throw Error(LDF_E_UNSUPPORTED, first_msg);
+    if (first_code != 0) throw Error(first_code, first_msg);
+    throw Error(LDF_E_UNSUPPORTED, "batch groupby: no LDS table layout fits these aggregations");
+  }
   ldf_batch_config best = tune(c, ok, s, [&](const ldf_batch_config& k) -> BatchImpl* {
     return create_groupby((int32_t)smp.size(), st.data(), keys, nkeys, aggs, naggs, k, s);
   });
@@ -2323,6 +2336,230 @@ struct Server {
   }
 };
 
+// ---- instances the work-queue kernels do not take --------------------------------------------
+// Composite or string keys, string output columns, and the groupby kinds COUNT_DISTINCT,
+// PRODUCT, VARIANCE, STDDEV and APPROX_MEDIAN: such a batch runs each instance through the plain
+// path (ldf_join / ldf_groupby on the run's stream: their allocations and one host
+// synchronisation per instance) and concatenates the instances' output columns into buffers the
+// batch owns.  No graph capture; the outputs are the plain calls' rows, instance by instance.
+
+struct ConcatSrc {
+  const uint8_t* data;    // values, or chars for strings
+  const uint8_t* valid;   // NULL: every row valid
+  const int64_t* offs;    // strings: length + 1 offsets
+  int64_t cbase;          // strings: the instance's first char in the output
+};
+
+// one thread per output row: instance by binary search over the row bases (ninst + 1 entries)
+__global__ void k_concat_rows(const ConcatSrc* __restrict__ src, const int64_t* __restrict__ bases, int ninst, int w,
+                              int64_t total, uint8_t* __restrict__ dst, uint32_t* __restrict__ dvalid,
+                              int64_t* __restrict__ doffs)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < total; r += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = ninst;  // bases[lo] <= r < bases[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (bases[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    const ConcatSrc c = src[lo];
+    const int64_t o   = r - bases[lo];
+    if (w > 0) {
+      for (int b = 0; b < w; ++b) dst[r * w + b] = c.data[o * w + b];
+    } else {
+      doffs[r] = c.offs[o] - c.offs[0] + c.cbase;
+      if (r == total - 1) doffs[total] = c.offs[o + 1] - c.offs[0] + c.cbase;
+    }
+    if (dvalid != nullptr) {
+      const uint32_t bit = c.valid == nullptr ? 1u : ((uint32_t)c.valid[o >> 3] >> (uint32_t)(o & 7)) & 1u;
+      if (bit) atomicOr(&dvalid[r >> 5], 1u << (uint32_t)(r & 31));
+    }
+  }
+}
+
+struct BatchPlain : BatchImpl {
+  bool is_join = false;
+  int32_t ninst = 0, join_type = 0, nulls_equal = 0;
+  std::vector<std::vector<ldf_column_view>> lcols, rcols;  // copies of the instances' views
+  std::vector<int32_t> lkeys, rkeys, lout, rout;
+  std::vector<ldf_agg> aggs;
+  int ncols_out = 0;
+  // the last run's concatenated columns (batch-owned, freed by the next run or destroy)
+  std::vector<ldf_column> outs;
+  std::vector<int64_t> bases;
+  std::vector<std::pair<void*, size_t>> outbufs;
+  hipStream_t out_stream = nullptr;
+
+  ~BatchPlain() override { free_outputs(); }
+
+  void free_outputs()
+  {
+    if (outbufs.empty()) return;
+    Alloc a(nullptr, out_stream);
+    for (auto& o : outbufs) a.deallocate(o.first, o.second);
+    outbufs.clear();
+  }
+  void* out_alloc(hipStream_t s, size_t bytes)
+  {
+    Alloc a(nullptr, s);
+    void* p = a.allocate(bytes > 0 ? bytes : 1);
+    outbufs.push_back({p, bytes > 0 ? bytes : 1});
+    return p;
+  }
+
+  void enqueue(hipStream_t s) override
+  {
+    free_outputs();
+    out_stream = s;
+    outs.assign(ncols_out, ldf_column{});
+    bases.assign(ninst + 1, 0);
+    std::vector<std::vector<ldf_column>> per(ninst, std::vector<ldf_column>(std::max(1, ncols_out)));
+    auto free_all = [&](int upto) {
+      for (int i = 0; i < upto; ++i) {
+        ldf_table_out t{ncols_out, per[i].data()};
+        ldf_free_table(&t, nullptr, s);
+      }
+    };
+    for (int i = 0; i < ninst; ++i) {
+      ldf_table_out t{ncols_out, per[i].data()};
+      int rc;
+      if (is_join) {
+        const ldf_table_view lv{(int32_t)lcols[i].size(), lcols[i].data()};
+        const ldf_table_view rv{(int32_t)rcols[i].size(), rcols[i].data()};
+        rc = ldf_join(&lv, &rv, lkeys.data(), rkeys.data(), (int32_t)lkeys.size(), join_type, nulls_equal,
+                      lout.data(), (int32_t)lout.size(), rout.data(), (int32_t)rout.size(), nullptr, s, &t);
+      } else {
+        const ldf_table_view tv{(int32_t)lcols[i].size(), lcols[i].data()};
+        rc = ldf_groupby(&tv, lkeys.data(), (int32_t)lkeys.size(), aggs.data(), (int32_t)aggs.size(), nullptr, s, &t);
+      }
+      if (rc != LDF_OK) {
+        const std::string msg = std::string("batch instance ") + std::to_string(i) + ": " + ldf_last_error();
+        free_all(i);
+        throw Error(rc, msg);
+      }
+      LDF_EXPECTS(t.ncols == ncols_out, "batch: an instance returned another column count");
+      bases[i + 1] = bases[i] + (ncols_out > 0 ? per[i][0].length : 0);
+    }
+    const int64_t total = bases[ninst];
+    int64_t* dbases     = static_cast<int64_t*>(out_alloc(s, (size_t)(ninst + 1) * 8));
+    LDF_HIP_TRY(hipMemcpyAsync(dbases, bases.data(), (size_t)(ninst + 1) * 8, hipMemcpyHostToDevice, s));
+    std::vector<ConcatSrc> hsrc(ninst);
+    ConcatSrc* dsrc = static_cast<ConcatSrc*>(out_alloc(s, (size_t)ninst * sizeof(ConcatSrc)));
+    for (int c = 0; c < ncols_out; ++c) {
+      ldf_column& o    = outs[c];
+      const int32_t ty = per[0][c].type_id;
+      const bool str   = ty == LDF_TYPE_LARGE_STRING;
+      const int w      = str ? 0 : type_width(ty);
+      bool nullable    = false;
+      int64_t nulls = 0, chars = 0;
+      std::vector<int64_t> first(ninst, 0);  // strings: each instance's first offset
+      for (int i = 0; i < ninst; ++i) {
+        const ldf_column& pc = per[i][c];
+        LDF_EXPECTS(pc.type_id == ty, "batch: instances returned different column types");
+        nullable |= pc.validity != nullptr;
+        nulls += pc.null_count > 0 ? pc.null_count : 0;
+        hsrc[i] = ConcatSrc{static_cast<const uint8_t*>(pc.data), pc.validity, pc.offsets, chars};
+        if (str && pc.length > 0) {
+          int64_t ends[2] = {0, 0};  // the instance's first and last offsets
+          LDF_HIP_TRY(hipMemcpyAsync(&ends[0], pc.offsets, 8, hipMemcpyDeviceToHost, s));
+          LDF_HIP_TRY(hipMemcpyAsync(&ends[1], pc.offsets + pc.length, 8, hipMemcpyDeviceToHost, s));
+          LDF_HIP_TRY(hipStreamSynchronize(s));
+          first[i] = ends[0];
+          chars += ends[1] - ends[0];
+        }
+      }
+      o.type_id        = ty;
+      o.length         = total;
+      o.null_count     = nullable ? nulls : 0;
+      o.data_bytes     = str ? chars : total * w;
+      o.data           = out_alloc(s, (size_t)std::max<int64_t>(1, o.data_bytes));
+      o.validity_bytes = nullable ? ((total + 63) / 64) * 8 : 0;
+      o.validity       = nullable ? static_cast<uint8_t*>(out_alloc(s, (size_t)std::max<int64_t>(8, o.validity_bytes)))
+                                  : nullptr;
+      o.offsets        = str ? static_cast<int64_t*>(out_alloc(s, (size_t)(total + 1) * 8)) : nullptr;
+      if (nullable) LDF_HIP_TRY(hipMemsetAsync(o.validity, 0, (size_t)std::max<int64_t>(8, o.validity_bytes), s));
+      if (str) {
+        if (total == 0) LDF_HIP_TRY(hipMemsetAsync(o.offsets, 0, 8, s));
+        for (int i = 0; i < ninst; ++i) {  // characters: one copy per instance
+          const ldf_column& pc = per[i][c];
+          if (pc.length == 0) continue;
+          const int64_t nb = (i + 1 < ninst ? hsrc[i + 1].cbase : chars) - hsrc[i].cbase;
+          if (nb <= 0) continue;
+          LDF_HIP_TRY(hipMemcpyAsync(static_cast<uint8_t*>(o.data) + hsrc[i].cbase,
+                                     static_cast<const uint8_t*>(pc.data) + first[i], (size_t)nb,
+                                     hipMemcpyDeviceToDevice, s));
+        }
+      }
+      if (total > 0) {
+        LDF_HIP_TRY(hipMemcpyAsync(dsrc, hsrc.data(), (size_t)ninst * sizeof(ConcatSrc), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_concat_rows, dim3(grid_for(total, 256)), dim3(256), 0, s, (const ConcatSrc*)dsrc,
+                           (const int64_t*)dbases, ninst, w, total, static_cast<uint8_t*>(o.data),
+                           reinterpret_cast<uint32_t*>(o.validity), o.offsets);
+        LDF_HIP_TRY(hipGetLastError());
+        // the source descriptors are rewritten for the next column only after this one's rows
+        LDF_HIP_TRY(hipStreamSynchronize(s));
+      }
+    }
+    free_all(ninst);
+  }
+
+  void result(hipStream_t s, ldf_table_out* out, int64_t* offsets) override
+  {
+    LDF_EXPECTS(out != nullptr && out->cols != nullptr, "batch: NULL output table");
+    LDF_EXPECTS(out->ncols >= ncols_out, "batch: output table has too few columns");
+    LDF_HIP_TRY(hipStreamSynchronize(s));
+    LDF_EXPECTS((int)outs.size() == ncols_out, "batch: result before the first run");
+    out->ncols = ncols_out;
+    for (int c = 0; c < ncols_out; ++c) out->cols[c] = outs[c];
+    if (offsets != nullptr)
+      for (int i = 0; i <= ninst; ++i) offsets[i] = bases[i];
+  }
+};
+
+std::vector<ldf_column_view> copy_view(const ldf_table_view& t)
+{
+  LDF_EXPECTS(t.ncols >= 0 && (t.ncols == 0 || t.cols != nullptr), "batch: NULL table columns");
+  return std::vector<ldf_column_view>(t.cols, t.cols + t.ncols);
+}
+
+BatchImpl* create_plain_join(int32_t ninst, const ldf_table_view* lhs, const ldf_table_view* rhs,
+                             const int32_t* lhs_keys, const int32_t* rhs_keys, int32_t nkeys, int32_t join_type,
+                             bool nulls_equal, const int32_t* lhs_out, int32_t n_lhs_out, const int32_t* rhs_out,
+                             int32_t n_rhs_out, hipStream_t s)
+{
+  std::unique_ptr<BatchPlain> b(new BatchPlain());
+  b->is_join     = true;
+  b->ninst       = ninst;
+  b->join_type   = join_type;
+  b->nulls_equal = nulls_equal ? 1 : 0;
+  b->cs          = s;
+  for (int i = 0; i < ninst; ++i) {
+    b->lcols.push_back(copy_view(lhs[i]));
+    b->rcols.push_back(copy_view(rhs[i]));
+  }
+  b->lkeys.assign(lhs_keys, lhs_keys + nkeys);
+  b->rkeys.assign(rhs_keys, rhs_keys + nkeys);
+  if (n_lhs_out > 0) b->lout.assign(lhs_out, lhs_out + n_lhs_out);
+  if (n_rhs_out > 0) b->rout.assign(rhs_out, rhs_out + n_rhs_out);
+  b->ncols_out = n_lhs_out + n_rhs_out;
+  std::memset(&b->cfg, 0, sizeof(b->cfg));
+  return b.release();
+}
+
+BatchImpl* create_plain_groupby(int32_t ninst, const ldf_table_view* tables, const int32_t* keys, int32_t nkeys,
+                                const ldf_agg* aggs, int32_t naggs, hipStream_t s)
+{
+  std::unique_ptr<BatchPlain> b(new BatchPlain());
+  b->ninst = ninst;
+  b->cs    = s;
+  for (int i = 0; i < ninst; ++i) b->lcols.push_back(copy_view(tables[i]));
+  b->lkeys.assign(keys, keys + nkeys);
+  if (naggs > 0) b->aggs.assign(aggs, aggs + naggs);
+  b->ncols_out = nkeys + naggs;
+  std::memset(&b->cfg, 0, sizeof(b->cfg));
+  return b.release();
+}
+
 }  // namespace
 }  // namespace ldf
 
@@ -2523,13 +2760,22 @@ extern "C" int ldf_batch_join_create(int32_t ninst, const ldf_table_view* lhs, c
     *batch             = nullptr;
     hipStream_t s      = (hipStream_t)stream;
     const bool ne      = nulls_equal != 0;
-    LDF_EXPECTS(ninst >= 1 && lhs != nullptr && rhs != nullptr && lhs_keys && rhs_keys && nkeys == 1,
-                "batch join: needs >= 1 instance and exactly one key column");
-    ldf_batch_config c = resolve_join_config(cfg, ninst, lhs, rhs, lhs_keys, rhs_keys, join_type, ne, lhs_out,
-                                             n_lhs_out, rhs_out, n_rhs_out, s);
+    LDF_EXPECTS(ninst >= 1 && lhs != nullptr && rhs != nullptr && lhs_keys && rhs_keys && nkeys >= 1,
+                "batch join: needs >= 1 instance and a key column");
     std::unique_ptr<ldf_batch> b(new ldf_batch());
-    b->impl.reset(create_join(ninst, lhs, rhs, lhs_keys, rhs_keys, nkeys, join_type, ne, lhs_out, n_lhs_out, rhs_out,
-                              n_rhs_out, c, s));
+    if (nkeys == 1) {
+      try {  // the work-queue form; a schema it does not take runs instance by instance
+        ldf_batch_config c = resolve_join_config(cfg, ninst, lhs, rhs, lhs_keys, rhs_keys, join_type, ne, lhs_out,
+                                                 n_lhs_out, rhs_out, n_rhs_out, s);
+        b->impl.reset(create_join(ninst, lhs, rhs, lhs_keys, rhs_keys, nkeys, join_type, ne, lhs_out, n_lhs_out,
+                                  rhs_out, n_rhs_out, c, s));
+      } catch (const Error& e) {
+        if (e.code != LDF_E_UNSUPPORTED) throw;
+      }
+    }
+    if (!b->impl)
+      b->impl.reset(create_plain_join(ninst, lhs, rhs, lhs_keys, rhs_keys, nkeys, join_type, ne, lhs_out, n_lhs_out,
+                                      rhs_out, n_rhs_out, s));
     *batch = b.release();
   });
 }
@@ -2542,11 +2788,18 @@ extern "C" int ldf_batch_groupby_create(int32_t ninst, const ldf_table_view* tab
     LDF_EXPECTS(batch != nullptr, "batch: NULL output handle");
     *batch        = nullptr;
     hipStream_t s = (hipStream_t)stream;
-    LDF_EXPECTS(ninst >= 1 && tables != nullptr && keys != nullptr && nkeys == 1,
-                "batch groupby: needs >= 1 instance and exactly one key column");
-    ldf_batch_config c = resolve_groupby_config(cfg, ninst, tables, keys, nkeys, aggs, naggs, s);
+    LDF_EXPECTS(ninst >= 1 && tables != nullptr && keys != nullptr && nkeys >= 1,
+                "batch groupby: needs >= 1 instance and a key column");
     std::unique_ptr<ldf_batch> b(new ldf_batch());
-    b->impl.reset(create_groupby(ninst, tables, keys, nkeys, aggs, naggs, c, s));
+    if (nkeys == 1) {
+      try {  // the work-queue form; a schema it does not take runs instance by instance
+        ldf_batch_config c = resolve_groupby_config(cfg, ninst, tables, keys, nkeys, aggs, naggs, s);
+        b->impl.reset(create_groupby(ninst, tables, keys, nkeys, aggs, naggs, c, s));
+      } catch (const Error& e) {
+        if (e.code != LDF_E_UNSUPPORTED) throw;
+      }
+    }
+    if (!b->impl) b->impl.reset(create_plain_groupby(ninst, tables, keys, nkeys, aggs, naggs, s));
     *batch = b.release();
   });
 }

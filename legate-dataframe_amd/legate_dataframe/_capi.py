@@ -583,12 +583,19 @@ class Batch:
         out = TableOut(self.ncols, cols)
         offs = (ctypes.c_int64 * (self.ninst + 1))()
         check(load().ldf_batch_result(self.h, stream_ptr(), ctypes.byref(out), offs))
+        import torch
+
         res = []
         for i in range(out.ncols):
             c = out.cols[i]
-            data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
+            offsets = None
+            if c.type_id == LARGE_STRING:
+                data = _view_bytes(c.data, c.data_bytes)
+                offsets = _view_bytes(c.offsets, (c.length + 1) * 8).view(torch.int64)
+            else:
+                data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
             valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
-            res.append(DeviceColumn(c.type_id, c.length, data, valid, None, c.null_count))
+            res.append(DeviceColumn(c.type_id, c.length, data, valid, offsets, c.null_count))
         return res, np.frombuffer(offs, dtype=np.int64).copy()
 
     def config_used(self) -> dict:

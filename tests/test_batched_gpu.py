@@ -172,21 +172,107 @@ def test_batch_autotune_and_identical_results(gpu):
         assert _rows(cols, offs[i], offs[i + 1]) == _rows(C.join(L[i], R[i], [0], [0], 0))
 
 
+def _values(cols, lo=0, hi=None):
+    """Columns' rows [lo, hi) as lists of values (None for nulls, raw bits for floats)."""
+    out = []
+    for c in cols:
+        v, ok = C.column_to_numpy(c)
+        if isinstance(v, list):  # strings
+            out.append(list(v[lo:hi]))
+            continue
+        if ok is None:
+            ok = np.ones(len(v), bool)
+        if v.dtype.kind == "f":
+            v = v.view(np.int64 if v.dtype.itemsize == 8 else np.int32)
+        out.append([int(x) if o else None for x, o in zip(v[lo:hi], ok[lo:hi])])
+    return out
+
+
+def _rowset(cols, lo=0, hi=None):
+    """The rows as a sorted multiset (a probe row's matches come in no fixed order)."""
+    rows = list(zip(*_values(cols, lo, hi)))
+    return sorted(rows, key=lambda r: tuple((0, "") if x is None else (1, x) for x in r))
+
+
+def _strings(rng, n, nkinds):
+    words = [f"w{i:05d}" * (1 + i % 3) for i in range(nkinds)]
+    return [words[j] for j in rng.integers(0, nkinds, n)]
+
+
+@pytest.mark.parametrize("jt", [0, 1, 2, 3, 4], ids=["inner", "left", "full", "semi", "anti"])
+def test_batch_join_instance_by_instance_schemas(gpu, jt):
+    """Schemas the work-queue kernels do not take -- two key columns, string payloads -- run
+    instance by instance through the plain join: each instance's rows equal ldf_join's on that
+    instance alone (as multisets: a probe row's matches come in no fixed order), strings
+    included."""
+    rng = np.random.default_rng(40 + jt)
+    sizes = [(0, 0), (200, 0), (0, 50), (3000, 300), (17, 900), (1234, 123)]
+    L, R = [], []
+    for npr, nb in sizes:
+        kr = max(4, nb)
+        pv = rng.random(npr) > 0.1
+        L.append([_col(rng.integers(0, kr, npr), C.INT64, pv), _col(rng.integers(0, 3, npr), C.INT32),
+                  C.column_from_strings(_strings(rng, npr, 50))])
+        R.append([_col(rng.integers(0, kr, nb), C.INT64), _col(rng.integers(0, 3, nb), C.INT32),
+                  C.column_from_strings([None if x else s for s, x in zip(_strings(rng, nb, 70), rng.random(nb) < 0.2)]),
+                  _col(rng.random(nb), C.DOUBLE)])
+    rhs_out = [] if jt in (3, 4) else [2, 3]
+    b = C.batch_join(L, R, [0, 1], [0, 1], jt, lhs_out=[0, 2], rhs_out=rhs_out)
+    assert b.config_used()["use_graph"] == 0
+    for rep in range(2):
+        b.run()
+        cols, offs = b.result()
+        assert len(cols) == 2 + len(rhs_out) and offs[0] == 0
+        for i in range(len(sizes)):
+            plain = C.join(L[i], R[i], [0, 1], [0, 1], jt, lhs_out=[0, 2], rhs_out=rhs_out)
+            assert offs[i + 1] - offs[i] == plain[0].length, (rep, i)
+            assert _rowset(cols, offs[i], offs[i + 1]) == _rowset(plain), (rep, i)
+    b.close()
+
+
+def test_batch_groupby_instance_by_instance_schemas(gpu):
+    """A string key with COUNT_DISTINCT and VARIANCE (kinds the work-queue kernels do not take):
+    each instance's groups equal ldf_groupby's on that instance alone (as sorted rows; VARIANCE
+    compared within 1e-12, its summation order being free)."""
+    rng = np.random.default_rng(77)
+    tabs = []
+    for n, nk in [(0, 1), (1, 1), (5000, 40), (20000, 900), (777, 700)]:
+        keys = _strings(rng, n, nk)
+        tabs.append([C.column_from_strings(keys), _col(rng.integers(0, 50, n), C.INT32, rng.random(n) > 0.1),
+                     _col(rng.random(n), C.DOUBLE)])
+    aggs = [(1, "count_distinct", None), (2, "variance", None), (2, "sum", None), (None, "count_all", None)]
+    b = C.batch_groupby(tabs, [0], aggs)
+    for rep in range(2):
+        b.run()
+        cols, offs = b.result()
+        for i, t in enumerate(tabs):
+            plain = C.groupby(t, [0], aggs)
+            assert offs[i + 1] - offs[i] == plain[0].length, (rep, i)
+            got, exp = _rowset(cols, offs[i], offs[i + 1]), _rowset(plain)
+            assert [(r[0], r[1], r[3], r[4]) for r in got] == [(r[0], r[1], r[3], r[4]) for r in exp], (rep, i)
+            gv = np.array([np.nan if r[2] is None else np.int64(r[2]).view(np.float64) for r in got])
+            ev = np.array([np.nan if r[2] is None else np.int64(r[2]).view(np.float64) for r in exp])
+            np.testing.assert_allclose(gv, ev, rtol=1e-12, equal_nan=True)
+    b.close()
+
+
 def test_batch_errors(gpu):
     rng = np.random.default_rng(1)
     L, R = _join_instances(rng, [(10, 10)], nullable=False)
     with pytest.raises(ValueError):  # not a JoinType
         C.batch_join(L, R, [0], [0], 7, config=CFGS[0])
     S = [[C.column_from_strings(["a", "b"])]]
-    with pytest.raises(C.LdfError):  # the key must be fixed width
-        C.batch_join(S, S, [0], [0], 0, config=CFGS[0])
+    b = C.batch_join(S, S, [0], [0], 0, config=CFGS[0])  # a string key: instance by instance
+    b.run()
+    cols, offs = b.result()
+    assert list(offs) == [0, 2] and C.column_to_numpy(cols[0])[0] == ["a", "b"]
+    b.close()
     with pytest.raises(ValueError):
         C.batch_join(L, R, [0], [0], 0, config={"load_factor": 1.5, "tile_rows": 1024, "lds_slots": 512})
     with pytest.raises(ValueError):  # SEMI with rhs columns
         C.batch_join(L, R, [0], [0], 3, rhs_out=[1], config=CFGS[0])
     tabs = _gb_instances(rng, [(100, 10)])
-    with pytest.raises(C.LdfError):  # not a batched kind
-        C.batch_groupby(tabs, [0], [(1, "count_distinct", None)], config=CFGS[0])
+    C.batch_groupby(tabs, [0], [(1, "count_distinct", None)], config=CFGS[0]).close()  # instance by instance
     big = [[_col(np.zeros(4, np.int32), C.INT32), _col(np.array([1e300, 1.0, 2.0, 3.0]), C.DOUBLE)]]
     b = C.batch_groupby(big, [0], [(1, "sum", None)], config=CFGS[0])
     b.run()
