@@ -2938,10 +2938,6 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     const double keys_per_part = 1.25 * (double)est / nparts;
     q.S                        = std::max(1, (int)std::ceil(keys_per_part / (0.45 * q.tcap)));
     dense                      = (int64_t)nparts * q.S * q.tcap;
-    // The hash table then holds only the hot keys and the keys that spill from their partition's
-    // LDS table (rare at the planned load): room for a quarter of the estimate (a table that
-    // overflows is rebuilt four times larger, as on the other path).
-    cap = std::min(cap, next_pow2(std::max<uint64_t>(1024, (uint64_t)(2.5 * ((double)hot_cap + 0.25 * (double)est)))));
   }
 
   for (;;) {
