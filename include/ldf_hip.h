@@ -348,13 +348,15 @@ void ldf_batch_destroy(ldf_batch* batch);
  *                      table (rhs, its key column), nulls_equal = true; output = every probe
  *                      column then every table column, as ldf_join would return them;
  *   LDF_QUERY_GROUPBY  groupby of the table rows [row_begin, row_end) on the table's key column
- *                      with `aggs` (kinds of the batched form), as ldf_groupby would return it.
+ *                      with `aggs` (any kind ldf_groupby takes), as ldf_groupby would return it.
  * Continuous batching: `slots` worker threads, each with its own HIP stream, repeatedly take the
  * waiting queries -- the oldest one and, behind it, those of the same kind (and aggregations)
  * that still fit -- while the memory they are estimated to need fits in `hbm_budget_bytes`
  * (resident indexes and unreleased results count against it), and run them as one batch; the
- * batches of different slots overlap on the GPU.  A groupby over more than 2^21 rows runs
- * through ldf_groupby on the slot's stream.  ldf_server_wait blocks until a query finished and
+ * batches of different slots overlap on the GPU.  A groupby over more than 2^21 rows or with
+ * a kind the batched groupby does not take, and every query on a table with a string column
+ * (such a table gets no key index) or with a string probe column, runs alone through
+ * ldf_groupby / ldf_join on the slot's stream.  ldf_server_wait blocks until a query finished and
  * describes its result (buffers owned by the server until ldf_server_release).  Results are
  * those of the plain entry points on the same query (row order aside), alone or in any mix.
  * No counterpart in the reference. */

@@ -2045,6 +2045,7 @@ struct ServedTable {
   std::vector<ldf_column_view> cols;
   int32_t key_col = 0;
   int64_t rows    = 0;
+  bool fixed      = true;  // every column fixed-width: the table has a key index and batches
   std::unique_ptr<JoinBatch> index;  // instance 0 builds the table's key index
   JoinIndex idx;
   size_t bytes = 0;
@@ -2231,7 +2232,24 @@ struct Server {
     Alloc a(nullptr, s);
     std::vector<Buf<uint64_t>> keep;  // validity slices of unaligned row ranges
     Query& h = *batch.front();
-    if (h.plain) {  // large groupby: the plain entry point on this stream
+    if (h.plain && h.q.kind == LDF_QUERY_JOIN) {  // the plain join of the probe with the table
+      const ServedTable& t = *tables[h.q.table];
+      ldf_table_view lv{(int32_t)h.probe_cols.size(), h.probe_cols.data()};
+      ldf_table_view rv{(int32_t)t.cols.size(), t.cols.data()};
+      std::vector<int32_t> lo(h.probe_cols.size()), ro(t.cols.size());
+      for (size_t i = 0; i < lo.size(); ++i) lo[i] = (int32_t)i;
+      for (size_t i = 0; i < ro.size(); ++i) ro[i] = (int32_t)i;
+      h.cols.resize(lo.size() + ro.size());
+      ldf_table_out out{(int32_t)h.cols.size(), h.cols.data()};
+      const int rc = ldf_join(&lv, &rv, &h.q.probe_key, &t.key_col, 1, LDF_JOIN_INNER, 1, lo.data(), (int32_t)lo.size(),
+                              ro.data(), (int32_t)ro.size(), nullptr, s, &out);
+      if (rc != LDF_OK) throw Error(rc, ldf_last_error());
+      LDF_HIP_TRY(hipStreamSynchronize(s));
+      h.res_bytes = 0;
+      for (auto& c : h.cols) h.res_bytes += column_bytes(c);
+      return;
+    }
+    if (h.plain) {  // groupby: the plain entry point on this stream
       const ServedTable& t = *tables[h.q.table];
       std::vector<ldf_column_view> v = slice(t, h.q.row_begin, h.q.row_end, a, keep);
       ldf_table_view tv{(int32_t)v.size(), v.data()};
@@ -2317,6 +2335,7 @@ struct Server {
     for (auto& c : v) {
       const int w = type_width(c.type_id);
       c.data      = static_cast<const uint8_t*>(c.data) + lo * w;
+      if (w == 0) c.offsets += lo;  // strings: the rows' offsets, still into the whole character buffer
       if (c.validity && (lo & 63) == 0) {
         c.validity += lo / 8;
       } else if (c.validity) {
@@ -2605,7 +2624,7 @@ extern "C" int ldf_server_add_table(ldf_server* server, const ldf_table_view* ta
     t->rows    = table->ncols > 0 ? table->cols[0].length : 0;
     for (auto& c : t->cols) {
       validate_view(c, t->rows, "server table");
-      if (type_width(c.type_id) <= 0) throw Error(LDF_E_UNSUPPORTED, "server: table columns must be fixed-width");
+      if (type_width(c.type_id) <= 0) t->fixed = false;  // its queries run through the plain calls
     }
     {
       std::lock_guard<std::mutex> lk(S.mu);
@@ -2616,6 +2635,12 @@ extern "C" int ldf_server_add_table(ldf_server* server, const ldf_table_view* ta
         for (size_t c = 0; c < t->cols.size(); ++c)
           LDF_EXPECTS(t0.cols[c].type_id == t->cols[c].type_id, "server: every table has the schema of the first");
       }
+    }
+    if (!t->fixed) {  // no key index: joins with this table run through ldf_join
+      std::lock_guard<std::mutex> lk(S.mu);
+      *table_id = (int32_t)S.tables.size();
+      S.tables.push_back(std::move(t));
+      return;
     }
     // the key index: a join batch of one instance with an empty probe side
     hipStream_t s = S.streams[0];
@@ -2658,9 +2683,10 @@ extern "C" int ldf_server_submit(ldf_server* server, const ldf_query* query, int
       LDF_EXPECTS(query->probe_key >= 0 && query->probe_key < query->probe.ncols, "server: probe key out of range");
       q->probe_cols.assign(query->probe.cols, query->probe.cols + query->probe.ncols);
       const int64_t n = q->probe_cols[0].length;
+      q->plain        = !t.fixed;  // string columns: the plain join, alone
       for (auto& c : q->probe_cols) {
         validate_view(c, n, "server probe");
-        if (type_width(c.type_id) <= 0) throw Error(LDF_E_UNSUPPORTED, "server: probe columns must be fixed-width");
+        if (type_width(c.type_id) <= 0) q->plain = true;
       }
       LDF_EXPECTS(q->probe_cols[query->probe_key].type_id == t.cols[t.key_col].type_id,
                   "join: lhs and rhs key column types differ");
@@ -2680,14 +2706,16 @@ extern "C" int ldf_server_submit(ldf_server* server, const ldf_query* query, int
       LDF_EXPECTS(query->naggs >= 0 && query->naggs <= kMaxBAgg && (query->naggs == 0 || query->aggs != nullptr),
                   "server: 0..16 aggregations");
       q->aggs.assign(query->aggs, query->aggs + query->naggs);
+      // large row ranges, tables with string columns and the kinds the batched groupby does not
+      // take run through the plain groupby, alone
+      q->plain = (query->row_end - query->row_begin) > (int64_t)1 << 21 || !t.fixed;
       for (auto& g : q->aggs) {
         if (g.kind != LDF_AGG_SUM && g.kind != LDF_AGG_MEAN && g.kind != LDF_AGG_COUNT && g.kind != LDF_AGG_COUNT_ALL &&
             g.kind != LDF_AGG_MIN && g.kind != LDF_AGG_MAX)
-          throw Error(LDF_E_UNSUPPORTED, "server: groupby queries take SUM, MEAN, COUNT, COUNT_ALL, MIN, MAX");
+          q->plain = true;
         LDF_EXPECTS(g.kind == LDF_AGG_COUNT_ALL || (g.in_col >= 0 && g.in_col < (int32_t)t.cols.size()),
                     "groupby: aggregation column index out of range");
       }
-      q->plain = (query->row_end - query->row_begin) > (int64_t)1 << 21;
     } else {
       throw Error(LDF_E_INVALID, "server: unknown query kind");
     }

@@ -554,6 +554,20 @@ def _view_bytes(ptr: int, nbytes: int):
     return torch.as_tensor(_DevBuf(ptr, nbytes), device="cuda")
 
 
+def _out_view(c) -> DeviceColumn:
+    """A DeviceColumn viewing an output column whose buffers the library keeps owning."""
+    import torch
+
+    offsets = None
+    if c.type_id == LARGE_STRING:
+        data = _view_bytes(c.data, c.data_bytes)
+        offsets = _view_bytes(c.offsets, (c.length + 1) * 8).view(torch.int64)
+    else:
+        data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
+    valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
+    return DeviceColumn(c.type_id, c.length, data, valid, offsets, c.null_count)
+
+
 def _config(config) -> Optional[BatchConfig]:
     if config is None:
         return None
@@ -583,19 +597,7 @@ class Batch:
         out = TableOut(self.ncols, cols)
         offs = (ctypes.c_int64 * (self.ninst + 1))()
         check(load().ldf_batch_result(self.h, stream_ptr(), ctypes.byref(out), offs))
-        import torch
-
-        res = []
-        for i in range(out.ncols):
-            c = out.cols[i]
-            offsets = None
-            if c.type_id == LARGE_STRING:
-                data = _view_bytes(c.data, c.data_bytes)
-                offsets = _view_bytes(c.offsets, (c.length + 1) * 8).view(torch.int64)
-            else:
-                data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
-            valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
-            res.append(DeviceColumn(c.type_id, c.length, data, valid, offsets, c.null_count))
+        res = [_out_view(out.cols[i]) for i in range(out.ncols)]
         return res, np.frombuffer(offs, dtype=np.int64).copy()
 
     def config_used(self) -> dict:
@@ -737,12 +739,7 @@ class Server:
         out = TableOut(40, cols)
         st = QueryStats()
         check(_server_lib().ldf_server_wait(self.h, qid, ctypes.byref(out), ctypes.byref(st)))
-        res = []
-        for i in range(out.ncols):
-            c = out.cols[i]
-            data = _view_bytes(c.data, c.length * WIDTH[c.type_id])
-            valid = _view_bytes(c.validity, c.validity_bytes) if c.validity else None
-            res.append(DeviceColumn(c.type_id, c.length, data, valid, None, c.null_count))
+        res = [_out_view(out.cols[i]) for i in range(out.ncols)]
         stats = {"submit_ns": st.submit_ns, "start_ns": st.start_ns, "end_ns": st.end_ns,
                  "batch_queries": st.batch_queries, "plain": st.plain}
         return res, stats

@@ -146,8 +146,10 @@ def test_served_errors(gpu):
         srv.submit_groupby(5, 0, 10, [(1, "sum", None)])
     with pytest.raises(ValueError):  # rows outside the table
         srv.submit_groupby(0, 10, 1001, [(1, "sum", None)])
-    with pytest.raises(C.LdfError):  # not a served aggregation
-        srv.submit_groupby(0, 0, 100, [(1, "count_distinct", None)])
+    qid = srv.submit_groupby(0, 0, 100, [(1, "count_distinct", None)])  # a plain-only kind: alone
+    cols, st = srv.wait(qid)
+    assert st["plain"] == 1 and cols[0].length == len(np.unique(C.column_to_numpy(t[0])[0][:100]))
+    srv.release(qid)
     with pytest.raises(ValueError):  # schema differs from the first table
         srv.add_table([_col(rng.integers(0, 100, 10), C.INT32), _col(rng.random(10), C.DOUBLE)])
     qid = srv.submit_groupby(0, 0, 1000, [(1, "sum", None), (None, "count_all", None)])
@@ -178,5 +180,61 @@ def test_served_groupby_unaligned_row_ranges_with_nulls(gpu):
         plain = C.groupby([_col(k[b:e], C.INT64, kv[b:e]), _col(a[b:e], C.INT64, av[b:e]), _col(f[b:e], C.DOUBLE)],
                           [0], AGGS)
         assert _rows(cols) == _rows(plain), (b, e)
+        srv.release(qid)
+    srv.close()
+
+
+def _vals(cols):
+    """Rows as a sorted multiset; strings as str, floats as raw bits, None for nulls."""
+    data = []
+    for c in cols:
+        v, ok = C.column_to_numpy(c)
+        if isinstance(v, list):
+            data.append(list(v))
+            continue
+        if ok is None:
+            ok = np.ones(len(v), bool)
+        if v.dtype.kind == "f":
+            v = v.view(np.int64 if v.dtype.itemsize == 8 else np.int32)
+        data.append([int(x) if o else None for x, o in zip(v, ok)])
+    return sorted(zip(*data), key=lambda r: tuple((0, "") if x is None else (1, x) for x in r))
+
+
+def test_served_string_columns_and_plain_kinds(gpu):
+    """A table with a string key and a string-only schema for the probes: such tables get no key
+    index and their queries run alone through the plain calls -- joins of the probe with the
+    whole table, groupbys over row ranges that start anywhere (string offsets of a range point into
+    the table's characters) with COUNT_DISTINCT and MEAN.  Each result equals the plain call's."""
+    rng = np.random.default_rng(23)
+    n = 120_000
+    words = np.array([f"k{i:06d}" * (1 + i % 3) for i in range(6000)], dtype=object)
+    keys = list(words[rng.integers(0, 6000, n)])
+    a, av = rng.integers(-(2**40), 2**40, n), rng.random(n) > 0.1
+    f = rng.random(n)
+    table = [C.column_from_strings(keys), _col(a, C.INT64, av), _col(f, C.DOUBLE)]
+    srv = C.Server(hbm_budget_bytes=8e9, slots=2, max_batch=8)
+    srv.add_table(table)
+    aggs = [(1, "sum", None), (2, "mean", None), (1, "count_distinct", None), (None, "count_all", None)]
+    ranges = [(0, n), (1, 999), (77, 100_077), (64, 128)]
+    probes = []
+    for size in (500, 20_000):
+        pk = list(words[rng.integers(0, 8000, size) % 6000])
+        probes.append([C.column_from_strings(pk), _col(rng.integers(0, 9, size), C.INT32)])
+    ids = [srv.submit_groupby(0, b, e, aggs) for b, e in ranges] + [srv.submit_join(0, pr, 0) for pr in probes]
+    for qid, (b, e) in zip(ids, ranges):
+        cols, st = srv.wait(qid)
+        assert st["plain"] == 1
+        plain = C.groupby([C.column_from_strings(keys[b:e]), _col(a[b:e], C.INT64, av[b:e]), _col(f[b:e], C.DOUBLE)],
+                          [0], aggs)
+        got, exp = _vals(cols), _vals(plain)
+        assert [(r[0], r[1], r[3], r[4]) for r in got] == [(r[0], r[1], r[3], r[4]) for r in exp], (b, e)
+        gm = np.array([np.int64(r[2]).view(np.float64) for r in got])
+        em = np.array([np.int64(r[2]).view(np.float64) for r in exp])
+        np.testing.assert_allclose(gm, em, rtol=1e-12)
+        srv.release(qid)
+    for qid, pr in zip(ids[len(ranges):], probes):
+        cols, st = srv.wait(qid)
+        assert st["plain"] == 1
+        assert _vals(cols) == _vals(C.join(pr, table, [0], [0], 0))
         srv.release(qid)
     srv.close()
