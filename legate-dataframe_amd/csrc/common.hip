@@ -223,6 +223,29 @@ void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
   hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, s, p, v, n);
   LDF_HIP_TRY(hipGetLastError());
 }
+// several fills in one launch: blockIdx.y picks the job
+__global__ void k_fill_jobs(FillJobs f)
+{
+  const FillJob jb = f.j[blockIdx.y];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < jb.n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (jb.w == 8) static_cast<uint64_t*>(jb.p)[i] = jb.v;
+    else static_cast<uint32_t*>(jb.p)[i] = (uint32_t)jb.v;
+  }
+}
+void fill_many(const FillJob* jobs, int n, hipStream_t s)
+{
+  LDF_EXPECTS(n >= 0 && n <= kMaxFillJobs, "fill_many: too many jobs");
+  FillJobs f{};
+  int64_t most = 0;
+  for (int i = 0; i < n; ++i) {
+    LDF_EXPECTS(jobs[i].w == 4 || jobs[i].w == 8, "fill_many: 4- or 8-byte words");
+    f.j[f.n++] = jobs[i];
+    most       = std::max<int64_t>(most, jobs[i].n);
+  }
+  if (f.n == 0 || most <= 0) return;
+  hipLaunchKernelGGL(k_fill_jobs, dim3(grid_for(most, 256, 256 * 32), f.n), dim3(256), 0, s, f);
+  LDF_HIP_TRY(hipGetLastError());
+}
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s)
 {
   if (n <= 0) return;

@@ -2711,9 +2711,12 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
   Buf<unsigned long long> sslots(a, scap + 2);
   Buf<unsigned int> scnt(a, scap + 2);
   Buf<unsigned long long> smisc(a, 2 + 3 + 64 + 2);  // nkeys, overflow, stats
-  fill_u64((uint64_t*)sslots.get(), kGEmpty, scap + 2, s);
-  fill_u32(scnt.get(), 0, scap + 2, s);
-  fill_u64((uint64_t*)smisc.get(), 0, 2 + 3 + 64 + 2, s);
+  {
+    const FillJob jobs[3] = {{sslots.get(), kGEmpty, (int64_t)scap + 2, 8},
+                             {scnt.get(), 0, (int64_t)scap + 2, 4},
+                             {smisc.get(), 0, 2 + 3 + 64 + 2, 8}};
+    fill_many(jobs, 3, s);
+  }
   {
     KeyTab st{sslots.get(), scap - 1, scap, smisc.get(), reinterpret_cast<int32_t*>(smisc.get() + 1)};
     if (S > 0) {
@@ -2944,11 +2947,13 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
     LDF_EXPECTS(cap + 2 + dense < (uint64_t)INT32_MAX, "groupby: too many groups");
     const int64_t nslot = (int64_t)cap + 2 + dense;
     Buf<unsigned long long> slots(a, nslot);
-    fill_u64((uint64_t*)slots.get(), kGEmpty, nslot, s);
     // status: [0] nkeys, [1] overflow (int), [2.. 2+nacc) err flags (int), then null counts
     const int nstatus = 2 + nacc + nkeys + naggs + 1;
     Buf<unsigned long long> status(a, nstatus);
-    fill_u64((uint64_t*)status.get(), 0, nstatus, s);
+    {
+      const FillJob jobs[2] = {{slots.get(), kGEmpty, nslot, 8}, {status.get(), 0, nstatus, 8}};
+      fill_many(jobs, 2, s);
+    }
     KeyTab t{slots.get(), cap - 1, cap, status.get(), reinterpret_cast<int32_t*>(status.get() + 1)};
 
     // global accumulator arrays: one buffer, filled with one launch
@@ -3085,8 +3090,7 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
           return e == nullptr || e[0] != '0';
         }();
         DenseCodes dense{nullptr, 0, 0};
-        dcodes = Buf<uint16_t>(a, 2);
-        fill_u32(reinterpret_cast<uint32_t*>(dcodes.get()), 0, 1, s);
+        dcodes = Buf<uint16_t>(a, 2);  // without a table its one entry is read and never used
         if (pack && dense_on && smin <= smax && smax - smin < (uint64_t)kDenseMaxRange &&
             (int64_t)(smax - smin + 1) <= n / 4) {
           dense.kmin  = (uint32_t)smin;

@@ -268,6 +268,19 @@ void exclusive_scan_u64_ws(const uint64_t* in, uint64_t* out, int64_t n, uint64_
 void exclusive_scan_u32_to_u64_ws(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* ws, hipStream_t s);
 void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+// n fills (4- or 8-byte words, at most kMaxFillJobs) in one launch
+struct FillJob {
+  void* p;
+  uint64_t v;
+  int64_t n;
+  int32_t w;
+};
+constexpr int kMaxFillJobs = 8;
+struct FillJobs {
+  FillJob j[kMaxFillJobs];
+  int32_t n;
+};
+void fill_many(const FillJob* jobs, int n, hipStream_t s);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
 // Small device-to-host readbacks go through a per-thread pinned buffer (grown on demand and kept
 // for the thread's lifetime): copies into pinned memory skip the runtime's staging of pageable
