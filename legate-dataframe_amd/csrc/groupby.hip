@@ -3216,15 +3216,18 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         ProfScope prof("gb_aggregate", s);
         launch_main3(dpart, plan.ncol, dim3(nblk), main_lds, s, keys, n, plan, hot, pb, hp, (const uint16_t*)codes.get());
       }
-      hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
-      LDF_HIP_TRY(hipGetLastError());
-      // The hot pairs' COUNT_DISTINCT chain (split by pair partition, deduplicated, summed per hot
-      // key) reads only the hot pairs and writes only the hot keys' counts: it runs on a second
-      // stream beside the partition pass and the cold pairs' deduplication, which each leave much
-      // of the GPU idle (one or two blocks per CU, waits on memory).  Every buffer either chain
-      // uses lives until the end of this attempt, after the caller's stream has waited for both.
+      // The hot keys' partials (k_gb_hot_reduce) and the hot pairs' COUNT_DISTINCT chain (split by
+      // pair partition, deduplicated, summed per hot key) read only the main pass's hot output and
+      // write only the hot keys' slots: they run on a second stream beside the partition pass and
+      // the cold pairs' deduplication, which each leave much of the GPU idle (one or two blocks per
+      // CU, waits on memory).  Every buffer either chain uses lives until the end of this attempt,
+      // after the caller's stream has waited for both.
       SideStream& side = side_stream();
       bool forked      = false;
+      if (!(dpart && nhp > 0)) {
+        hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, s, plan, hot, hp, nblk);
+        LDF_HIP_TRY(hipGetLastError());
+      }
       if (dpart && nhp > 0) {
         hpoffs = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
         exclusive_scan_u32_to_u64(hpcnt.get(), hpoffs.get(), (int64_t)kHPairParts * nblk, a);
@@ -3232,6 +3235,8 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         LDF_HIP_TRY(hipEventRecord(side.fork, s));
         LDF_HIP_TRY(hipStreamWaitEvent(side.s, side.fork, 0));
         forked = true;
+        hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, side.s, plan, hot, hp, nblk);
+        LDF_HIP_TRY(hipGetLastError());
         ProfScope prof("gb_hot_distinct", side.s);
         const AccDev& d = plan.a[dacc];
         hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const uint32_t*)pb.hwords,
