@@ -3229,14 +3229,17 @@ void run_groupby(const ldf_table_view* tv, const int32_t* key_idx, int32_t nkeys
         LDF_HIP_TRY(hipGetLastError());
       }
       if (dpart && nhp > 0) {
-        hpoffs = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
-        exclusive_scan_u32_to_u64(hpcnt.get(), hpoffs.get(), (int64_t)kHPairParts * nblk, a);
+        hpoffs   = Buf<uint64_t>(a, (size_t)kHPairParts * nblk + 1);
         part_cnt = Buf<uint32_t>(a, (size_t)kHPairParts * hot_cap);
         LDF_HIP_TRY(hipEventRecord(side.fork, s));
         LDF_HIP_TRY(hipStreamWaitEvent(side.s, side.fork, 0));
         forked = true;
         hipLaunchKernelGGL(k_gb_hot_reduce, dim3(hot_cap), dim3(256), 0, side.s, plan, hot, hp, nblk);
         LDF_HIP_TRY(hipGetLastError());
+        {  // the pair partitions' cursors, scanned on the second stream (its scratch there too)
+          Alloc aside(a.raw(), side.s);
+          exclusive_scan_u32_to_u64(hpcnt.get(), hpoffs.get(), (int64_t)kHPairParts * nblk, aside);
+        }
         ProfScope prof("gb_hot_distinct", side.s);
         const AccDev& d = plan.a[dacc];
         hipLaunchKernelGGL(k_gb_pair_part, dim3(nblk), dim3(kHPairParts), 0, side.s, (const uint32_t*)pb.hwords,
